@@ -1,0 +1,133 @@
+/*
+ * dsdx_oracle.h -- CPU restatement of dsDecompX's Host communication path on a
+ * 1x1x1 periodic (self-loopback) process grid.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or as the timed CPU baseline).  The product path (dsdecompx_amd) never
+ * links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Arithmetic is
+ * plain IEEE double with FP contraction disabled (the Makefile passes
+ * -ffp-contract=off), matching the reference Host build (g++ -O3 on x86-64,
+ * no -march, hence no FMA).
+ *
+ * Parity pinning (see tests/test_oracle_*.py):
+ *   dsdxo_neighbors_build   closed-form slot/shift table, opp(opp(m)) == m,
+ *                           independent itertools enumeration
+ *   dsdxo_halo_build        exact-tie known answers (x == lo+w, x == hi-w),
+ *                           exact-rational geometric brute force of the ghost
+ *                           set, hand-derived ghost order, the reference
+ *                           examples' ghost counts (corner particle -> 7)
+ *   dsdxo_halo_write_ghosts owner-copy invariants, +0.0 shift, zero forces
+ *   dsdxo_reverse_exchange  order-sensitive known answers (catastrophic
+ *                           cancellation), exact integer-valued conservation
+ *                           against an independent bincount, Min/Max/Replace
+ *                           known answers, NaN/-0.0 std::min/max semantics
+ *   dsdxo_forward_exchange  exact copies, adjointness with reverse(Sum)
+ *   dsdxo_migrate_wrap      x=-1e-17 -> L (no fold-back), exact-rational
+ *                           congruence mod L, non-periodic axes untouched
+ */
+#ifndef DSDX_ORACLE_H
+#define DSDX_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSDXO_MAX_NEIGHBORS 26
+#define DSDXO_MAX_FIELDS 16
+
+/* Global box, halo width and neighbour policy (include/domain.hpp:11-30). */
+typedef struct {
+    double lower[3];
+    double upper[3];
+    int32_t periodic[3];
+    double width;
+    int32_t edges_and_corners;
+} dsdxo_box;
+
+/* Neighbour table of the 1x1x1 grid (include/topology.hpp:14-20). */
+typedef struct {
+    int32_t count;
+    int32_t direction[DSDXO_MAX_NEIGHBORS][3];
+    double shift[DSDXO_MAX_NEIGHBORS][3];
+    int32_t slot[DSDXO_MAX_NEIGHBORS];
+    int32_t active[DSDXO_MAX_NEIGHBORS]; /* rank >= 0 */
+} dsdxo_neighbors;
+
+/* Host halo plan (include/decomposition.hpp HaloPlanEntry): per neighbour
+ * index n the owned indices it sends, and per neighbour index the receive
+ * count.  recv_from[n] is the neighbour index whose send list fills receive
+ * block n (opposite slot, src/backends/mpi_backend.cpp:366-375). */
+typedef struct {
+    int64_t send_count[DSDXO_MAX_NEIGHBORS];
+    int32_t* source[DSDXO_MAX_NEIGHBORS];
+    int64_t send_capacity[DSDXO_MAX_NEIGHBORS];
+    int64_t recv_count[DSDXO_MAX_NEIGHBORS];
+    int32_t recv_from[DSDXO_MAX_NEIGHBORS];
+    int64_t ghost_total;
+} dsdxo_plan;
+
+/* SoA particle arrays; any of vx..type may be NULL (src/halo.cpp:44-53). */
+typedef struct {
+    double* x; double* y; double* z;
+    double* vx; double* vy; double* vz;
+    double* fx; double* fy; double* fz;
+    double* mass; double* charge;
+    int32_t* type; int32_t* global_id;
+} dsdxo_particles;
+
+/* FieldType / ReduceOp enumerators of include/field.hpp:22-36. */
+enum { DSDXO_INT32 = 0, DSDXO_INT64, DSDXO_UINT32, DSDXO_UINT64,
+       DSDXO_FLOAT32, DSDXO_FLOAT64 };
+enum { DSDXO_SUM = 0, DSDXO_MIN, DSDXO_MAX, DSDXO_REPLACE };
+
+/* One registered application field (include/field.hpp:74-84): AoS,
+ * `components` elements of `type` per particle. */
+typedef struct {
+    void* data;
+    int32_t type;
+    int32_t components;
+    int32_t op;
+} dsdxo_field;
+
+int dsdxo_direction_slot(int dx, int dy, int dz);
+int dsdxo_opposite_slot(int dx, int dy, int dz);
+void dsdxo_neighbors_build(const dsdxo_box* box, dsdxo_neighbors* out);
+
+int dsdxo_halo_build(const dsdxo_box* box, const dsdxo_neighbors* nb,
+                     int64_t owned, const double* x, const double* y,
+                     const double* z, dsdxo_plan* plan);
+void dsdxo_plan_free(dsdxo_plan* plan);
+
+/* Owner index of every ghost, in ghost-array order (length ghost_total). */
+void dsdxo_ghost_owners(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
+                        int32_t* owners);
+
+int dsdxo_halo_write_ghosts(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
+                            int64_t owned, const dsdxo_particles* p,
+                            const dsdxo_field* fields, int32_t field_count);
+
+int dsdxo_reverse_exchange(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
+                           int64_t owned, const dsdxo_field* fields,
+                           int32_t field_count);
+
+int dsdxo_reverse_halo(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
+                       int64_t owned, double* fx, double* fy, double* fz);
+
+int dsdxo_forward_exchange(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
+                           int64_t owned, const dsdxo_field* fields,
+                           int32_t field_count);
+
+void dsdxo_migrate_wrap(const dsdxo_box* box, int64_t owned, double* x,
+                        double* y, double* z);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,299 @@
+"""ctypes front end of the C oracle (oracle/dsdx_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+as the checker and by ``bench.py``'s cpu_baseline leg.  The product package
+``dsdecompx_amd`` never imports it.
+
+All arrays are numpy arrays on the host.  ``Particles`` mirrors the SoA layout
+of the reference's ``ParticleView`` (include/field.hpp:113-131): 11 float64
+arrays and 2 int32 arrays, each of length ``capacity`` >= owned + ghosts.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_dsdx.so")
+
+MAX_NEIGHBORS = 26
+
+# FieldType / ReduceOp of include/field.hpp:22-36
+INT32, INT64, UINT32, UINT64, FLOAT32, FLOAT64 = range(6)
+SUM, MIN, MAX, REPLACE = range(4)
+NUMPY_TYPE = {INT32: np.int32, INT64: np.int64, UINT32: np.uint32,
+              UINT64: np.uint64, FLOAT32: np.float32, FLOAT64: np.float64}
+FIELD_TYPE = {np.dtype(v): k for k, v in NUMPY_TYPE.items()}
+
+
+class _Box(C.Structure):
+    _fields_ = [("lower", C.c_double * 3), ("upper", C.c_double * 3),
+                ("periodic", C.c_int32 * 3), ("width", C.c_double),
+                ("edges_and_corners", C.c_int32)]
+
+
+class _Neighbors(C.Structure):
+    _fields_ = [("count", C.c_int32),
+                ("direction", (C.c_int32 * 3) * MAX_NEIGHBORS),
+                ("shift", (C.c_double * 3) * MAX_NEIGHBORS),
+                ("slot", C.c_int32 * MAX_NEIGHBORS),
+                ("active", C.c_int32 * MAX_NEIGHBORS)]
+
+
+class _Plan(C.Structure):
+    _fields_ = [("send_count", C.c_int64 * MAX_NEIGHBORS),
+                ("source", C.POINTER(C.c_int32) * MAX_NEIGHBORS),
+                ("send_capacity", C.c_int64 * MAX_NEIGHBORS),
+                ("recv_count", C.c_int64 * MAX_NEIGHBORS),
+                ("recv_from", C.c_int32 * MAX_NEIGHBORS),
+                ("ghost_total", C.c_int64)]
+
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int32)
+
+
+class _Particles(C.Structure):
+    _fields_ = [(n, _DP) for n in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy",
+                                   "fz", "mass", "charge")] + \
+               [("type", _IP), ("global_id", _IP)]
+
+
+class _Field(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("type", C.c_int32),
+                ("components", C.c_int32), ("op", C.c_int32)]
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.dsdxo_direction_slot.restype = C.c_int
+        L.dsdxo_opposite_slot.restype = C.c_int
+        L.dsdxo_neighbors_build.argtypes = [C.POINTER(_Box), C.POINTER(_Neighbors)]
+        L.dsdxo_halo_build.argtypes = [C.POINTER(_Box), C.POINTER(_Neighbors), C.c_int64,
+                                       _DP, _DP, _DP, C.POINTER(_Plan)]
+        L.dsdxo_halo_build.restype = C.c_int
+        L.dsdxo_plan_free.argtypes = [C.POINTER(_Plan)]
+        L.dsdxo_ghost_owners.argtypes = [C.POINTER(_Neighbors), C.POINTER(_Plan), _IP]
+        L.dsdxo_halo_write_ghosts.argtypes = [C.POINTER(_Neighbors), C.POINTER(_Plan),
+                                              C.c_int64, C.POINTER(_Particles),
+                                              C.POINTER(_Field), C.c_int32]
+        L.dsdxo_reverse_exchange.argtypes = [C.POINTER(_Neighbors), C.POINTER(_Plan),
+                                             C.c_int64, C.POINTER(_Field), C.c_int32]
+        L.dsdxo_reverse_exchange.restype = C.c_int
+        L.dsdxo_reverse_halo.argtypes = [C.POINTER(_Neighbors), C.POINTER(_Plan),
+                                         C.c_int64, _DP, _DP, _DP]
+        L.dsdxo_reverse_halo.restype = C.c_int
+        L.dsdxo_forward_exchange.argtypes = [C.POINTER(_Neighbors), C.POINTER(_Plan),
+                                             C.c_int64, C.POINTER(_Field), C.c_int32]
+        L.dsdxo_migrate_wrap.argtypes = [C.POINTER(_Box), C.c_int64, _DP, _DP, _DP]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_DP)
+
+
+def _ip(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_IP)
+
+
+@dataclass
+class Box:
+    """Global domain + halo configuration (include/domain.hpp:11-30)."""
+    lower: tuple = (0.0, 0.0, 0.0)
+    upper: tuple = (1.0, 1.0, 1.0)
+    periodic: tuple = (True, True, True)
+    width: float = 0.0
+    edges_and_corners: bool = True
+
+    def c(self) -> _Box:
+        b = _Box()
+        for a in range(3):
+            b.lower[a] = float(self.lower[a])
+            b.upper[a] = float(self.upper[a])
+            b.periodic[a] = 1 if self.periodic[a] else 0
+        b.width = float(self.width)
+        b.edges_and_corners = 1 if self.edges_and_corners else 0
+        return b
+
+
+@dataclass
+class Neighbors:
+    direction: np.ndarray
+    shift: np.ndarray
+    slot: np.ndarray
+    active: np.ndarray
+    raw: _Neighbors = field(repr=False)
+
+
+def neighbors(box: Box) -> Neighbors:
+    nb = _Neighbors()
+    lib().dsdxo_neighbors_build(C.byref(box.c()), C.byref(nb))
+    n = nb.count
+    return Neighbors(
+        direction=np.array([[nb.direction[i][a] for a in range(3)] for i in range(n)],
+                           dtype=np.int32).reshape(n, 3),
+        shift=np.array([[nb.shift[i][a] for a in range(3)] for i in range(n)],
+                       dtype=np.float64).reshape(n, 3),
+        slot=np.array([nb.slot[i] for i in range(n)], dtype=np.int32),
+        active=np.array([nb.active[i] for i in range(n)], dtype=bool),
+        raw=nb)
+
+
+class Particles:
+    """Host SoA storage of capacity ``capacity`` (owned first, then ghosts)."""
+    F64 = ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz", "mass", "charge")
+    I32 = ("type", "global_id")
+
+    def __init__(self, capacity: int):
+        self.capacity = int(capacity)
+        for n in self.F64:
+            setattr(self, n, np.zeros(self.capacity, np.float64))
+        for n in self.I32:
+            setattr(self, n, np.zeros(self.capacity, np.int32))
+
+    def grow(self, capacity: int, keep: int) -> None:
+        if capacity <= self.capacity:
+            return
+        for n in self.F64 + self.I32:
+            old = getattr(self, n)
+            new = np.zeros(capacity, old.dtype)
+            new[:keep] = old[:keep]
+            setattr(self, n, new)
+        self.capacity = capacity
+
+    def c(self, omit=()) -> _Particles:
+        p = _Particles()
+        for n in self.F64:
+            setattr(p, n, None if n in omit else _dp(getattr(self, n)))
+        for n in self.I32:
+            setattr(p, n, None if n in omit else _ip(getattr(self, n)))
+        return p
+
+
+class Field:
+    """A registered application field: AoS array [capacity, components]."""
+
+    def __init__(self, data: np.ndarray, op: int = SUM):
+        assert data.flags.c_contiguous
+        self.data = data
+        self.type = FIELD_TYPE[data.dtype]
+        self.components = 1 if data.ndim == 1 else int(np.prod(data.shape[1:]))
+        self.op = op
+
+    def c(self) -> _Field:
+        return _Field(self.data.ctypes.data, self.type, self.components, self.op)
+
+
+def _fields_c(fields):
+    fields = list(fields or [])
+    arr = (_Field * max(1, len(fields)))()
+    for i, f in enumerate(fields):
+        arr[i] = f.c()
+    return arr, len(fields)
+
+
+class Plan:
+    """The Host halo plan of the last build (src/halo.cpp:164-238)."""
+
+    def __init__(self):
+        self.raw = _Plan()
+
+    def __del__(self):
+        try:
+            lib().dsdxo_plan_free(C.byref(self.raw))
+        except Exception:
+            pass
+
+    @property
+    def ghost_total(self) -> int:
+        return int(self.raw.ghost_total)
+
+    def send_counts(self, nb: Neighbors) -> np.ndarray:
+        return np.array([self.raw.send_count[n] for n in range(len(nb.slot))], np.int64)
+
+    def recv_counts(self, nb: Neighbors) -> np.ndarray:
+        return np.array([self.raw.recv_count[n] for n in range(len(nb.slot))], np.int64)
+
+    def send_list(self, n: int) -> np.ndarray:
+        c = int(self.raw.send_count[n])
+        if c == 0:
+            return np.zeros(0, np.int32)
+        return np.ctypeslib.as_array(self.raw.source[n], shape=(c,)).copy()
+
+
+class Oracle:
+    """Host-path restatement bound to one box (a Decomp on a 1x1x1 grid)."""
+
+    def __init__(self, box: Box):
+        self.box = box
+        self._box = box.c()
+        self.nb = neighbors(box)
+        self.plan = Plan()
+
+    def migrate(self, p: Particles, owned: int) -> None:
+        lib().dsdxo_migrate_wrap(C.byref(self._box), owned, _dp(p.x), _dp(p.y), _dp(p.z))
+
+    def build(self, p: Particles, owned: int) -> int:
+        rc = lib().dsdxo_halo_build(C.byref(self._box), C.byref(self.nb.raw), owned,
+                                     _dp(p.x), _dp(p.y), _dp(p.z), C.byref(self.plan.raw))
+        if rc != 0:
+            raise MemoryError("oracle halo build failed")
+        return self.plan.ghost_total
+
+    def write_ghosts(self, p: Particles, owned: int, fields=(), omit=()) -> None:
+        arr, n = _fields_c(fields)
+        lib().dsdxo_halo_write_ghosts(C.byref(self.nb.raw), C.byref(self.plan.raw), owned,
+                                      C.byref(p.c(omit)), arr, n)
+
+    def exchange_halo(self, p: Particles, owned: int, fields=(), omit=()) -> int:
+        g = self.build(p, owned)
+        if owned + g > p.capacity:
+            raise ValueError("particle storage too small for the ghosts")
+        self.write_ghosts(p, owned, fields, omit)
+        return g
+
+    def ghost_owners(self) -> np.ndarray:
+        out = np.zeros(max(1, self.plan.ghost_total), np.int32)
+        lib().dsdxo_ghost_owners(C.byref(self.nb.raw), C.byref(self.plan.raw), _ip(out))
+        return out[:self.plan.ghost_total]
+
+    def reverse_halo(self, p: Particles, owned: int) -> None:
+        rc = lib().dsdxo_reverse_halo(C.byref(self.nb.raw), C.byref(self.plan.raw), owned,
+                                      _dp(p.fx), _dp(p.fy), _dp(p.fz))
+        if rc != 0:
+            raise RuntimeError("oracle reverse halo failed")
+
+    def reverse_exchange(self, owned: int, fields) -> None:
+        arr, n = _fields_c(fields)
+        if lib().dsdxo_reverse_exchange(C.byref(self.nb.raw), C.byref(self.plan.raw),
+                                        owned, arr, n) != 0:
+            raise RuntimeError("oracle reverse exchange failed")
+
+    def forward_exchange(self, owned: int, fields) -> None:
+        arr, n = _fields_c(fields)
+        lib().dsdxo_forward_exchange(C.byref(self.nb.raw), C.byref(self.plan.raw),
+                                     owned, arr, n)
