@@ -1,0 +1,140 @@
+"""ctypes binding of the C ABI in include/dsdx_hip.h.
+
+Loads ``dsdecompx_amd/lib/libdsdecompx_amd.so`` (built in-tree by
+``dsdecompx_amd.build``).  There is no fallback: if the library is missing
+the import fails with an error that says how to build it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import build as _build
+
+MAX_NEIGHBORS = 26
+MAX_FIELDS = 16
+
+OK, EINVAL, ECAPACITY, EDEVICE, ESTATE = 0, -1, -2, -3, -4
+INT32, INT64, UINT32, UINT64, FLOAT32, FLOAT64 = range(6)
+SUM, MIN, MAX, REPLACE = range(4)
+
+
+class Box(C.Structure):
+    _fields_ = [("lower", C.c_double * 3), ("upper", C.c_double * 3),
+                ("periodic", C.c_int32 * 3), ("width", C.c_double),
+                ("edges_and_corners", C.c_int32)]
+
+
+_VP = C.c_void_p
+
+
+class Particles(C.Structure):
+    _fields_ = [(n, _VP) for n in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy", "fz",
+                                   "mass", "charge", "type", "global_id")]
+
+
+class Field(C.Structure):
+    _fields_ = [("data", _VP), ("type", C.c_int32), ("components", C.c_int32),
+                ("op", C.c_int32)]
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+class CapacityError(DeviceError):
+    pass
+
+
+_lib = None
+
+
+def library_path() -> str:
+    return _build.LIB
+
+
+def lib():
+    """The loaded C-ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = _build.LIB
+    if not os.path.exists(path):
+        raise ImportError(
+            f"dsdecompx_amd: {path} is missing; build it with `python -m dsdecompx_amd.build`")
+    L = C.CDLL(path)
+    P = C.c_void_p
+    u64, i32, u32 = C.c_uint64, C.c_int32, C.c_uint32
+    sig = {
+        "dsdx_last_error": ([], C.c_char_p),
+        "dsdx_plan_create": ([C.POINTER(P)], C.c_int),
+        "dsdx_plan_destroy": ([P], C.c_int),
+        "dsdx_plan_reset": ([P], C.c_int),
+        "dsdx_plan_reserve": ([P, u64, u64], C.c_int),
+        "dsdx_migrate_wrap": ([C.POINTER(Box), P, P, P, u64, P], C.c_int),
+        "dsdx_halo_build": ([P, C.POINTER(Box), C.POINTER(Particles), C.POINTER(Field), u32,
+                             u64, u64, P, C.POINTER(u64)], C.c_int),
+        "dsdx_halo_scatter": ([P, C.POINTER(Particles), C.POINTER(Field), u32, u64, u64, P],
+                              C.c_int),
+        "dsdx_halo_refresh": ([P, C.POINTER(Box), C.POINTER(Particles), C.POINTER(Field), u32,
+                               u64, i32, P], C.c_int),
+        "dsdx_plan_query": ([P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u64),
+                             C.POINTER(u64), C.POINTER(i32), P], C.c_int),
+        "dsdx_plan_ghost_owners": ([P, P, P], C.c_int),
+        "dsdx_reverse_halo": ([P, P, P, P, u64, P], C.c_int),
+        "dsdx_reverse_exchange": ([P, C.POINTER(Field), u32, u64, P], C.c_int),
+        "dsdx_forward_exchange": ([P, C.POINTER(Field), u32, u64, P], C.c_int),
+        "dsdx_timing_enable": ([i32], C.c_int),
+        "dsdx_timing_reset": ([], C.c_int),
+        "dsdx_timing_read": ([C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names of the C-ABI functions this binding declares."""
+    return ["dsdx_last_error", "dsdx_plan_create", "dsdx_plan_destroy", "dsdx_plan_reset",
+            "dsdx_plan_reserve", "dsdx_migrate_wrap", "dsdx_halo_build", "dsdx_halo_scatter",
+            "dsdx_halo_refresh", "dsdx_plan_query", "dsdx_plan_ghost_owners",
+            "dsdx_reverse_halo", "dsdx_reverse_exchange", "dsdx_forward_exchange",
+            "dsdx_timing_enable", "dsdx_timing_reset", "dsdx_timing_read"]
+
+
+KERNELS = ("migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "reverse_halo",
+           "reverse_exchange", "forward_exchange", "halo_refresh", "ghost_owners")
+
+
+def timing_enable(on: bool = True) -> None:
+    lib().dsdx_timing_enable(1 if on else 0)
+
+
+def timing_reset() -> None:
+    check(lib().dsdx_timing_reset(), "timing reset")
+
+
+def timing_read() -> dict:
+    """{kernel: (total_ms, launches)} of the HIP-event kernel timers."""
+    out = {}
+    for k in KERNELS:
+        ms, n = C.c_double(), C.c_uint64()
+        check(lib().dsdx_timing_read(k.encode(), C.byref(ms), C.byref(n)), "timing read")
+        out[k] = (ms.value, n.value)
+    return out
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc == OK:
+        return
+    msg = lib().dsdx_last_error().decode()
+    if rc == ECAPACITY:
+        raise CapacityError(msg)
+    if rc == EINVAL:
+        raise ValueError(msg)
+    if rc == ESTATE:
+        raise RuntimeError(msg)
+    raise DeviceError(msg or f"{what} failed with {rc}")

@@ -1,0 +1,202 @@
+/*
+ * dsdx_hip.h -- C ABI of the MI355X (gfx950) particle-communication kernels.
+ *
+ * These entry points replace the reference's device operator layer
+ * (src/cuda/kernels.hpp:40-176, namespace dsdecompx::cuda) on a 1x1x1
+ * self-loopback grid: each of the 26 neighbour slots is this GPU, and the
+ * ghost block of receive slot m is filled directly from the owners in the
+ * send list of the opposite slot 25-m (the reference's self-neighbour rule,
+ * src/backends/mpi_backend.cpp:366-375), so no wire records and no transport
+ * are involved.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Particle arrays are device (or managed)
+ *     SoA arrays of length >= capacity; ghosts are written after the owned
+ *     particles, at [owned, owned + ghosts).
+ *   - `stream` is a hipStream_t passed as void* (the reference's
+ *     ExecutionStream, include/field.hpp:17).  All work is ordered on it.
+ *   - Every function returns DSDX_OK or a negative DSDX_E* code;
+ *     dsdx_last_error() returns the calling thread's message for the last
+ *     failure.  Nothing throws across this boundary.
+ *   - Results are bit-identical to the reference Host path
+ *     (src/halo.cpp, src/migration.cpp): strict FP64 slab compares
+ *     (x < lower+width, x > upper-width), the periodic wrap without fold-back,
+ *     ghost order = receive slot, then owned order, and reverse sums added
+ *     per owner in ascending send-slot order (no atomics).
+ */
+#ifndef DSDX_HIP_H
+#define DSDX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSDX_MAX_NEIGHBORS 26
+#define DSDX_MAX_FIELDS 16
+
+enum {
+    DSDX_OK = 0,
+    DSDX_EINVAL = -1,    /* bad argument (std::invalid_argument) */
+    DSDX_ECAPACITY = -2, /* particle storage too small for owned + ghosts */
+    DSDX_EDEVICE = -3,   /* HIP runtime / launch failure (std::runtime_error) */
+    DSDX_ESTATE = -4     /* no plan, or plan does not match (std::logic_error) */
+};
+
+/* FieldType and ReduceOp, same enumerator values as include/field.hpp:22-36. */
+enum { DSDX_INT32 = 0, DSDX_INT64, DSDX_UINT32, DSDX_UINT64, DSDX_FLOAT32,
+       DSDX_FLOAT64 };
+enum { DSDX_SUM = 0, DSDX_MIN, DSDX_MAX, DSDX_REPLACE };
+
+/* Device SoA particle storage (ParticleView, include/field.hpp:113-131). */
+typedef struct dsdx_particles {
+    double* x; double* y; double* z;
+    double* vx; double* vy; double* vz;
+    double* fx; double* fy; double* fz;
+    double* mass; double* charge;
+    int32_t* type; int32_t* global_id;
+} dsdx_particles;
+
+/* One registered application field (ParticleFieldView,
+ * include/field.hpp:74-84): AoS, `components` elements of `type` per
+ * particle.  `op` is only read by dsdx_reverse_exchange. */
+typedef struct dsdx_field {
+    void* data;
+    int32_t type;
+    int32_t components;
+    int32_t op;
+} dsdx_field;
+
+/* Global domain and halo settings (GlobalDomain + HaloConfig,
+ * include/domain.hpp:11-30).  On the 1x1x1 grid the local domain is the
+ * global one. */
+typedef struct dsdx_box {
+    double lower[3];
+    double upper[3];
+    int32_t periodic[3];
+    double width;
+    int32_t edges_and_corners;
+} dsdx_box;
+
+/* Device-resident halo plan: per-chunk slab classification, per-slot send
+ * counts, the owner -> ghost map used by reverse/forward exchanges.
+ * Replaces the device plan state of src/internal/decomp_impl.hpp:58-106 and
+ * the source_indices buffer of src/cuda/execution.hpp:14-52. */
+typedef struct dsdx_plan dsdx_plan;
+
+const char* dsdx_last_error(void);
+
+int dsdx_plan_create(dsdx_plan** plan);
+int dsdx_plan_destroy(dsdx_plan* plan);
+
+/* Drops the plan (the device halo state cleared by a migration,
+ * src/migration.cpp:355-359): reverse and forward exchanges become no-ops. */
+int dsdx_plan_reset(dsdx_plan* plan);
+
+/* Pre-sizes the plan's buffers for `owned` particles and `ghosts` ghosts, so
+ * that a following dsdx_halo_build with a null `ghosts` pointer (e.g. inside
+ * a captured hipGraph) allocates nothing. */
+int dsdx_plan_reserve(dsdx_plan* plan, uint64_t owned, uint64_t ghosts);
+
+/* Periodic wrap of the owned positions in place; replaces
+ * launch_classify_destinations + launch_build_migration_layout + the unpack
+ * (src/cuda/kernels.hpp:152-176) at P = 1, where every particle is retained
+ * in order (src/migration.cpp:102-116, :437-479).  Non-periodic axes are left
+ * unchanged. */
+int dsdx_migrate_wrap(const dsdx_box* box, double* x, double* y, double* z,
+                      uint64_t owned, void* stream);
+
+/* Halo rebuild: slab classification of the owned particles, per-slot stable
+ * compaction and the ghost write, fused for the loopback grid.  Replaces
+ * launch_halo_classify + launch_halo_scatter + launch_unpack_records
+ * (src/cuda/kernels.hpp:78-96, :132-137) and the host wire path
+ * (src/halo.cpp:164-238).  Ghosts receive the owner's fields with the
+ * periodic shift added to x, y, z and zero forces; every registered field is
+ * copied.  All 13 particle arrays must be non-null (src/halo.cpp:369-376)
+ * except fx, fy, fz, which are zeroed only when present.
+ *
+ * If `ghosts` is non-null the call waits for the ghost count and stores it
+ * there (the reference's count readback, src/halo.cpp:482-491); when
+ * owned + ghosts > capacity it returns DSDX_ECAPACITY without writing any
+ * ghost, and dsdx_halo_scatter may be called with larger storage.  If
+ * `ghosts` is null nothing waits; an overflow is recorded in the plan and
+ * reported by dsdx_plan_query. */
+int dsdx_halo_build(dsdx_plan* plan, const dsdx_box* box,
+                    const dsdx_particles* particles, const dsdx_field* fields,
+                    uint32_t field_count, uint64_t owned, uint64_t capacity,
+                    void* stream, uint64_t* ghosts);
+
+/* Writes the ghosts of the plan's last classification (after a
+ * DSDX_ECAPACITY, with storage of at least owned + ghosts). */
+int dsdx_halo_scatter(dsdx_plan* plan, const dsdx_particles* particles,
+                      const dsdx_field* fields, uint32_t field_count,
+                      uint64_t owned, uint64_t capacity, void* stream);
+
+/* Reused-plan refresh: every ghost re-reads its owner along the existing
+ * plan (launch_halo_gather / launch_halo_gather_positions +
+ * launch_unpack_positions, src/cuda/kernels.hpp:103-128).  With
+ * positions_only only x, y, z are rewritten and forces zeroed; otherwise the
+ * whole record and the registered fields are rewritten.  A non-null `box`
+ * replaces the periodic shifts of the plan (Decomp::rescale_domain keeps the
+ * ghost list but moves the images, src/decomp.cpp:109-144); null keeps the
+ * shifts of the build. */
+int dsdx_halo_refresh(dsdx_plan* plan, const dsdx_box* box,
+                      const dsdx_particles* particles,
+                      const dsdx_field* fields, uint32_t field_count,
+                      uint64_t owned, int32_t positions_only, void* stream);
+
+/* Waits for the plan's counts: per-slot send counts (send_counts[s] for
+ * slot s), receive counts (receive slot m receives send_counts[25-m]), total
+ * ghosts and distinct owners, and the overflow flag of an unwaited build.
+ * Any output pointer may be null.  With a null stream nothing waits: the
+ * counts read back by the last waited dsdx_halo_build are returned. */
+int dsdx_plan_query(const dsdx_plan* plan, uint32_t* send_counts,
+                    uint32_t* receive_counts, uint64_t* ghosts,
+                    uint64_t* owners, int32_t* overflow, void* stream);
+
+/* Owner (owned index) of every ghost, in ghost order: the reference's
+ * halo_source_indices permuted into receive order (src/cuda/pack.cu:172-177).
+ * `owners` is a device array of at least `ghosts` int32. */
+int dsdx_plan_ghost_owners(const dsdx_plan* plan, int32_t* owners,
+                           void* stream);
+
+/* reverse_halo: fx, fy, fz of every ghost added to its owner, per owner in
+ * ascending send-slot order, deterministically (Host order of
+ * src/halo.cpp:1351-1372 via :1336-1346).  Replaces
+ * launch_pack_reverse_records + launch_accumulate_reverse_records
+ * (src/cuda/kernels.hpp:139-150), whose FP64 atomics have no fixed order. */
+int dsdx_reverse_halo(const dsdx_plan* plan, double* fx, double* fy,
+                      double* fz, uint64_t owned, void* stream);
+
+/* reverse_exchange of registered fields with Sum / Min / Max / Replace, in
+ * the same deterministic order (src/halo.cpp:1396-1532).  Replaces
+ * launch_pack_reverse_fields + launch_reduce_reverse_fields
+ * (src/cuda/kernels.hpp:55-64). */
+int dsdx_reverse_exchange(const dsdx_plan* plan, const dsdx_field* fields,
+                          uint32_t field_count, uint64_t owned, void* stream);
+
+/* forward_exchange: owner values copied into every ghost slot that holds a
+ * copy of the owner.  Replaces launch_gather_fields + launch_scatter_fields
+ * (src/cuda/kernels.hpp:69-72, src/halo.cpp:1084-1207). */
+int dsdx_forward_exchange(const dsdx_plan* plan, const dsdx_field* fields,
+                          uint32_t field_count, uint64_t owned, void* stream);
+
+/* Per-kernel device timing, process-wide: when enabled, every kernel launch
+ * is bracketed by two HIP events on its stream.  dsdx_timing_read waits for
+ * the recorded events and returns the accumulated milliseconds and launch
+ * count of one kernel: "migrate_wrap", "halo_classify", "halo_scan",
+ * "halo_scatter", "reverse_halo", "reverse_exchange", "forward_exchange",
+ * "halo_refresh" or "ghost_owners".  The GPU-event counterpart of the
+ * reference's host-clock timings (include/decomposition.hpp:30-38,
+ * src/halo.cpp:316-323). */
+int dsdx_timing_enable(int32_t enable);
+int dsdx_timing_reset(void);
+int dsdx_timing_read(const char* kernel, double* total_ms, uint64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

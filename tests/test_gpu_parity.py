@@ -1,0 +1,261 @@
+"""GPU parity of the HIP path (through the C ABI) against the CPU oracle on
+identical seeded inputs: ghost lists, positions and every particle array
+bit-exact; reverse sums bit-exact (same per-owner order); migrate wrap
+bit-exact."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.gpu_util import (F64, I32, assert_bits_equal, compare_particles, host, say, synth,
+                            to_device, to_oracle)
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # name, n, L, width, periodic, edges, lower
+    ("empty", 0, 8.0, 1.0, (True, True, True), True, 0.0),
+    ("one", 1, 8.0, 1.0, (True, True, True), True, 0.0),
+    ("small", 1000, 10.0, 1.5, (True, True, True), True, 0.0),
+    ("ragged_chunk", 4099, 16.0, 2.8, (True, True, True), True, 0.0),
+    ("cfg1", 32768, 32.0, 2.8, (True, True, True), True, 0.0),
+    ("z_nonperiodic", 20000, 27.0, 2.8, (True, True, False), True, 0.0),
+    ("faces_only", 20000, 27.0, 2.8, (True, True, True), False, 0.0),
+    ("shifted_box", 15000, 24.6, 2.8, (True, True, True), True, -7.3),
+    ("wide_halo", 3000, 5.0, 3.1, (True, True, True), True, 0.0),
+    ("dense_small", 2048, 11.95, 2.8, (True, True, True), True, 0.0),
+]
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dsdecompx_amd import _abi
+    _abi.lib()
+    # the device path requires an explicit (non-null) stream, like the reference
+    torch.cuda.set_stream(torch.cuda.Stream())
+    return torch
+
+
+def _run_build(case, seed=1, fields=False):
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    name, n, L, w, per, edges, lower = case
+    upper = tuple(lower + L for _ in range(3))
+    box_o = po.Box(lower=(lower,) * 3, upper=upper, periodic=per, width=w, edges_and_corners=edges)
+    box_d = BoxConfig(lower=(lower,) * 3, upper=upper, periodic=per, width=w,
+                      edges_and_corners=edges)
+    d = synth(n, L, seed, lower=lower)
+    cap = max(64, n * 27 + 1) if n < 5000 else n * 3
+    ora = po.Oracle(box_o)
+    op = to_oracle(d, cap)
+    g_ref = ora.exchange_halo(op, n)
+    dev = to_device(d, max(1, n + 16))   # deliberately small: forces the regrow path
+    halo = LoopbackHalo(box_d)
+    g = halo.exchange_halo(dev, n)
+    return ora, op, g_ref, halo, dev, g, n
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_halo_build_bit_exact(gpu, case):
+    say("halo build", case)
+    ora, op, g_ref, halo, dev, g, n = _run_build(case)
+    assert g == g_ref
+    compare_particles(dev, op, n + g, case[0])
+    send, recv, g2, owners, ov = halo.query()
+    nb = ora.nb
+    ref_send = dict(zip(nb.slot.tolist(), ora.plan.send_counts(nb).tolist()))
+    assert [send[s] for s in range(26)] == [ref_send.get(s, 0) for s in range(26)]
+    assert recv == [send[25 - m] for m in range(26)]
+    assert g2 == g and ov == 0
+    own = host(halo.ghost_owners(g))
+    assert_bits_equal(own, ora.ghost_owners(), "ghost owners")
+    assert owners == len(np.unique(own))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_reverse_halo_bit_exact(gpu, case):
+    import torch
+    say("reverse halo", case)
+    ora, op, g_ref, halo, dev, g, n = _run_build(case, seed=2)
+    rng = np.random.default_rng(5)
+    for k in ("fx", "fy", "fz"):
+        vals = rng.uniform(-1, 1, n + g)
+        getattr(op, k)[:n + g] = vals
+        getattr(dev, k)[:n + g] = torch.from_numpy(vals).to(dev.device)
+    ora.reverse_halo(op, n)
+    halo.reverse_halo(dev, n)
+    for k in ("fx", "fy", "fz"):
+        assert_bits_equal(host(getattr(dev, k)[:n + g]), getattr(op, k)[:n + g], k)
+
+
+def test_reverse_order_known_answer(gpu):
+    """Catastrophic cancellation makes only ascending send-slot order give 4.75."""
+    import torch
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    say("reverse order KAT")
+    d = synth(1, 10.0, 0)
+    d["x"][:], d["y"][:], d["z"][:] = 0.2, 0.3, 0.4
+    dev = to_device(d, 16)
+    halo = LoopbackHalo(BoxConfig(upper=(10.0,) * 3, width=1.0))
+    assert halo.exchange_halo(dev, 1) == 7
+    blocks = [13, 15, 16, 21, 22, 24, 25]
+    value = {0: 1e16, 1: 1.0, 3: -1e16, 4: 1.0, 9: 0.5, 10: 0.25, 12: 3.0}
+    fx = np.zeros(8)
+    fx[0] = 1.0
+    for s, v in value.items():
+        fx[1 + blocks.index(25 - s)] = v
+    dev.fx[:8] = torch.from_numpy(fx).cuda()
+    halo.reverse_halo(dev, 1)
+    assert host(dev.fx[:1])[0] == 4.75
+
+
+def test_migrate_wrap_bit_exact(gpu):
+    import torch
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    say("migrate wrap")
+    rng = np.random.default_rng(17)
+    for lower, L, per in [(0.0, 32.0, (True, True, True)), (-3.7, 15.8, (True, False, True)),
+                          (0.0, 256.0, (True, True, False))]:
+        n = 200000
+        x = rng.uniform(lower - 3 * L, lower + 4 * L, n)
+        x[:8] = [-1e-17, lower + L, lower, -0.0, lower + L + 1e-12, np.nextafter(lower, -1),
+                 np.inf, np.nan]
+        y = np.roll(x, 3)
+        z = np.roll(x, 7)
+        box_o = po.Box(lower=(lower,) * 3, upper=(lower + L,) * 3, periodic=per)
+        ox, oy, oz = x.copy(), y.copy(), z.copy()
+        po.lib().dsdxo_migrate_wrap(__import__("ctypes").byref(box_o.c()), n, po._dp(ox),
+                                    po._dp(oy), po._dp(oz))
+        d = synth(n, L, 1)
+        d["x"], d["y"], d["z"] = x, y, z
+        dev = to_device(d, n)
+        halo = LoopbackHalo(BoxConfig(lower=(lower,) * 3, upper=(lower + L,) * 3, periodic=per))
+        halo.migrate(dev, n)
+        for k, ref in (("x", ox), ("y", oy), ("z", oz)):
+            got = host(getattr(dev, k)[:n])
+            nan = np.isnan(ref)
+            assert np.array_equal(np.isnan(got), nan)
+            assert_bits_equal(got[~nan], ref[~nan], f"wrap {k} lower={lower}")
+
+
+FIELD_TYPES = [np.int32, np.int64, np.uint32, np.uint64, np.float32, np.float64]
+
+
+def test_fields_forward_and_reverse(gpu):
+    import torch
+    from dsdecompx_amd import _abi
+    say("fields")
+    case = ("fields", 6000, 14.0, 2.8, (True, True, True), True, 0.0)
+    ora, op, g_ref, halo, dev, g, n = _run_build(case, seed=3)
+    rng = np.random.default_rng(8)
+    for dt in FIELD_TYPES:
+        for comps in (1, 3):
+            for op_code in (_abi.SUM, _abi.MIN, _abi.MAX, _abi.REPLACE):
+                shape = (n + g,) if comps == 1 else (n + g, comps)
+                if np.issubdtype(dt, np.integer):
+                    info = np.iinfo(dt)
+                    vals = rng.integers(max(info.min, -2**40), min(info.max, 2**40), shape,
+                                        dtype=np.int64 if dt != np.uint64 else np.uint64).astype(dt)
+                else:
+                    vals = rng.standard_normal(shape).astype(dt)
+                ref = vals.copy()
+                ora.reverse_exchange(n, [po.Field(ref, op_code)])
+                t = torch.from_numpy(vals.copy()).cuda()
+                halo.reverse_exchange([(t, op_code)], n)
+                assert_bits_equal(host(t).reshape(-1), ref.reshape(-1),
+                                  f"reverse {np.dtype(dt).name}x{comps} op{op_code}")
+            # forward: ghosts get exact owner copies, owners untouched
+            shape = (n + g,) if comps == 1 else (n + g, comps)
+            vals = (rng.integers(0, 1000, shape)).astype(dt)
+            ref = vals.copy()
+            ora.forward_exchange(n, [po.Field(ref)])
+            t = torch.from_numpy(vals.copy()).cuda()
+            halo.forward_exchange([t], n)
+            assert_bits_equal(host(t).reshape(-1), ref.reshape(-1), f"forward {np.dtype(dt).name}")
+
+
+def test_fields_travel_with_halo_build(gpu):
+    """Registered fields are copied into the ghosts by the rebuild itself
+    (src/halo.cpp:120-147, :874-890)."""
+    import torch
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    say("fields in halo build")
+    n, L = 5000, 13.0
+    d = synth(n, L, 4)
+    rng = np.random.default_rng(4)
+    dens = np.zeros(n * 3)
+    dens[:n] = rng.uniform(0, 2, n)
+    vec = np.zeros((n * 3, 3))
+    vec[:n] = rng.uniform(-1, 1, (n, 3))
+    box_o = po.Box(upper=(L,) * 3, width=2.8)
+    ora = po.Oracle(box_o)
+    op = to_oracle(d, n * 3)
+    rd, rv = dens.copy(), vec.copy()
+    g = ora.exchange_halo(op, n, fields=[po.Field(rd), po.Field(rv)])
+    dev = to_device(d, n * 3)
+    td, tv = torch.from_numpy(dens.copy()).cuda(), torch.from_numpy(vec.copy()).cuda()
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=2.8))
+    assert halo.exchange_halo(dev, n, fields=[td, tv]) == g
+    assert_bits_equal(host(td)[:n + g], rd[:n + g], "density")
+    assert_bits_equal(host(tv)[:n + g].reshape(-1), rv[:n + g].reshape(-1), "vec3")
+
+
+def test_refresh_matches_rewrite(gpu):
+    """A reused plan re-reads every owner (positions_only or the whole
+    record); the oracle rewrites the ghosts of its stored plan the same way."""
+    import torch
+    say("refresh")
+    case = ("refresh", 8000, 15.0, 2.8, (True, True, True), True, 0.0)
+    ora, op, g_ref, halo, dev, g, n = _run_build(case, seed=6)
+    rng = np.random.default_rng(6)
+    for full in (False, True):
+        dx = rng.normal(0, 0.05, (3, n))
+        for a, k in enumerate(("x", "y", "z")):
+            getattr(op, k)[:n] += dx[a]
+            getattr(dev, k)[:n] = torch.from_numpy(getattr(op, k)[:n].copy()).cuda()
+        if full:
+            op.vx[:n] += 1.0
+            dev.vx[:n] = torch.from_numpy(op.vx[:n].copy()).cuda()
+        before = {k: host(getattr(dev, k)[:n + g]).copy() for k in F64 + I32}
+        halo.refresh(dev, n, positions_only=not full)
+        ref = to_oracle({k: getattr(op, k)[:n].copy() for k in F64 + I32}, n + g)
+        for k in F64 + I32:
+            getattr(ref, k)[n:n + g] = before[k][n:]
+        ora.write_ghosts(ref, n)
+        if not full:   # positions_only: only x, y, z and zero forces change
+            for k in ("vx", "vy", "vz", "mass", "charge", "type", "global_id"):
+                getattr(ref, k)[n:n + g] = before[k][n:]
+        compare_particles(dev, ref, n + g, f"refresh full={full}")
+
+
+def test_multi_step_cfg1(gpu):
+    """cfg1 loop: displace N(0,0.05), migrate, exchange_halo, forces U[-1,1]
+    on owned+ghosts, reverse_halo; every array bit-exact after every step."""
+    import torch
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    n, L, w = 32768, 32.0, 2.8
+    d = synth(n, L, 100)
+    box_o = po.Box(upper=(L,) * 3, width=w)
+    ora = po.Oracle(box_o)
+    op = to_oracle(d, n * 3)
+    dev = to_device(d, n * 3)
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
+    rng = np.random.default_rng(1234)
+    for step in range(10):
+        say("cfg1 step", step)
+        disp = rng.normal(0, 0.05, (3, n))
+        for a, k in enumerate(("x", "y", "z")):
+            getattr(op, k)[:n] += disp[a]
+            getattr(dev, k)[:n] += torch.from_numpy(disp[a]).cuda()
+        ora.migrate(op, n)
+        halo.migrate(dev, n)
+        g = ora.exchange_halo(op, n)
+        assert halo.exchange_halo(dev, n) == g
+        f = rng.uniform(-1, 1, (3, n + g))
+        for a, k in enumerate(("fx", "fy", "fz")):
+            getattr(op, k)[:n + g] = f[a]
+            getattr(dev, k)[:n + g] = torch.from_numpy(f[a]).cuda()
+        ora.reverse_halo(op, n)
+        halo.reverse_halo(dev, n)
+        compare_particles(dev, op, n + g, f"step {step}")
