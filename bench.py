@@ -91,21 +91,18 @@ def cpu_baseline(name, n, L, w, sigma, periodic, steps, seed=7):
                       f"oracle (C, -O2, one thread) on {os.cpu_count()} visible host cores"}
 
 
-def run(args):
+def measure_abi(args, n, L, w, sigma, periodic):
+    """The hot path through the C ABI from Python (ctypes), torch tensors as
+    device storage."""
     import torch
     from dsdecompx_amd import _abi
     from dsdecompx_amd.halo import BoxConfig, DeviceParticles, LoopbackHalo
 
-    name = args.config
-    n, L, w, sigma, periodic = WORKLOADS[name]
-    if args.particles:
-        n = args.particles
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     torch.cuda.set_stream(torch.cuda.Stream(dev))   # the path requires a non-null stream
     gen = torch.Generator(device=dev)
     gen.manual_seed(args.seed)
-
     cap = int(n * 1.12) + 4096
     p = DeviceParticles(cap, dev)
     for k in ("x", "y", "z"):
@@ -141,7 +138,6 @@ def run(args):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-
     _abi.timing_reset()
     _abi.timing_enable(True)
     timers = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
@@ -149,32 +145,170 @@ def run(args):
     torch.cuda.synchronize()
     t_wall = time.perf_counter()
     for s in range(args.steps):
-        g = step(timers[s])
-        ghosts.append(g)
+        ghosts.append(step(timers[s]))
         owners.append(_owners(halo))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
     _abi.timing_enable(False)
     kern = _abi.timing_read()
-
     hot_ms = sum(t[0].elapsed_time(t[1]) + t[2].elapsed_time(t[3]) for t in timers)
-    ms = hot_ms / args.steps
-    G = sum(ghosts) / len(ghosts)
-    U = sum(owners) / len(owners)
+    return {"hot_ms_per_step": hot_ms / args.steps, "wall_ms_per_step": 1e3 * wall / args.steps,
+            "ghosts_per_step": sum(ghosts) / len(ghosts), "owners": sum(owners) / len(owners),
+            "kernels": {k: {"total_ms": v[0], "launches": v[1]} for k, v in kern.items()},
+            "path": "C ABI from Python (dsdx_migrate_wrap, dsdx_halo_build, dsdx_reverse_halo)"}
+
+
+def measure_cpp(args, n, L, w, sigma, periodic):
+    """The hot path through the drop-in C++ API (dsdecompx::Decomp), driven
+    by the compiled driver dsdecompx_amd/bin/comm_bench."""
+    import subprocess
+    exe = os.path.join(ROOT, "dsdecompx_amd", "bin", "comm_bench")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    cmd = [exe, str(n), repr(L), repr(w), repr(sigma), "1" if periodic[2] else "0",
+           str(args.steps), str(args.warmup), str(args.seed)]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        raise SystemExit(f"comm_bench failed ({r.returncode}): {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    res["path"] = "C++ API (dsdecompx::Decomp migrate / exchange_halo / reverse_halo)"
+    return res
+
+
+def measure_batched(args):
+    """cfg5: 512 replica boxes, particle counts log-uniform in [2048, 262144],
+    density U[0.6, 1.2], w = 2.8, displacement N(0, 0.1).  The same steps
+    (application displacement + migrate + exchange_halo + forces +
+    reverse_halo) run as the plain per-box loop and as the batched form
+    (eager, and captured in one hipGraph); replica-steps/s of each, and the
+    final states of loop and graph compared bit for bit."""
+    import numpy as np
+    import torch
+    from dsdecompx_amd import _abi
+    from dsdecompx_amd.batch import LoopedReplicas, ReplicaBatch, ReplicaSpec, capture_step
+    from dsdecompx_amd.halo import BoxConfig
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    torch.cuda.set_stream(torch.cuda.Stream(dev))
+    _abi.timing_enable(False)
+    rng = np.random.default_rng(args.seed + 5)
+    R = args.replicas
+    n = np.exp(rng.uniform(np.log(2048), np.log(262144), R)).astype(int)
+    rho = rng.uniform(0.6, 1.2, R)
+    specs = []
+    for k in range(R):
+        L = float((n[k] / rho[k]) ** (1.0 / 3.0))
+        specs.append(ReplicaSpec(int(n[k]), BoxConfig(upper=(L, L, L), width=2.8)))
+    b = ReplicaBatch(specs, dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 6)
+    lengths = torch.zeros(b.total, dtype=torch.float64, device=dev)
+    owned_mask = torch.zeros(b.total, dtype=torch.bool, device=dev)
+    for r, sp in enumerate(specs):
+        lengths[b.owned_slice(r)] = sp.box.upper[0]
+        owned_mask[b.owned_slice(r)] = True
+    for k in ("x", "y", "z"):
+        getattr(b, k).uniform_(0.0, 1.0, generator=gen).mul_(lengths)
+    b.mass.fill_(1.0)
+    start = {k: getattr(b, k).clone() for k in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy",
+                                               "fz", "mass", "charge", "type", "global_id")}
+    pool = [torch.randn((3, b.total), dtype=torch.float64, device=dev, generator=gen)
+            .mul_(0.1).mul_(owned_mask) for _ in range(4)]
+    frc = torch.rand((3, b.total), dtype=torch.float64, device=dev, generator=gen).mul_(2).sub_(1)
+    step_disp = torch.zeros((3, b.total), dtype=torch.float64, device=dev)
+
+    def restore():
+        for k, v in start.items():
+            getattr(b, k).copy_(v)
+
+    def app_halo():
+        b.x.add_(step_disp[0]); b.y.add_(step_disp[1]); b.z.add_(step_disp[2])
+
+    def app_rev():
+        b.fx.copy_(frc[0]); b.fy.copy_(frc[1]); b.fz.copy_(frc[2])
+
+    steps = args.batch_steps
+
+    def timed(run_step, count):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(count):
+            step_disp.copy_(pool[k % 4])
+            run_step()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    loop = LoopedReplicas(b)
+
+    def loop_step():
+        app_halo(); loop.migrate(); loop.exchange_halo(); app_rev(); loop.reverse_halo()
+
+    def eager_step():
+        app_halo(); b.migrate(); b.exchange_halo(); app_rev(); b.reverse_halo()
+
+    restore()
+    timed(loop_step, 1)
+    restore()
+    t_loop = timed(loop_step, steps)
+    loop_state = {k: getattr(b, k).clone() for k in start}
+    restore()
+    timed(eager_step, 1)
+    restore()
+    t_eager = timed(eager_step, steps)
+    restore()
+    graph = capture_step(b, app_halo, app_rev, warmup=1)
+    restore()
+    t_graph = timed(graph.replay, steps)
+    equal = all(torch.equal(getattr(b, k).view(torch.int64) if getattr(b, k).dtype == torch.float64
+                            else getattr(b, k),
+                            loop_state[k].view(torch.int64) if loop_state[k].dtype == torch.float64
+                            else loop_state[k]) for k in start)
+    ghosts, owners, status = b.counts()
+    total = int(n.sum())
+    return {
+        "workload": f"cfg5: {R} periodic replica boxes, {total} particles (log-uniform "
+                    f"2048..262144 per box, density U[0.6,1.2]), halo width 2.8, "
+                    f"displacement N(0,0.1)",
+        "replicas": R, "particles": total, "steps": steps,
+        "ghosts": int(sum(ghosts)), "overflowed_replicas": int(sum(1 for s_ in status if s_)),
+        "looped_replica_steps_per_s": R * steps / t_loop,
+        "batched_eager_replica_steps_per_s": R * steps / t_eager,
+        "batched_graph_replica_steps_per_s": R * steps / t_graph,
+        "graph_vs_looped": t_loop / t_graph,
+        "eager_vs_looped": t_loop / t_eager,
+        "graph_ms_per_step": 1e3 * t_graph / steps,
+        "looped_ms_per_step": 1e3 * t_loop / steps,
+        "bit_exact_graph_vs_looped": bool(equal),
+        "shape": list(b.shape()),
+        "timing": "wall clock over the steps, application displacement and force fill included "
+                  "in every form",
+    }
+
+
+def run(args):
+    name = args.config
+    n, L, w, sigma, periodic = WORKLOADS[name]
+    if args.particles:
+        n = args.particles
+    if not (periodic[0] and periodic[1]):
+        raise SystemExit("x and y are periodic in every workload")
+    res = (measure_cpp if args.path == "cpp" else measure_abi)(args, n, L, w, sigma, periodic)
+    ms = res["hot_ms_per_step"]
+    G = res["ghosts_per_step"]
+    U = res["owners"]
     axes = sum(1 for a in periodic if a)
     ab = algorithmic_bytes(n, G, U, axes)
     step_bytes = sum(ab.values())
 
     per_kernel = {}
-    for k, (tot, cnt) in kern.items():
-        if cnt:
-            per_kernel[k] = {"avg_ms": tot / cnt, "launches": cnt}
+    for k, v in res["kernels"].items():
+        if v["launches"]:
+            per_kernel[k] = {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"]}
+            if ab.get(k):
+                per_kernel[k]["algorithmic_GBps"] = ab[k] / (per_kernel[k]["avg_ms"] * 1e-3) / 1e9
     dominant = max((k for k in per_kernel if k in ab), key=lambda k: per_kernel[k]["avg_ms"])
-    dom_ms = per_kernel[dominant]["avg_ms"]
-    dom_gbs = ab[dominant] / (dom_ms * 1e-3) / 1e9
-    for k in per_kernel:
-        if k in ab and ab[k] > 0:
-            per_kernel[k]["algorithmic_GBps"] = ab[k] / (per_kernel[k]["avg_ms"] * 1e-3) / 1e9
+    dom_gbs = ab[dominant] / (per_kernel[dominant]["avg_ms"] * 1e-3) / 1e9
 
     out = {
         "metric": METRIC,
@@ -190,11 +324,11 @@ def run(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{name}: {n} particles uniform in a periodic box L={L} "
-                        f"(periodic={list(periodic)}), halo width {w}, displacement N(0,{sigma}) "
-                        f"per axis per step, 1x1x1 self-loopback grid",
+            "workload": f"{name}: {n} particles uniform in a box L={L} (periodic={list(periodic)}), "
+                        f"halo width {w}, displacement N(0,{sigma}) per axis per step, "
+                        f"1x1x1 self-loopback grid",
             "particles": n, "box": L, "halo_width": w, "periodic": list(periodic),
-            "path": "C ABI (dsdx_migrate_wrap, dsdx_halo_build, dsdx_reverse_halo)",
+            "path": res["path"],
         },
         "roofline": {
             "bound": "hbm", "kernel": dominant,
@@ -208,13 +342,13 @@ def run(args):
         },
         "ghosts_per_step": G,
         "owners_per_step": U,
-        "wall_ms_per_step": 1e3 * wall / args.steps,
+        "wall_ms_per_step": res["wall_ms_per_step"],
         "kernels": per_kernel,
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(name, n, L, w, sigma, periodic, args.cpu_steps)
-    else:
-        out["cpu_baseline"] = None
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(name, n, L, w, sigma, periodic,
+                                                                 args.cpu_steps)
+    if not args.no_batched:
+        out["batched"] = measure_batched(args)
     return out
 
 
@@ -234,9 +368,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="cfg3", choices=sorted(WORKLOADS))
     ap.add_argument("--particles", type=int, default=0, help="override the particle count")
+    ap.add_argument("--path", default="cpp", choices=("cpp", "abi"),
+                    help="cpp: the C++ Decomp API driver; abi: the C ABI from Python")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-batched", action="store_true", help="skip the cfg5 batched comparison")
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per launch of the dominant kernel from rocprofv3 counters")
     args = ap.parse_args()

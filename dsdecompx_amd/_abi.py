@@ -33,6 +33,11 @@ class Particles(C.Structure):
                                    "mass", "charge", "type", "global_id")]
 
 
+class Replica(C.Structure):
+    _fields_ = [("box", Box), ("offset", C.c_uint64), ("capacity", C.c_uint64),
+                ("owned", C.c_uint64)]
+
+
 class Field(C.Structure):
     _fields_ = [("data", _VP), ("type", C.c_int32), ("components", C.c_int32),
                 ("op", C.c_int32)]
@@ -84,6 +89,21 @@ def lib():
         "dsdx_reverse_halo": ([P, P, P, P, u64, P], C.c_int),
         "dsdx_reverse_exchange": ([P, C.POINTER(Field), u32, u64, P], C.c_int),
         "dsdx_forward_exchange": ([P, C.POINTER(Field), u32, u64, P], C.c_int),
+        "dsdx_plan_set_shape": ([P, i32, i32, i32], C.c_int),
+        "dsdx_plan_get_shape": ([P, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
+        "dsdx_batch_create": ([C.POINTER(Replica), u32, P, C.POINTER(P)], C.c_int),
+        "dsdx_batch_destroy": ([P], C.c_int),
+        "dsdx_batch_set_shape": ([P, i32, i32, i32, P], C.c_int),
+        "dsdx_batch_get_shape": ([P, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
+        "dsdx_batch_migrate_wrap": ([P, P, P, P, P], C.c_int),
+        "dsdx_batch_halo_build": ([P, C.POINTER(Particles), C.POINTER(Field), u32, P], C.c_int),
+        "dsdx_batch_refresh": ([P, C.POINTER(Particles), C.POINTER(Field), u32, i32, P], C.c_int),
+        "dsdx_batch_reverse_halo": ([P, P, P, P, P], C.c_int),
+        "dsdx_batch_reverse_exchange": ([P, C.POINTER(Field), u32, P], C.c_int),
+        "dsdx_batch_forward_exchange": ([P, C.POINTER(Field), u32, P], C.c_int),
+        "dsdx_batch_ghost_owners": ([P, P, P], C.c_int),
+        "dsdx_batch_counts": ([P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), P], C.c_int),
+        "dsdx_batch_device_totals": ([P], C.c_void_p),
         "dsdx_timing_enable": ([i32], C.c_int),
         "dsdx_timing_reset": ([], C.c_int),
         "dsdx_timing_read": ([C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)], C.c_int),
@@ -102,7 +122,13 @@ def exported_symbols():
             "dsdx_plan_reserve", "dsdx_migrate_wrap", "dsdx_halo_build", "dsdx_halo_scatter",
             "dsdx_halo_refresh", "dsdx_plan_query", "dsdx_plan_ghost_owners",
             "dsdx_reverse_halo", "dsdx_reverse_exchange", "dsdx_forward_exchange",
-            "dsdx_timing_enable", "dsdx_timing_reset", "dsdx_timing_read"]
+            "dsdx_timing_enable", "dsdx_timing_reset", "dsdx_timing_read",
+            "dsdx_plan_set_shape", "dsdx_plan_get_shape", "dsdx_batch_create",
+            "dsdx_batch_destroy", "dsdx_batch_set_shape", "dsdx_batch_get_shape",
+            "dsdx_batch_migrate_wrap", "dsdx_batch_halo_build", "dsdx_batch_refresh",
+            "dsdx_batch_reverse_halo", "dsdx_batch_reverse_exchange",
+            "dsdx_batch_forward_exchange", "dsdx_batch_ghost_owners", "dsdx_batch_counts",
+            "dsdx_batch_device_totals"]
 
 
 KERNELS = ("migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "reverse_halo",

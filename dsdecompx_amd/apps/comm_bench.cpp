@@ -1,0 +1,209 @@
+// comm_bench.cpp -- timing driver of the drop-in C++ API on one MI355X, the
+// counterpart of the reference's examples/communication_test/main.cu:
+// every step moves all particles by N(0, sigma) per axis, then runs
+// migrate -> exchange_halo (rebuild) -> [forces U[-1,1] on owned + ghosts]
+// -> reverse_halo through dsdecompx::Decomp on a HIP stream.
+//
+// HIP events on that stream bracket migrate + exchange_halo and reverse_halo;
+// their sum is the hot-path time.  The per-kernel HIP-event timers of the
+// library (dsdx_timing_*) give each kernel's average duration.  Prints one
+// JSON object on stdout.
+//
+// usage: comm_bench particles L width sigma z_periodic steps warmup seed
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "dsdecompx.hpp"
+#include "dsdx_hip.h"
+
+using namespace dsdecompx;
+
+namespace {
+
+void ok(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "HIP error %s: %s\n", what, hipGetErrorString(e));
+        std::exit(3);
+    }
+}
+
+__device__ __forceinline__ uint64_t mix(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// uniform in [0, 1) from (seed, stream, index)
+__device__ __forceinline__ double unit(uint64_t seed, uint64_t stream, uint64_t i)
+{
+    const uint64_t z = mix(seed * 0x9E3779B97F4A7C15ull + stream * 0xD1B54A32D192ED03ull + i);
+    return static_cast<double>(z >> 11) * 0x1.0p-53;
+}
+
+__global__ void place(double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                      double* m, double* q, int* t, int* id, uint64_t n, double L, uint64_t seed)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    x[i] = L * unit(seed, 0, i);
+    y[i] = L * unit(seed, 1, i);
+    z[i] = L * unit(seed, 2, i);
+    vx[i] = unit(seed, 3, i) - 0.5;
+    vy[i] = unit(seed, 4, i) - 0.5;
+    vz[i] = unit(seed, 5, i) - 0.5;
+    m[i] = 1.0;
+    q[i] = 0.0;
+    t[i] = 0;
+    id[i] = static_cast<int>(i);
+}
+
+// Box-Muller displacement N(0, sigma) per axis
+__global__ void move(double* x, double* y, double* z, uint64_t n, double sigma, uint64_t seed,
+                     uint64_t step)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    double* axis[3] = {x, y, z};
+    for (int a = 0; a < 3; ++a) {
+        const double u1 = 1.0 - unit(seed, 16 + 2 * a + 8 * step, i);
+        const double u2 = unit(seed, 17 + 2 * a + 8 * step, i);
+        axis[a][i] += sigma * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    }
+}
+
+__global__ void fill_forces(double* fx, double* fy, double* fz, uint64_t count, uint64_t seed,
+                            uint64_t step)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= count) return;
+    fx[i] = 2.0 * unit(seed, 1000 + 3 * step, i) - 1.0;
+    fy[i] = 2.0 * unit(seed, 1001 + 3 * step, i) - 1.0;
+    fz[i] = 2.0 * unit(seed, 1002 + 3 * step, i) - 1.0;
+}
+
+// Particles within `w` of a face that has images: the distinct owners U of
+// the halo (bench bookkeeping for the reverse-halo byte count).
+__global__ void count_boundary(const double* x, const double* y, const double* z, uint64_t n,
+                               double lo, double hi, int zper, unsigned long long* out)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    bool b = false;
+    if (i < n) {
+        b = x[i] < lo || x[i] > hi || y[i] < lo || y[i] > hi;
+        if (zper) b = b || z[i] < lo || z[i] > hi;
+    }
+    const unsigned long long ballot = __ballot(b);
+    if ((threadIdx.x & 63) == 0 && ballot) atomicAdd(out, static_cast<unsigned long long>(__popcll(ballot)));
+}
+
+unsigned grid(uint64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc < 9) {
+        std::fprintf(stderr, "usage: %s particles L width sigma z_periodic steps warmup seed\n", argv[0]);
+        return 2;
+    }
+    const uint64_t n = std::strtoull(argv[1], nullptr, 10);
+    const double L = std::atof(argv[2]);
+    const double w = std::atof(argv[3]);
+    const double sigma = std::atof(argv[4]);
+    const bool zper = std::atoi(argv[5]) != 0;
+    const int steps = std::atoi(argv[6]);
+    const int warmup = std::atoi(argv[7]);
+    const uint64_t seed = std::strtoull(argv[8], nullptr, 10);
+
+    Config config;
+    config.domain.upper = {L, L, L};
+    config.domain.periodic = {true, true, zper};
+    config.halo.width = w;
+    Decomp decomp(MPI_COMM_WORLD, config);
+    hipStream_t s = nullptr;
+    ok(hipStreamCreate(&s), "stream");
+
+    ParticleInfo p = ParticleInfo::allocate(MemorySpace::CudaDevice, n);
+    hipLaunchKernelGGL(place, dim3(grid(n)), dim3(256), 0, s, p.x_data(), p.y_data(), p.z_data(),
+                       p.vx_data(), p.vy_data(), p.vz_data(), p.mass_data(), p.charge_data(),
+                       p.type_data(), p.global_id_data(), n, L, seed);
+    ok(hipStreamSynchronize(s), "place");
+
+    hipEvent_t ev[4];
+    for (auto& e : ev) ok(hipEventCreate(&e), "event");
+    double hot_ms = 0.0;
+    double ghosts_sum = 0.0;
+    uint64_t last_ghosts = 0;
+    std::chrono::steady_clock::time_point wall0;
+    for (int step = 0; step < warmup + steps; ++step) {
+        const bool timed = step >= warmup;
+        if (step == warmup) {
+            ok(hipStreamSynchronize(s), "sync");
+            dsdx_timing_reset();
+            dsdx_timing_enable(1);
+            wall0 = std::chrono::steady_clock::now();
+        }
+        hipLaunchKernelGGL(move, dim3(grid(n)), dim3(256), 0, s, p.x_data(), p.y_data(), p.z_data(),
+                           n, sigma, seed, static_cast<uint64_t>(step));
+        if (timed) ok(hipEventRecord(ev[0], s), "record");
+        decomp.migrate(p, s);
+        const HaloResult r = decomp.exchange_halo(p, {true, s});
+        if (timed) ok(hipEventRecord(ev[1], s), "record");
+        const uint64_t total = n + r.ghost_count;
+        hipLaunchKernelGGL(fill_forces, dim3(grid(total)), dim3(256), 0, s, p.fx_data(), p.fy_data(),
+                           p.fz_data(), total, seed, static_cast<uint64_t>(step));
+        if (timed) ok(hipEventRecord(ev[2], s), "record");
+        decomp.reverse_halo(p, s);
+        if (timed) {
+            ok(hipEventRecord(ev[3], s), "record");
+            ok(hipEventSynchronize(ev[3]), "sync");
+            float a = 0.0f, b = 0.0f;
+            ok(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
+            ok(hipEventElapsedTime(&b, ev[2], ev[3]), "elapsed");
+            hot_ms += a + b;
+            ghosts_sum += static_cast<double>(r.ghost_count);
+        }
+        last_ghosts = r.ghost_count;
+    }
+    ok(hipStreamSynchronize(s), "sync");
+    const double wall_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    dsdx_timing_enable(0);
+
+    unsigned long long* d_u = nullptr;
+    ok(hipMalloc(&d_u, sizeof(*d_u)), "malloc");
+    ok(hipMemsetAsync(d_u, 0, sizeof(*d_u), s), "memset");
+    hipLaunchKernelGGL(count_boundary, dim3(grid(n)), dim3(256), 0, s, p.x_data(), p.y_data(),
+                       p.z_data(), n, w, L - w, zper ? 1 : 0, d_u);
+    unsigned long long owners = 0;
+    ok(hipMemcpyAsync(&owners, d_u, sizeof(owners), hipMemcpyDeviceToHost, s), "copy");
+    ok(hipStreamSynchronize(s), "sync");
+    ok(hipFree(d_u), "free");
+
+    const char* kernels[] = {"migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "reverse_halo"};
+    std::printf("{\"particles\": %llu, \"steps\": %d, \"warmup\": %d, \"hot_ms_per_step\": %.6f, "
+                "\"wall_ms_per_step\": %.6f, \"ghosts_per_step\": %.1f, \"last_ghosts\": %llu, "
+                "\"owners\": %llu, \"kernels\": {",
+                static_cast<unsigned long long>(n), steps, warmup, hot_ms / steps, wall_ms / steps,
+                ghosts_sum / steps, static_cast<unsigned long long>(last_ghosts), owners);
+    for (int k = 0; k < 5; ++k) {
+        double ms = 0.0;
+        uint64_t launches = 0;
+        dsdx_timing_read(kernels[k], &ms, &launches);
+        std::printf("%s\"%s\": {\"total_ms\": %.6f, \"launches\": %llu}", k ? ", " : "", kernels[k], ms,
+                    static_cast<unsigned long long>(launches));
+    }
+    std::printf("}}\n");
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(s);
+    return 0;
+}

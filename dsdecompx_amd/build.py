@@ -78,5 +78,27 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+APPS = os.path.join(PKG, "apps")
+BINDIR = os.path.join(PKG, "bin")
+
+
+def build_apps() -> list:
+    """Programs of apps/ (the C++ benchmark driver), linked to the library."""
+    os.makedirs(BINDIR, exist_ok=True)
+    out = []
+    for f in sorted(os.listdir(APPS)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(BINDIR, f[:-4])
+        cmd = [HIPCC] + COMMON + DEVICE + ["-x", "hip", os.path.join(APPS, f), "-L", LIBDIR,
+                                           "-ldsdecompx_amd", "-Wl,-rpath,$ORIGIN/../lib", "-o", exe]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"building {f} failed:\n{r.stdout}")
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_apps())

@@ -1,17 +1,22 @@
 // abi.cpp -- C ABI of the gfx950 halo kernels (include/dsdx_hip.h): argument
-// validation, the device halo plan and its growable buffers, launches.
+// validation, the device halo plan (a replica table plus growable buffers),
+// launches, and the batched form.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <string>
+#include <unordered_map>
+#include <vector>
 
+#include "abi_internal.hpp"
 #include "dsdx_hip.h"
 #include "kernels.hpp"
 #include "timing.hpp"
 
-namespace {
+namespace dsdx::abi {
 
 thread_local std::string g_error;
 
@@ -27,68 +32,45 @@ int hip_fail(hipError_t status, const char* operation)
     return DSDX_EDEVICE;
 }
 
-#define DSDX_HIP(call, what)                         \
-    do {                                             \
-        const hipError_t status_ = (call);           \
-        if (status_ != hipSuccess) return hip_fail(status_, what); \
-    } while (0)
-
-// Device allocation that grows by 1.5x (at least 4 KiB) and keeps nothing on
-// growth (src/cuda/cuda_buffer.cpp:10-54 keeps the contents; no caller here
-// needs them).  Waits for the device before freeing the old block, which may
-// still be read by queued kernels.
-struct DeviceBuffer {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-
-    hipError_t reserve(size_t need)
-    {
-        if (need <= bytes) return hipSuccess;
-        const size_t grown = std::max<size_t>({need, bytes + bytes / 2, 4096});
-        if (ptr) {
-            const hipError_t s = hipDeviceSynchronize();
-            if (s != hipSuccess) return s;
-            (void)hipFree(ptr);
-            ptr = nullptr;
-            bytes = 0;
-        }
-        const hipError_t s = hipMalloc(&ptr, grown);
-        if (s == hipSuccess) bytes = grown;
-        return s;
+hipError_t DeviceBuffer::reserve(size_t need)
+{
+    if (need <= bytes) return hipSuccess;
+    const size_t grown = std::max<size_t>({need, bytes + bytes / 2, 4096});
+    if (ptr) {
+        const hipError_t s = hipDeviceSynchronize();
+        if (s != hipSuccess) return s;
+        (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
     }
-    template <class T> T* as() const { return static_cast<T*>(ptr); }
-    ~DeviceBuffer() { if (ptr) (void)hipFree(ptr); }
-};
+    const hipError_t s = hipMalloc(&ptr, grown);
+    if (s == hipSuccess) bytes = grown;
+    return s;
+}
 
-}  // namespace
-
-struct dsdx_plan {
-    dsdx::BoxParams bp{};
-    bool classified = false;   // classify + scan of `owned` particles are on the device
-    bool built = false;        // ghosts and owner map written: reverse/forward usable
-    uint64_t owned = 0;
-    uint32_t chunks = 0;
-    DeviceBuffer entries, counts, prefix, totals, owner_index, owner_code, image_start, ghost_list;
-    uint64_t ghost_list_capacity = 0;   // elements
-    uint32_t* host_totals = nullptr;    // pinned copy of totals
-    bool totals_cached = false;         // host_totals hold the current plan's counts
-
-    dsdx::PlanView view() const
-    {
-        return {totals.as<uint32_t>(), owner_index.as<uint32_t>(), owner_code.as<uint8_t>(),
-                image_start.as<uint32_t>(), ghost_list.as<uint32_t>()};
-    }
-};
+DeviceBuffer::~DeviceBuffer()
+{
+    if (ptr) (void)hipFree(ptr);
+}
 
 namespace {
-
-using dsdx::BoxParams;
 
 int direction_slot(int dx, int dy, int dz)
 {
     const int raw = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
     return raw < 13 ? raw : raw - 1;
 }
+
+size_t field_type_size(int32_t type)
+{
+    switch (type) {
+    case DSDX_INT32: case DSDX_UINT32: case DSDX_FLOAT32: return 4;
+    case DSDX_INT64: case DSDX_UINT64: case DSDX_FLOAT64: return 8;
+    default: return 0;
+    }
+}
+
+}  // namespace
 
 // Neighbour table of the 1x1x1 grid (src/topology.cpp:85-166 with every
 // direction a self-neighbour): active send slots, their periodic shifts, and
@@ -101,7 +83,7 @@ int make_box_params(const dsdx_box* box, BoxParams* bp)
         if (!(box->upper[a] > box->lower[a]))
             return fail(DSDX_EINVAL, "global-domain extent must be positive");
     }
-    if (box->width < 0.0) return fail(DSDX_EINVAL, "halo width cannot be negative");
+    if (!(box->width >= 0.0)) return fail(DSDX_EINVAL, "halo width cannot be negative");
     std::memset(bp, 0, sizeof(*bp));
     double extent[3];
     for (int a = 0; a < 3; ++a) {
@@ -113,7 +95,7 @@ int make_box_params(const dsdx_box* box, BoxParams* bp)
         bp->periodic[a] = box->periodic[a] ? 1 : 0;
     }
     uint32_t active = 0;
-    int dir[dsdx::kSlots][3] = {};
+    int dir[kSlots][3] = {};
     for (int dx = -1; dx <= 1; ++dx)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dz = -1; dz <= 1; ++dz) {
@@ -132,7 +114,7 @@ int make_box_params(const dsdx_box* box, BoxParams* bp)
             }
     for (uint32_t code = 0; code < 64; ++code) {
         uint32_t mask = 0;
-        for (int s = 0; s < dsdx::kSlots; ++s) {
+        for (int s = 0; s < kSlots; ++s) {
             if (!((active >> s) & 1u)) continue;
             bool covered = true;
             for (int a = 0; a < 3; ++a) {
@@ -146,16 +128,7 @@ int make_box_params(const dsdx_box* box, BoxParams* bp)
     return DSDX_OK;
 }
 
-size_t field_type_size(int32_t type)
-{
-    switch (type) {
-    case DSDX_INT32: case DSDX_UINT32: case DSDX_FLOAT32: return 4;
-    case DSDX_INT64: case DSDX_UINT64: case DSDX_FLOAT64: return 8;
-    default: return 0;
-    }
-}
-
-int make_field_params(const dsdx_field* fields, uint32_t count, dsdx::FieldParams* fp)
+int make_field_params(const dsdx_field* fields, uint32_t count, FieldParams* fp)
 {
     std::memset(fp, 0, sizeof(*fp));
     if (count > DSDX_MAX_FIELDS)
@@ -174,6 +147,25 @@ int make_field_params(const dsdx_field* fields, uint32_t count, dsdx::FieldParam
     return DSDX_OK;
 }
 
+int make_typed_fields(const dsdx_field* fields, uint32_t count, TypedFields* tf)
+{
+    std::memset(tf, 0, sizeof(*tf));
+    if (count > DSDX_MAX_FIELDS)
+        return fail(DSDX_EINVAL, "at most 16 fields can be reverse-exchanged on device memory");
+    if (count && !fields) return fail(DSDX_EINVAL, "null field table");
+    for (uint32_t f = 0; f < count; ++f) {
+        if (!fields[f].data || fields[f].components <= 0 || field_type_size(fields[f].type) == 0 ||
+            fields[f].op < DSDX_SUM || fields[f].op > DSDX_REPLACE)
+            return fail(DSDX_EINVAL, "invalid reverse exchange field");
+        tf->type[f] = fields[f].type;
+        tf->op[f] = fields[f].op;
+        tf->components[f] = static_cast<uint32_t>(fields[f].components);
+        tf->data[f] = fields[f].data;
+    }
+    tf->count = count;
+    return DSDX_OK;
+}
+
 int check_particles(const dsdx_particles* p, uint64_t owned)
 {
     if (!p) return fail(DSDX_EINVAL, "null particle view");
@@ -183,72 +175,176 @@ int check_particles(const dsdx_particles* p, uint64_t owned)
     return DSDX_OK;
 }
 
-hipStream_t as_stream(void* stream) { return static_cast<hipStream_t>(stream); }
-
-constexpr uint64_t kMaxOwned = (1ull << 31) - 1;
-
-int reserve_plan(dsdx_plan* plan, uint64_t owned, uint64_t ghost_list_elems)
+// Sizes every buffer for the current table and shape (ghost list: at least
+// `ghost_list_elems`).
+int reserve_buffers(Plan* plan, uint64_t ghost_list_elems)
 {
-    const uint64_t chunks = (owned + dsdx::kChunk - 1) / dsdx::kChunk;
-    DSDX_HIP(plan->entries.reserve(std::max<uint64_t>(1, chunks) * dsdx::kChunk * 4), "hipMalloc");
-    DSDX_HIP(plan->counts.reserve(std::max<uint64_t>(1, chunks) * dsdx::kCols * 4), "hipMalloc");
-    DSDX_HIP(plan->prefix.reserve(std::max<uint64_t>(1, chunks) * dsdx::kCols * 4), "hipMalloc");
-    DSDX_HIP(plan->totals.reserve((dsdx::kCols + 2) * 4), "hipMalloc");
-    DSDX_HIP(plan->owner_index.reserve(std::max<uint64_t>(1, owned) * 4), "hipMalloc");
-    DSDX_HIP(plan->owner_code.reserve(std::max<uint64_t>(1, owned)), "hipMalloc");
-    DSDX_HIP(plan->image_start.reserve(std::max<uint64_t>(1, owned) * 4), "hipMalloc");
+    const uint64_t c = std::max<uint64_t>(1, plan->chunks);
+    const uint64_t slots = c * plan->chunk_size();
+    const uint64_t replicas = std::max<size_t>(1, plan->desc.size());
+    DSDX_HIP(plan->entries.reserve(slots * 4), "hipMalloc");
+    DSDX_HIP(plan->counts.reserve(c * kCols * 4), "hipMalloc");
+    DSDX_HIP(plan->prefix.reserve(c * kCols * 4), "hipMalloc");
+    DSDX_HIP(plan->totals.reserve(replicas * kTotals * 4), "hipMalloc");
+    DSDX_HIP(plan->owner_index.reserve(slots * 4), "hipMalloc");
+    DSDX_HIP(plan->owner_code.reserve(slots), "hipMalloc");
+    DSDX_HIP(plan->image_start.reserve(slots * 4), "hipMalloc");
     if (ghost_list_elems > plan->ghost_list_capacity) {
         DSDX_HIP(plan->ghost_list.reserve(ghost_list_elems * 4), "hipMalloc");
         plan->ghost_list_capacity = plan->ghost_list.bytes / 4;
     }
-    if (!plan->host_totals) {
+    const size_t words = replicas * kTotals;
+    if (words > plan->host_totals_words) {
+        if (plan->host_totals) (void)hipHostFree(plan->host_totals);
+        plan->host_totals = nullptr;
+        plan->host_totals_words = 0;
         void* host = nullptr;
-        DSDX_HIP(hipHostMalloc(&host, (dsdx::kCols + 2) * 4, hipHostMallocDefault), "hipHostMalloc");
+        DSDX_HIP(hipHostMalloc(&host, words * 4, hipHostMallocDefault), "hipHostMalloc");
         plan->host_totals = static_cast<uint32_t*>(host);
+        plan->host_totals_words = words;
     }
     return DSDX_OK;
 }
 
-int read_totals(dsdx_plan* plan, hipStream_t stream)
+// Chunk numbering of the table for the current shape.
+void number_chunks(Plan* plan)
 {
-    DSDX_HIP(hipMemcpyAsync(plan->host_totals, plan->totals.ptr, (dsdx::kCols + 1) * 4,
-                            hipMemcpyDeviceToHost, stream),
+    const uint32_t K = plan->chunk_size();
+    uint32_t next = 0;
+    plan->max_replica_chunks = 0;
+    plan->chunk_replica.clear();
+    for (size_t r = 0; r < plan->desc.size(); ++r) {
+        ReplicaDesc& d = plan->desc[r];
+        d.first_chunk = next;
+        d.chunks = (d.owned + K - 1) / K;
+        next += d.chunks;
+        plan->max_replica_chunks = std::max(plan->max_replica_chunks, d.chunks);
+        if (plan->desc.size() > 1)
+            plan->chunk_replica.insert(plan->chunk_replica.end(), d.chunks, static_cast<uint32_t>(r));
+    }
+    plan->chunks = next;
+}
+
+// Copies the table to the device and waits (the host copy may change after).
+int upload_table(Plan* plan, hipStream_t stream)
+{
+    DSDX_HIP(plan->d_desc.reserve(plan->desc.size() * sizeof(ReplicaDesc)), "hipMalloc");
+    DSDX_HIP(hipMemcpyAsync(plan->d_desc.ptr, plan->desc.data(), plan->desc.size() * sizeof(ReplicaDesc),
+                            hipMemcpyHostToDevice, stream),
+             "upload replica table");
+    if (plan->desc.size() > 1) {
+        DSDX_HIP(plan->d_chunk_replica.reserve(std::max<size_t>(1, plan->chunk_replica.size()) * 4),
+                 "hipMalloc");
+        if (!plan->chunk_replica.empty())
+            DSDX_HIP(hipMemcpyAsync(plan->d_chunk_replica.ptr, plan->chunk_replica.data(),
+                                    plan->chunk_replica.size() * 4, hipMemcpyHostToDevice, stream),
+                     "upload chunk table");
+    }
+    DSDX_HIP(hipStreamSynchronize(stream), "wait for the replica table");
+    plan->desc_on_device = true;
+    return DSDX_OK;
+}
+
+// Single-box table: one replica at offset 0; uploaded only when it changed.
+// With ghost_list_elems == 0 only the table is kept (no plan buffers).
+int set_single(Plan* plan, const BoxParams& bp, uint64_t owned, uint64_t capacity,
+               uint64_t ghost_list_elems, hipStream_t stream)
+{
+    ReplicaDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.bp = bp;
+    d.capacity = capacity;
+    d.owned = static_cast<uint32_t>(owned);
+    ReplicaDesc before;
+    std::memset(&before, 0, sizeof(before));
+    if (plan->desc.size() == 1) before = plan->desc[0];
+    plan->desc.assign(1, d);
+    number_chunks(plan);
+    if (ghost_list_elems) {
+        if (int rc = reserve_buffers(plan, ghost_list_elems)) return rc;
+        plan->desc[0].ghost_list_capacity = plan->ghost_list_capacity;
+    }
+    if (!plan->desc_on_device || std::memcmp(&before, &plan->desc[0], sizeof(ReplicaDesc)) != 0)
+        return upload_table(plan, stream);
+    return DSDX_OK;
+}
+
+int read_totals(Plan* plan, hipStream_t stream)
+{
+    const size_t words = plan->desc.size() * kTotals;
+    DSDX_HIP(hipMemcpyAsync(plan->host_totals, plan->totals.ptr, words * 4, hipMemcpyDeviceToHost, stream),
              "copy halo counts to the host");
     DSDX_HIP(hipStreamSynchronize(stream), "wait for halo counts");
     plan->totals_cached = true;
     return DSDX_OK;
 }
 
-int launch_scatter(dsdx_plan* plan, const dsdx_particles* p, const dsdx::FieldParams& fp,
-                   uint64_t capacity, hipStream_t stream)
+int run_classify(Plan* plan, const dsdx_particles* p, hipStream_t s)
 {
-    dsdx::timing::Scope timed(dsdx::timing::kScatter, stream);
-    DSDX_HIP(dsdx::launch_scatter(*p, static_cast<uint32_t>(plan->owned), plan->bp, fp,
-                                  plan->entries.as<uint32_t>(), plan->counts.as<uint32_t>(),
-                                  plan->prefix.as<uint32_t>(), plan->totals.as<uint32_t>(),
-                                  plan->owner_index.as<uint32_t>(), plan->owner_code.as<uint8_t>(),
-                                  plan->image_start.as<uint32_t>(), plan->ghost_list.as<uint32_t>(),
-                                  plan->chunks, capacity, plan->ghost_list_capacity, stream),
+    const Layout L = plan->layout();
+    const PlanBuffers B = plan->buffers();
+    {
+        timing::Scope timed(timing::kClassify, s);
+        DSDX_HIP(launch_classify(p->x, p->y, p->z, L, B, plan->shape, s), "launch halo classify");
+    }
+    {
+        timing::Scope timed(timing::kScan, s);
+        DSDX_HIP(launch_scan(L, B, plan->shape, s), "launch halo scan");
+    }
+    plan->classified = true;
+    return DSDX_OK;
+}
+
+int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipStream_t s)
+{
+    timing::Scope timed(timing::kScatter, s);
+    DSDX_HIP(launch_scatter(*p, fp, plan->layout(), plan->buffers(), plan->shape, s),
              "launch halo scatter");
     return DSDX_OK;
 }
 
-// Scatter again after an overflow: clear the status word, grow the owner ->
-// ghost map to the known ghost count, launch, and wait for the status.
-int rescatter(dsdx_plan* plan, const dsdx_particles* p, const dsdx::FieldParams& fp,
-              uint64_t capacity, hipStream_t stream, uint64_t ghosts)
+int shape_from_env(Shape* s)
 {
-    if (plan->owned + ghosts > capacity)
-        return fail(DSDX_ECAPACITY, "device halo output capacity is too small");
-    if (int rc = reserve_plan(plan, plan->owned, ghosts + ghosts / 4 + 1024)) return rc;
-    DSDX_HIP(hipMemsetAsync(plan->totals.as<uint32_t>() + dsdx::kStatusWord, 0, 4, stream),
-             "clear halo status");
-    if (int rc = launch_scatter(plan, p, fp, capacity, stream)) return rc;
+    if (const char* v = std::getenv("DSDX_ITEMS")) s->items = std::atoi(v);
+    if (const char* v = std::getenv("DSDX_WAVES")) s->waves = std::atoi(v);
+    if (const char* v = std::getenv("DSDX_SCAN_THREADS")) s->scan_threads = std::atoi(v);
+    return shape_supported(*s) ? DSDX_OK : fail(DSDX_EINVAL, "unsupported launch shape");
+}
+
+}  // namespace dsdx::abi
+
+using namespace dsdx;
+using namespace dsdx::abi;
+
+struct dsdx_plan : Plan {};
+struct dsdx_batch : Plan {};
+
+namespace {
+
+hipStream_t as_stream(void* stream) { return static_cast<hipStream_t>(stream); }
+
+constexpr uint64_t kMaxOwned = (1ull << 31) - 1;
+
+// Single box: scatter again after an overflow with the capacity and ghost
+// list grown to the known ghost count, then wait for the status.
+int rescatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, uint64_t capacity,
+              hipStream_t stream, uint64_t ghosts)
+{
+    const uint64_t owned = plan->desc[0].owned;
+    if (owned + ghosts > capacity) return fail(DSDX_ECAPACITY, "device halo output capacity is too small");
+    const BoxParams bp = plan->desc[0].bp;
+    if (int rc = set_single(plan, bp, owned, capacity, ghosts + ghosts / 4 + 1024, stream)) return rc;
+    DSDX_HIP(hipMemsetAsync(plan->totals.as<uint32_t>() + kStatusWord, 0, 4, stream), "clear halo status");
+    if (int rc = run_scatter(plan, p, fp, stream)) return rc;
     if (int rc = read_totals(plan, stream)) return rc;
-    if (plan->host_totals[dsdx::kStatusWord] != 0)
-        return fail(DSDX_EDEVICE, "halo scatter overflow after growth");
+    if (plan->host_totals[kStatusWord] != 0) return fail(DSDX_EDEVICE, "halo scatter overflow after growth");
     plan->built = true;
     return DSDX_OK;
+}
+
+bool single_matches(const Plan* plan, uint64_t owned)
+{
+    return plan->desc.size() == 1 && owned == plan->desc[0].owned;
 }
 
 }  // namespace
@@ -261,13 +357,17 @@ int dsdx_plan_create(dsdx_plan** plan)
 {
     if (!plan) return fail(DSDX_EINVAL, "null plan pointer");
     *plan = new (std::nothrow) dsdx_plan();
-    return *plan ? DSDX_OK : fail(DSDX_EDEVICE, "out of host memory");
+    if (!*plan) return fail(DSDX_EDEVICE, "out of host memory");
+    if (int rc = shape_from_env(&(*plan)->shape)) {
+        delete *plan;
+        *plan = nullptr;
+        return rc;
+    }
+    return DSDX_OK;
 }
 
 int dsdx_plan_destroy(dsdx_plan* plan)
 {
-    if (!plan) return DSDX_OK;
-    if (plan->host_totals) (void)hipHostFree(plan->host_totals);
     delete plan;
     return DSDX_OK;
 }
@@ -278,8 +378,6 @@ int dsdx_plan_reset(dsdx_plan* plan)
     plan->classified = false;
     plan->built = false;
     plan->totals_cached = false;
-    plan->owned = 0;
-    plan->chunks = 0;
     return DSDX_OK;
 }
 
@@ -287,24 +385,71 @@ int dsdx_plan_reserve(dsdx_plan* plan, uint64_t owned, uint64_t ghosts)
 {
     if (!plan) return fail(DSDX_EINVAL, "null plan");
     if (owned > kMaxOwned) return fail(DSDX_EINVAL, "too many particles for one device plan");
-    return reserve_plan(plan, owned, std::max<uint64_t>(ghosts, 1));
+    const uint32_t K = plan->chunk_size();
+    const uint32_t saved = plan->chunks;
+    plan->chunks = static_cast<uint32_t>((owned + K - 1) / K);
+    const int rc = reserve_buffers(plan, std::max<uint64_t>(ghosts, 1));
+    plan->chunks = saved;
+    return rc;
 }
 
-int dsdx_migrate_wrap(const dsdx_box* box, double* x, double* y, double* z, uint64_t owned,
-                      void* stream)
+int dsdx_plan_set_shape(dsdx_plan* plan, int32_t items, int32_t waves, int32_t scan_threads)
+{
+    if (!plan) return fail(DSDX_EINVAL, "null plan");
+    const Shape s{items, waves, scan_threads};
+    if (!shape_supported(s)) return fail(DSDX_EINVAL, "unsupported launch shape");
+    if (s.items != plan->shape.items) {
+        plan->classified = plan->built = plan->totals_cached = false;
+        plan->desc_on_device = false;
+    }
+    plan->shape = s;
+    return DSDX_OK;
+}
+
+int dsdx_plan_get_shape(const dsdx_plan* plan, int32_t* items, int32_t* waves, int32_t* scan_threads)
+{
+    if (!plan) return fail(DSDX_EINVAL, "null plan");
+    if (items) *items = plan->shape.items;
+    if (waves) *waves = plan->shape.waves;
+    if (scan_threads) *scan_threads = plan->shape.scan_threads;
+    return DSDX_OK;
+}
+
+int dsdx_migrate_wrap(const dsdx_box* box, double* x, double* y, double* z, uint64_t owned, void* stream)
 {
     BoxParams bp;
     if (int rc = make_box_params(box, &bp)) return rc;
     if (!stream) return fail(DSDX_EINVAL, "device migration requires an explicit stream");
     if (owned && (!x || !y || !z)) return fail(DSDX_EINVAL, "incomplete device particle view");
-    dsdx::timing::Scope timed(dsdx::timing::kMigrate, as_stream(stream));
-    DSDX_HIP(dsdx::launch_migrate_wrap(x, y, z, owned, bp, as_stream(stream)), "launch migrate wrap");
+    if (owned > kMaxOwned) return fail(DSDX_EINVAL, "too many particles for one device plan");
+    if (owned == 0) return DSDX_OK;
+    // One-replica tables (the wrap needs no plan buffers), cached per thread
+    // by box and count so that a loop over many boxes uploads each table
+    // once.  Never freed: they live until the process ends, so no HIP call
+    // runs from a static destructor.
+    thread_local auto* cache = new std::unordered_map<std::string, Plan*>();
+    std::string key(reinterpret_cast<const char*>(&bp), sizeof(bp));
+    key.append(reinterpret_cast<const char*>(&owned), sizeof(owned));
+    const hipStream_t s = as_stream(stream);
+    Plan* wrap_plan = nullptr;
+    const auto found = cache->find(key);
+    if (found != cache->end()) {
+        wrap_plan = found->second;
+    } else {
+        if (cache->size() > 4096) cache->clear();   // bounded; the old tables stay allocated
+        wrap_plan = new Plan();
+        if (int rc = shape_from_env(&wrap_plan->shape)) return rc;
+        if (int rc = set_single(wrap_plan, bp, owned, owned, 0, s)) return rc;
+        (*cache)[key] = wrap_plan;
+    }
+    timing::Scope timed(timing::kMigrate, s);
+    DSDX_HIP(launch_migrate_wrap(x, y, z, wrap_plan->layout(), wrap_plan->shape, s), "launch migrate wrap");
     return DSDX_OK;
 }
 
 int dsdx_halo_build(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles* particles,
-                    const dsdx_field* fields, uint32_t field_count, uint64_t owned,
-                    uint64_t capacity, void* stream, uint64_t* ghosts)
+                    const dsdx_field* fields, uint32_t field_count, uint64_t owned, uint64_t capacity,
+                    void* stream, uint64_t* ghosts)
 {
     if (!plan) return fail(DSDX_EINVAL, "null plan");
     if (!stream) return fail(DSDX_EINVAL, "device halo requires an explicit stream");
@@ -313,61 +458,44 @@ int dsdx_halo_build(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles* 
     BoxParams bp;
     if (int rc = make_box_params(box, &bp)) return rc;
     if (int rc = check_particles(particles, owned)) return rc;
-    dsdx::FieldParams fp;
+    FieldParams fp;
     if (int rc = make_field_params(fields, field_count, &fp)) return rc;
     const hipStream_t s = as_stream(stream);
 
     plan->classified = plan->built = plan->totals_cached = false;
     const uint64_t guess = std::max<uint64_t>(plan->ghost_list_capacity, owned + 1024);
-    if (int rc = reserve_plan(plan, owned, guess)) return rc;
-    plan->bp = bp;
-    plan->owned = owned;
-    plan->chunks = static_cast<uint32_t>((owned + dsdx::kChunk - 1) / dsdx::kChunk);
-
-    {
-    dsdx::timing::Scope timed(dsdx::timing::kClassify, s);
-    DSDX_HIP(dsdx::launch_classify(particles->x, particles->y, particles->z,
-                                   static_cast<uint32_t>(owned), bp, plan->entries.as<uint32_t>(),
-                                   plan->counts.as<uint32_t>(), plan->chunks, s),
-             "launch halo classify");
-    }
-    {
-    dsdx::timing::Scope timed(dsdx::timing::kScan, s);
-    DSDX_HIP(dsdx::launch_scan(plan->counts.as<uint32_t>(), plan->prefix.as<uint32_t>(),
-                               plan->totals.as<uint32_t>(), plan->chunks, s),
-             "launch halo scan");
-    }
-    plan->classified = true;
-    if (int rc = launch_scatter(plan, particles, fp, capacity, s)) return rc;
+    if (int rc = set_single(plan, bp, owned, capacity, guess, s)) return rc;
+    if (int rc = run_classify(plan, particles, s)) return rc;
+    if (int rc = run_scatter(plan, particles, fp, s)) return rc;
     if (!ghosts) {
         plan->built = true;
         return DSDX_OK;
     }
     if (int rc = read_totals(plan, s)) return rc;
-    const uint64_t g = plan->host_totals[dsdx::kColImages];
-    const uint32_t status = plan->host_totals[dsdx::kStatusWord];
+    const uint64_t g = plan->host_totals[kColImages];
+    const uint32_t status = plan->host_totals[kStatusWord];
     *ghosts = g;
     if (status == 0) {
         plan->built = true;
         return DSDX_OK;
     }
-    if (status & dsdx::kOverflowStorage)
-        return fail(DSDX_ECAPACITY, "device halo output capacity is too small");
+    if (status & kOverflowStorage) return fail(DSDX_ECAPACITY, "device halo output capacity is too small");
     return rescatter(plan, particles, fp, capacity, s, g);
 }
 
 int dsdx_halo_scatter(dsdx_plan* plan, const dsdx_particles* particles, const dsdx_field* fields,
                       uint32_t field_count, uint64_t owned, uint64_t capacity, void* stream)
 {
-    if (!plan || !plan->classified) return fail(DSDX_ESTATE, "no classified halo to scatter");
+    if (!plan || !plan->classified || plan->desc.size() != 1)
+        return fail(DSDX_ESTATE, "no classified halo to scatter");
     if (!stream) return fail(DSDX_EINVAL, "device halo requires an explicit stream");
-    if (owned != plan->owned) return fail(DSDX_EINVAL, "owned count differs from the classified halo");
+    if (owned != plan->desc[0].owned) return fail(DSDX_EINVAL, "owned count differs from the classified halo");
     if (int rc = check_particles(particles, owned)) return rc;
-    dsdx::FieldParams fp;
+    FieldParams fp;
     if (int rc = make_field_params(fields, field_count, &fp)) return rc;
     const hipStream_t s = as_stream(stream);
     if (int rc = read_totals(plan, s)) return rc;
-    return rescatter(plan, particles, fp, capacity, s, plan->host_totals[dsdx::kColImages]);
+    return rescatter(plan, particles, fp, capacity, s, plan->host_totals[kColImages]);
 }
 
 int dsdx_halo_refresh(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles* particles,
@@ -375,19 +503,23 @@ int dsdx_halo_refresh(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles
                       int32_t positions_only, void* stream)
 {
     if (!plan || !plan->built) return fail(DSDX_ESTATE, "no device halo plan to refresh");
+    if (!stream) return fail(DSDX_EINVAL, "device halo requires an explicit stream");
+    if (!single_matches(plan, owned)) return fail(DSDX_EINVAL, "owned count differs from the halo plan");
+    if (int rc = check_particles(particles, owned)) return rc;
+    FieldParams fp;
+    if (int rc = make_field_params(fields, field_count, &fp)) return rc;
+    const hipStream_t s = as_stream(stream);
     if (box) {
         BoxParams bp;
         if (int rc = make_box_params(box, &bp)) return rc;
-        std::memcpy(plan->bp.shift, bp.shift, sizeof(bp.shift));
+        if (std::memcmp(plan->desc[0].bp.shift, bp.shift, sizeof(bp.shift)) != 0) {
+            std::memcpy(plan->desc[0].bp.shift, bp.shift, sizeof(bp.shift));
+            if (int rc = upload_table(plan, s)) return rc;
+        }
     }
-    if (!stream) return fail(DSDX_EINVAL, "device halo requires an explicit stream");
-    if (owned != plan->owned) return fail(DSDX_EINVAL, "owned count differs from the halo plan");
-    if (int rc = check_particles(particles, owned)) return rc;
-    dsdx::FieldParams fp;
-    if (int rc = make_field_params(fields, field_count, &fp)) return rc;
-    dsdx::timing::Scope timed(dsdx::timing::kRefresh, as_stream(stream));
-    DSDX_HIP(dsdx::launch_refresh(*particles, fp, static_cast<uint32_t>(owned), positions_only ? 1 : 0,
-                                  plan->view(), plan->bp, as_stream(stream)),
+    timing::Scope timed(timing::kRefresh, s);
+    DSDX_HIP(launch_refresh(*particles, fp, positions_only ? 1 : 0, plan->layout(), plan->buffers(),
+                            plan->shape, s),
              "launch halo refresh");
     return DSDX_OK;
 }
@@ -396,24 +528,24 @@ int dsdx_plan_query(const dsdx_plan* plan, uint32_t* send_counts, uint32_t* rece
                     uint64_t* ghosts, uint64_t* owners, int32_t* overflow, void* stream)
 {
     if (!plan) return fail(DSDX_EINVAL, "null plan");
-    uint32_t local[dsdx::kCols + 1] = {};
+    uint32_t local[kTotals] = {};
     if (plan->classified) {
         // A null stream returns the counts of the last waited build without
         // touching the device.
         if (stream) {
             if (int rc = read_totals(const_cast<dsdx_plan*>(plan), as_stream(stream))) return rc;
-        }
-        if (!stream && !plan->totals_cached)
+        } else if (!plan->totals_cached) {
             return fail(DSDX_EINVAL, "plan query without a stream needs a waited build");
+        }
         std::memcpy(local, plan->host_totals, sizeof(local));
     }
-    for (int s = 0; s < dsdx::kSlots; ++s) {
+    for (int s = 0; s < kSlots; ++s) {
         if (send_counts) send_counts[s] = local[s];
-        if (receive_counts) receive_counts[s] = local[dsdx::kSlots - 1 - s];
+        if (receive_counts) receive_counts[s] = local[kSlots - 1 - s];
     }
-    if (ghosts) *ghosts = local[dsdx::kColImages];
-    if (owners) *owners = local[dsdx::kColOwners];
-    if (overflow) *overflow = static_cast<int32_t>(local[dsdx::kStatusWord]);
+    if (ghosts) *ghosts = local[kColImages];
+    if (owners) *owners = local[kColOwners];
+    if (overflow) *overflow = static_cast<int32_t>(local[kStatusWord]);
     return DSDX_OK;
 }
 
@@ -422,9 +554,9 @@ int dsdx_plan_ghost_owners(const dsdx_plan* plan, int32_t* owners, void* stream)
     if (!plan) return fail(DSDX_EINVAL, "null plan");
     if (!plan->built) return DSDX_OK;
     if (!stream || !owners) return fail(DSDX_EINVAL, "ghost owner query needs output and stream");
-    dsdx::timing::Scope timed(dsdx::timing::kGhostOwners, as_stream(stream));
-    DSDX_HIP(dsdx::launch_ghost_owners(owners, static_cast<uint32_t>(plan->owned), plan->view(),
-                                       plan->bp, as_stream(stream)),
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kGhostOwners, s);
+    DSDX_HIP(launch_ghost_owners(owners, plan->layout(), plan->buffers(), plan->shape, s),
              "launch ghost owners");
     return DSDX_OK;
 }
@@ -435,12 +567,12 @@ int dsdx_reverse_halo(const dsdx_plan* plan, double* fx, double* fy, double* fz,
     if (!plan) return fail(DSDX_EINVAL, "null plan");
     if (!stream) return fail(DSDX_EINVAL, "device reverse halo requires an explicit stream");
     if (!plan->built) return DSDX_OK;   // no device halo: nothing to return
-    if (owned != plan->owned)
+    if (!single_matches(plan, owned))
         return fail(DSDX_EINVAL, "reverse exchange owned count differs from the last device halo");
     if (!fx || !fy || !fz) return fail(DSDX_EINVAL, "invalid device reverse-halo view");
-    dsdx::timing::Scope timed(dsdx::timing::kReverse, as_stream(stream));
-    DSDX_HIP(dsdx::launch_reverse_forces(fx, fy, fz, static_cast<uint32_t>(owned), plan->view(),
-                                         plan->bp, as_stream(stream)),
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kReverse, s);
+    DSDX_HIP(launch_reverse_forces(fx, fy, fz, plan->layout(), plan->buffers(), plan->shape, s),
              "launch reverse halo");
     return DSDX_OK;
 }
@@ -450,26 +582,14 @@ int dsdx_reverse_exchange(const dsdx_plan* plan, const dsdx_field* fields, uint3
 {
     if (!plan) return fail(DSDX_EINVAL, "null plan");
     if (!stream) return fail(DSDX_EINVAL, "device reverse exchange requires an explicit stream");
-    if (field_count > DSDX_MAX_FIELDS)
-        return fail(DSDX_EINVAL, "at most 16 fields can be reverse-exchanged on device memory");
-    dsdx::TypedFields tf;
-    std::memset(&tf, 0, sizeof(tf));
-    for (uint32_t f = 0; f < field_count; ++f) {
-        if (!fields[f].data || fields[f].components <= 0 || field_type_size(fields[f].type) == 0 ||
-            fields[f].op < DSDX_SUM || fields[f].op > DSDX_REPLACE)
-            return fail(DSDX_EINVAL, "invalid reverse exchange field");
-        tf.type[f] = fields[f].type;
-        tf.op[f] = fields[f].op;
-        tf.components[f] = static_cast<uint32_t>(fields[f].components);
-        tf.data[f] = fields[f].data;
-    }
-    tf.count = field_count;
+    TypedFields tf;
+    if (int rc = make_typed_fields(fields, field_count, &tf)) return rc;
     if (!plan->built || field_count == 0) return DSDX_OK;
-    if (owned != plan->owned)
+    if (!single_matches(plan, owned))
         return fail(DSDX_EINVAL, "reverse exchange owned count differs from the last device halo");
-    dsdx::timing::Scope timed(dsdx::timing::kReverseFields, as_stream(stream));
-    DSDX_HIP(dsdx::launch_reverse_fields(tf, static_cast<uint32_t>(owned), plan->view(), plan->bp,
-                                         as_stream(stream)),
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kReverseFields, s);
+    DSDX_HIP(launch_reverse_fields(tf, plan->layout(), plan->buffers(), plan->shape, s),
              "launch reverse exchange");
     return DSDX_OK;
 }
@@ -479,16 +599,210 @@ int dsdx_forward_exchange(const dsdx_plan* plan, const dsdx_field* fields, uint3
 {
     if (!plan) return fail(DSDX_EINVAL, "null plan");
     if (!stream) return fail(DSDX_EINVAL, "device forward exchange requires an explicit stream");
-    dsdx::FieldParams fp;
+    FieldParams fp;
     if (int rc = make_field_params(fields, field_count, &fp)) return rc;
     if (!plan->built || field_count == 0) return DSDX_OK;
-    if (owned != plan->owned)
+    if (!single_matches(plan, owned))
         return fail(DSDX_EINVAL, "forward exchange owned count differs from the last device halo");
-    dsdx::timing::Scope timed(dsdx::timing::kForward, as_stream(stream));
-    DSDX_HIP(dsdx::launch_forward_fields(fp, static_cast<uint32_t>(owned), plan->view(), plan->bp,
-                                         as_stream(stream)),
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kForward, s);
+    DSDX_HIP(launch_forward_fields(fp, plan->layout(), plan->buffers(), plan->shape, s),
              "launch forward exchange");
     return DSDX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: R replicas in one set of SoA arrays, one launch per phase,
+// nothing waits on the host (graph-capturable).
+
+int dsdx_batch_create(const dsdx_replica* replicas, uint32_t count, void* stream, dsdx_batch** out)
+{
+    if (!out) return fail(DSDX_EINVAL, "null batch pointer");
+    *out = nullptr;
+    if (!replicas || count == 0) return fail(DSDX_EINVAL, "a batch needs at least one replica");
+    if (!stream) return fail(DSDX_EINVAL, "batch setup requires an explicit stream");
+    dsdx_batch* b = new (std::nothrow) dsdx_batch();
+    if (!b) return fail(DSDX_EDEVICE, "out of host memory");
+    int rc = shape_from_env(&b->shape);
+    b->desc.resize(count);
+    uint64_t ghost_list = 0;
+    for (uint32_t r = 0; r < count && !rc; ++r) {
+        const dsdx_replica& in = replicas[r];
+        ReplicaDesc& d = b->desc[r];
+        std::memset(&d, 0, sizeof(d));
+        rc = make_box_params(&in.box, &d.bp);
+        if (!rc && (in.owned > kMaxOwned || in.capacity < in.owned))
+            rc = fail(DSDX_EINVAL, "replica capacity below its owned count");
+        if (!rc && r > 0 && in.offset < replicas[r - 1].offset + replicas[r - 1].capacity)
+            rc = fail(DSDX_EINVAL, "replica storage regions must be ascending and disjoint");
+        d.offset = in.offset;
+        d.capacity = in.capacity;
+        d.owned = static_cast<uint32_t>(in.owned);
+        d.ghost_list_offset = ghost_list;
+        d.ghost_list_capacity = in.capacity - in.owned;
+        ghost_list += d.ghost_list_capacity;
+    }
+    if (!rc) {
+        number_chunks(b);
+        rc = reserve_buffers(b, std::max<uint64_t>(1, ghost_list));
+    }
+    if (!rc) rc = upload_table(b, as_stream(stream));
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return DSDX_OK;
+}
+
+int dsdx_batch_destroy(dsdx_batch* batch)
+{
+    delete batch;
+    return DSDX_OK;
+}
+
+int dsdx_batch_set_shape(dsdx_batch* batch, int32_t items, int32_t waves, int32_t scan_threads,
+                         void* stream)
+{
+    if (!batch) return fail(DSDX_EINVAL, "null batch");
+    if (!stream) return fail(DSDX_EINVAL, "batch setup requires an explicit stream");
+    const Shape s{items, waves, scan_threads};
+    if (!shape_supported(s)) return fail(DSDX_EINVAL, "unsupported launch shape");
+    const bool relayout = s.items != batch->shape.items;
+    batch->shape = s;
+    if (!relayout) return DSDX_OK;
+    batch->classified = batch->built = batch->totals_cached = false;
+    number_chunks(batch);
+    if (int rc = reserve_buffers(batch, batch->ghost_list_capacity)) return rc;
+    return upload_table(batch, as_stream(stream));
+}
+
+int dsdx_batch_get_shape(const dsdx_batch* batch, int32_t* items, int32_t* waves, int32_t* scan_threads)
+{
+    if (!batch) return fail(DSDX_EINVAL, "null batch");
+    if (items) *items = batch->shape.items;
+    if (waves) *waves = batch->shape.waves;
+    if (scan_threads) *scan_threads = batch->shape.scan_threads;
+    return DSDX_OK;
+}
+
+int dsdx_batch_migrate_wrap(const dsdx_batch* batch, double* x, double* y, double* z, void* stream)
+{
+    if (!batch) return fail(DSDX_EINVAL, "null batch");
+    if (!stream) return fail(DSDX_EINVAL, "device migration requires an explicit stream");
+    if (!x || !y || !z) return fail(DSDX_EINVAL, "incomplete device particle view");
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kMigrate, s);
+    DSDX_HIP(launch_migrate_wrap(x, y, z, batch->layout(), batch->shape, s), "launch migrate wrap");
+    return DSDX_OK;
+}
+
+int dsdx_batch_halo_build(dsdx_batch* batch, const dsdx_particles* particles, const dsdx_field* fields,
+                          uint32_t field_count, void* stream)
+{
+    if (!batch) return fail(DSDX_EINVAL, "null batch");
+    if (!stream) return fail(DSDX_EINVAL, "device halo requires an explicit stream");
+    if (int rc = check_particles(particles, 1)) return rc;
+    FieldParams fp;
+    if (int rc = make_field_params(fields, field_count, &fp)) return rc;
+    const hipStream_t s = as_stream(stream);
+    batch->totals_cached = false;
+    if (int rc = run_classify(batch, particles, s)) return rc;
+    if (int rc = run_scatter(batch, particles, fp, s)) return rc;
+    batch->built = true;
+    return DSDX_OK;
+}
+
+int dsdx_batch_refresh(dsdx_batch* batch, const dsdx_particles* particles, const dsdx_field* fields,
+                       uint32_t field_count, int32_t positions_only, void* stream)
+{
+    if (!batch || !batch->built) return fail(DSDX_ESTATE, "no device halo plan to refresh");
+    if (!stream) return fail(DSDX_EINVAL, "device halo requires an explicit stream");
+    if (int rc = check_particles(particles, 1)) return rc;
+    FieldParams fp;
+    if (int rc = make_field_params(fields, field_count, &fp)) return rc;
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kRefresh, s);
+    DSDX_HIP(launch_refresh(*particles, fp, positions_only ? 1 : 0, batch->layout(), batch->buffers(),
+                            batch->shape, s),
+             "launch halo refresh");
+    return DSDX_OK;
+}
+
+int dsdx_batch_reverse_halo(const dsdx_batch* batch, double* fx, double* fy, double* fz, void* stream)
+{
+    if (!batch) return fail(DSDX_EINVAL, "null batch");
+    if (!stream) return fail(DSDX_EINVAL, "device reverse halo requires an explicit stream");
+    if (!fx || !fy || !fz) return fail(DSDX_EINVAL, "invalid device reverse-halo view");
+    if (!batch->built) return DSDX_OK;
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kReverse, s);
+    DSDX_HIP(launch_reverse_forces(fx, fy, fz, batch->layout(), batch->buffers(), batch->shape, s),
+             "launch reverse halo");
+    return DSDX_OK;
+}
+
+int dsdx_batch_reverse_exchange(const dsdx_batch* batch, const dsdx_field* fields, uint32_t field_count,
+                                void* stream)
+{
+    if (!batch) return fail(DSDX_EINVAL, "null batch");
+    if (!stream) return fail(DSDX_EINVAL, "device reverse exchange requires an explicit stream");
+    TypedFields tf;
+    if (int rc = make_typed_fields(fields, field_count, &tf)) return rc;
+    if (!batch->built || field_count == 0) return DSDX_OK;
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kReverseFields, s);
+    DSDX_HIP(launch_reverse_fields(tf, batch->layout(), batch->buffers(), batch->shape, s),
+             "launch reverse exchange");
+    return DSDX_OK;
+}
+
+int dsdx_batch_forward_exchange(const dsdx_batch* batch, const dsdx_field* fields, uint32_t field_count,
+                                void* stream)
+{
+    if (!batch) return fail(DSDX_EINVAL, "null batch");
+    if (!stream) return fail(DSDX_EINVAL, "device forward exchange requires an explicit stream");
+    FieldParams fp;
+    if (int rc = make_field_params(fields, field_count, &fp)) return rc;
+    if (!batch->built || field_count == 0) return DSDX_OK;
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kForward, s);
+    DSDX_HIP(launch_forward_fields(fp, batch->layout(), batch->buffers(), batch->shape, s),
+             "launch forward exchange");
+    return DSDX_OK;
+}
+
+int dsdx_batch_ghost_owners(const dsdx_batch* batch, int32_t* owners, void* stream)
+{
+    if (!batch) return fail(DSDX_EINVAL, "null batch");
+    if (!stream || !owners) return fail(DSDX_EINVAL, "ghost owner query needs output and stream");
+    if (!batch->built) return DSDX_OK;
+    const hipStream_t s = as_stream(stream);
+    DSDX_HIP(launch_ghost_owners(owners, batch->layout(), batch->buffers(), batch->shape, s),
+             "launch ghost owners");
+    return DSDX_OK;
+}
+
+int dsdx_batch_counts(const dsdx_batch* batch, uint32_t* ghosts, uint32_t* owners, uint32_t* status,
+                      void* stream)
+{
+    if (!batch) return fail(DSDX_EINVAL, "null batch");
+    if (!stream) return fail(DSDX_EINVAL, "batch counts require an explicit stream");
+    Plan* plan = const_cast<dsdx_batch*>(batch);
+    if (!plan->classified) return fail(DSDX_ESTATE, "no halo has been built");
+    if (int rc = read_totals(plan, as_stream(stream))) return rc;
+    for (size_t r = 0; r < plan->desc.size(); ++r) {
+        const uint32_t* t = plan->host_totals + r * kTotals;
+        if (ghosts) ghosts[r] = t[kColImages];
+        if (owners) owners[r] = t[kColOwners];
+        if (status) status[r] = t[kStatusWord];
+    }
+    return DSDX_OK;
+}
+
+const uint32_t* dsdx_batch_device_totals(const dsdx_batch* batch)
+{
+    return batch ? batch->totals.as<uint32_t>() : nullptr;
 }
 
 }  // extern "C"

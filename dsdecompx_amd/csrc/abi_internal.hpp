@@ -1,0 +1,103 @@
+// abi_internal.hpp -- host-side plan state shared by the C ABI (abi.cpp) and
+// the autotuner (autotune.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "dsdx_hip.h"
+
+namespace dsdx::abi {
+
+int fail(int code, const std::string& message);
+int hip_fail(hipError_t status, const char* operation);
+
+#define DSDX_HIP(call, what)                                                  \
+    do {                                                                      \
+        const hipError_t status_ = (call);                                    \
+        if (status_ != hipSuccess) return ::dsdx::abi::hip_fail(status_, what); \
+    } while (0)
+
+// Device allocation that grows by 1.5x (at least 4 KiB) and keeps nothing on
+// growth (src/cuda/cuda_buffer.cpp:10-54 keeps the contents; no caller here
+// needs them).  Waits for the device before freeing the old block, which may
+// still be read by queued kernels.
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    hipError_t reserve(size_t need);
+    template <class T> T* as() const { return static_cast<T*>(ptr); }
+    ~DeviceBuffer();
+};
+
+// A device halo plan over R replicas (R = 1 for the single-box entry points).
+struct Plan {
+    Shape shape;
+    std::vector<ReplicaDesc> desc;        // host copy of the table on the device
+    std::vector<uint32_t> chunk_replica;  // host copy (R > 1 only)
+    DeviceBuffer d_desc, d_chunk_replica;
+    bool desc_on_device = false;
+    uint32_t chunks = 0;
+    uint32_t max_replica_chunks = 0;
+
+    bool classified = false;   // classify + scan of the current table are on the device
+    bool built = false;        // ghosts and owner map written: reverse/forward usable
+    bool totals_cached = false;
+
+    DeviceBuffer entries, counts, prefix, totals, owner_index, owner_code, image_start, ghost_list;
+    uint64_t ghost_list_capacity = 0;   // elements
+    uint32_t* host_totals = nullptr;    // pinned copy of totals (R x kTotals)
+    size_t host_totals_words = 0;
+
+    Plan() = default;
+    Plan(const Plan&) = delete;
+    Plan& operator=(const Plan&) = delete;
+    ~Plan()
+    {
+        if (host_totals) (void)hipHostFree(host_totals);
+    }
+
+    uint32_t chunk_size() const { return static_cast<uint32_t>(shape.items) * kWave; }
+
+    Layout layout() const
+    {
+        Layout L;
+        L.desc = d_desc.as<ReplicaDesc>();
+        L.chunk_replica = desc.size() > 1 ? d_chunk_replica.as<uint32_t>() : nullptr;
+        L.replicas = static_cast<uint32_t>(desc.size());
+        L.chunks = chunks;
+        L.max_replica_chunks = max_replica_chunks;
+        L.chunk_size = chunk_size();
+        return L;
+    }
+
+    PlanBuffers buffers() const
+    {
+        return {entries.as<uint32_t>(), counts.as<uint32_t>(), prefix.as<uint32_t>(),
+                totals.as<uint32_t>(), owner_index.as<uint32_t>(), owner_code.as<uint8_t>(),
+                image_start.as<uint32_t>(), ghost_list.as<uint32_t>()};
+    }
+};
+
+int make_box_params(const dsdx_box* box, BoxParams* bp);
+int make_field_params(const dsdx_field* fields, uint32_t count, FieldParams* fp);
+int make_typed_fields(const dsdx_field* fields, uint32_t count, TypedFields* tf);
+int check_particles(const dsdx_particles* p, uint64_t owned);
+int reserve_buffers(Plan* plan, uint64_t ghost_list_elems);
+void number_chunks(Plan* plan);
+int upload_table(Plan* plan, hipStream_t stream);
+int set_single(Plan* plan, const BoxParams& bp, uint64_t owned, uint64_t capacity,
+               uint64_t ghost_list_elems, hipStream_t stream);
+int read_totals(Plan* plan, hipStream_t stream);
+int run_classify(Plan* plan, const dsdx_particles* p, hipStream_t s);
+int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipStream_t s);
+int shape_from_env(Shape* s);
+
+}  // namespace dsdx::abi

@@ -1,19 +1,25 @@
 // device.hpp -- shared host/device definitions of the gfx950 halo kernels.
 //
-// Layout of a halo plan in HBM (all u32 unless noted), for a box of `owned`
-// particles split into `chunks` chunks of kChunk consecutive particles:
-//   entries[chunks * kChunk]   boundary particles of each chunk, compacted in
-//                              owned order: (index in chunk << 8) | slab code
-//   counts[kCols][chunks]      per chunk: send count of each of the 26 slots,
-//                              boundary owners, images (column-major so the
-//                              scan reads each column contiguously)
-//   prefix[kCols][chunks]      exclusive scan of counts over chunks
-//   totals[kCols + 2]          column totals, then the overflow status word
-//   owner_index[owned]         owner u -> owned index (first U entries used)
-//   owner_code[owned] (u8)     owner u -> slab code
-//   image_start[owned]         owner u -> first entry in ghost_list
-//   ghost_list[capacity]       per owner, its ghosts (index relative to
-//                              `owned`) in ascending send-slot order
+// Every launch covers a batch of R independent boxes ("replicas"); a single
+// box is a batch of one.  Replica r owns the particle slots
+// [offset_r, offset_r + capacity_r) of the SoA arrays (owned first, then its
+// ghosts) and the chunks [first_chunk_r, first_chunk_r + chunks_r) of the
+// global chunk numbering; a chunk is kItems * 64 consecutive owned particles
+// of one replica, processed by one wave.
+//
+// Plan buffers in HBM (u32 unless noted), C = total chunks, K = chunk size:
+//   entries[C * K]          per chunk, its boundary particles compacted in
+//                           owned order: (index in chunk << 8) | slab code
+//   counts[kCols][C]        per chunk: send count of each of the 26 slots,
+//                           boundary owners, images (column-major)
+//   prefix[kCols][C]        exclusive scan of counts over each replica's chunks
+//   totals[R][kTotals]      per replica: column totals, then the status word
+//   owner_index[C * K]      owner slot first_chunk_r * K + u -> owned index
+//   owner_code[C * K] (u8)  ... -> slab code
+//   image_start[C * K]      ... -> first entry of its ghosts in ghost_list
+//   ghost_list[...]         per replica at ghost_list_offset_r: per owner its
+//                           ghosts (index relative to the replica's owned
+//                           count) in ascending send-slot order
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,15 +35,10 @@ constexpr int kSlots = 26;
 constexpr int kCols = 28;          // 26 slots + owners + images
 constexpr int kColOwners = 26;
 constexpr int kColImages = 27;
-constexpr int kStatusWord = kCols; // totals[kCols]: overflow bits
+constexpr int kStatusWord = kCols; // totals[r][kStatusWord]: overflow bits
+constexpr int kTotals = 32;        // words per replica in totals
 constexpr uint32_t kOverflowStorage = 1u;
 constexpr uint32_t kOverflowGhostList = 2u;
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / kWave;
-// Items per lane of the classify and scatter passes; a chunk is one wave's
-// contiguous share of kWave * kItems particles.
-constexpr int kItems = 8;
-constexpr int kChunk = kWave * kItems;
 
 // Slab code: bit 2a = low slab of axis a (x < lower+width), bit 2a+1 = high
 // slab (x > upper-width).  lut[code] is the set of active send slots whose
@@ -53,6 +54,46 @@ struct BoxParams {
     int32_t pad;
 };
 
+struct ReplicaDesc {
+    BoxParams bp;
+    uint64_t offset;               // first particle slot of the replica
+    uint64_t capacity;             // particle slots (owned + ghosts)
+    uint64_t ghost_list_offset;
+    uint64_t ghost_list_capacity;
+    uint32_t owned;
+    uint32_t first_chunk;
+    uint32_t chunks;
+    uint32_t pad;
+};
+
+// Launch shape of the classify / scatter / owner-major kernels: items per
+// lane (the chunk is kWave * items particles) and waves per block.
+struct Shape {
+    int items = 8;
+    int waves = 4;
+    int scan_threads = 1024;
+};
+
+struct Layout {
+    const ReplicaDesc* desc;        // device, one per replica
+    const uint32_t* chunk_replica;  // device, replica of each chunk; null for one replica
+    uint32_t replicas;
+    uint32_t chunks;                // total over all replicas
+    uint32_t max_replica_chunks;    // largest chunks_r (scan work per block)
+    uint32_t chunk_size;            // kWave * items
+};
+
+struct PlanBuffers {
+    uint32_t* entries;
+    uint32_t* counts;
+    uint32_t* prefix;
+    uint32_t* totals;
+    uint32_t* owner_index;
+    uint8_t* owner_code;
+    uint32_t* image_start;
+    uint32_t* ghost_list;
+};
+
 struct FieldParams {
     uint32_t count;
     uint32_t words[DSDX_MAX_FIELDS];   // 32-bit words per particle
@@ -65,14 +106,6 @@ struct TypedFields {
     int32_t op[DSDX_MAX_FIELDS];
     uint32_t components[DSDX_MAX_FIELDS];
     void* data[DSDX_MAX_FIELDS];
-};
-
-struct PlanView {
-    const uint32_t* totals;
-    const uint32_t* owner_index;
-    const uint8_t* owner_code;
-    const uint32_t* image_start;
-    const uint32_t* ghost_list;
 };
 
 }  // namespace dsdx

@@ -1,7 +1,13 @@
 // halo_kernels.hip -- gfx950 kernels of the loopback particle-communication
-// path.  Bandwidth-bound stream compaction; no MFMA.  Built with
+// path.  Bandwidth-bound stream compaction and gathers; no MFMA.  Built with
 // -ffp-contract=off: every FP64 operation is separately rounded, as in the
 // reference Host build, so positions and sums are bit-identical to it.
+//
+// Every kernel walks a batch of replicas (device/halo layout in device.hpp);
+// one wave owns one chunk of one replica, so replica parameters are
+// wave-uniform and come in through scalar loads.
+#include <type_traits>
+
 #include "device.hpp"
 #include "kernels.hpp"
 
@@ -9,6 +15,8 @@ namespace dsdx {
 namespace {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Number of set bits of `mask` below this lane.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
@@ -35,13 +43,19 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v)
     return v;
 }
 
-__device__ __forceinline__ uint32_t uniform(uint32_t v)
+// The chunk of this wave and its replica (both wave-uniform).
+template <int WAVES>
+__device__ __forceinline__ uint32_t wave_chunk()
 {
-    return __builtin_amdgcn_readfirstlane(v);
+    return uniform(blockIdx.x * WAVES + threadIdx.x / kWave);
 }
 
-__device__ __forceinline__ uint32_t slab_code(double px, double py, double pz,
-                                              const BoxParams& bp)
+__device__ __forceinline__ uint32_t replica_of(const Layout& L, uint32_t chunk)
+{
+    return L.chunk_replica ? uniform(L.chunk_replica[chunk]) : 0u;
+}
+
+__device__ __forceinline__ uint32_t slab_code(double px, double py, double pz, const BoxParams& bp)
 {
     // Strict compares of src/halo.cpp:200-205 against thresholds computed on
     // the host exactly as lower+width and upper-width.
@@ -57,41 +71,51 @@ __device__ __forceinline__ uint32_t slab_code(double px, double py, double pz,
 // Pass 1: one wave per chunk.  Slab-classifies every owned particle,
 // compacts the boundary ones (non-empty slot set) into the chunk's entry
 // region in owned order, and counts per send slot through an LDS histogram of
-// slab codes.  Reads 24 B per particle, writes 4 B per boundary particle.
-__global__ __launch_bounds__(kBlock) void classify_kernel(
-    const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ z, uint32_t owned, BoxParams bp,
-    uint32_t* __restrict__ entries, uint32_t* __restrict__ counts, uint32_t chunks)
+// slab codes (one LDS atomic per boundary particle).  Reads 24 B per
+// particle, writes 4 B per boundary particle and 28 counters per chunk.
+template <int ITEMS, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    Layout L, PlanBuffers B)
 {
-    __shared__ uint32_t s_lut[64];
-    __shared__ uint32_t s_hist[kWavesPerBlock][64];
+    constexpr uint32_t K = ITEMS * kWave;
+    __shared__ uint32_t s_lut[WAVES][64];
+    __shared__ uint32_t s_hist[WAVES][64];
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t chunk = blockIdx.x * kWavesPerBlock + wave;
-    if (threadIdx.x < 64) s_lut[threadIdx.x] = bp.lut[threadIdx.x];
+    const uint32_t chunk = wave_chunk<WAVES>();
+    const bool live = chunk < L.chunks;
+    const ReplicaDesc* d = nullptr;
+    if (live) {
+        d = L.desc + replica_of(L, chunk);
+        s_lut[wave][lane] = d->bp.lut[lane];
+    }
     s_hist[wave][lane] = 0u;
     __syncthreads();
 
     uint32_t running = 0;
-    if (chunk < chunks) {
-        const uint64_t base = static_cast<uint64_t>(chunk) * kChunk;
-        double px[kItems], py[kItems], pz[kItems];
+    if (live) {
+        const uint32_t owned = d->owned;
+        const uint64_t off = d->offset;
+        const uint32_t first = (chunk - d->first_chunk) * K;
+        double px[ITEMS], py[ITEMS], pz[ITEMS];
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) {
-            const uint64_t i = base + static_cast<uint64_t>(k) * kWave + lane;
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t i = first + k * kWave + lane;
             const bool valid = i < owned;
-            px[k] = valid ? x[i] : 0.0;
-            py[k] = valid ? y[i] : 0.0;
-            pz[k] = valid ? z[i] : 0.0;
+            px[k] = valid ? x[off + i] : 0.0;
+            py[k] = valid ? y[off + i] : 0.0;
+            pz[k] = valid ? z[off + i] : 0.0;
         }
+        const uint64_t ebase = static_cast<uint64_t>(chunk) * K;
 #pragma unroll
-        for (int k = 0; k < kItems; ++k) {
-            const uint64_t i = base + static_cast<uint64_t>(k) * kWave + lane;
-            const uint32_t code = i < owned ? slab_code(px[k], py[k], pz[k], bp) : 0u;
-            const bool boundary = s_lut[code] != 0u;
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t i = first + k * kWave + lane;
+            const uint32_t code = i < owned ? slab_code(px[k], py[k], pz[k], d->bp) : 0u;
+            const bool boundary = s_lut[wave][code] != 0u;
             const uint64_t ballot = __ballot(boundary);
             if (boundary) {
-                entries[base + running + lanes_below(ballot)] =
+                B.entries[ebase + running + lanes_below(ballot)] =
                     ((static_cast<uint32_t>(k) * kWave + lane) << 8) | code;
                 atomicAdd(&s_hist[wave][code], 1u);
             }
@@ -99,37 +123,41 @@ __global__ __launch_bounds__(kBlock) void classify_kernel(
         }
     }
     __syncthreads();
-    if (chunk < chunks) {
+    if (live) {
         // lane s < 26: send count of slot s; lane 26: owners; lane 27: images
         uint32_t acc = 0;
         for (int c = 1; c < 64; ++c) {
             const uint32_t h = s_hist[wave][c];
-            const uint32_t m = s_lut[c];
+            const uint32_t m = s_lut[wave][c];
             if (lane < kSlots) acc += ((m >> lane) & 1u) ? h : 0u;
             else if (lane == kColImages) acc += h * static_cast<uint32_t>(__popc(m));
         }
         if (lane == kColOwners) acc = running;
-        if (lane < kCols) counts[static_cast<uint64_t>(lane) * chunks + chunk] = acc;
+        if (lane < kCols) B.counts[static_cast<uint64_t>(lane) * L.chunks + chunk] = acc;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Pass 2: exclusive scan of every count column over the chunks; one block of
-// 1024 threads per column.  totals[col] receives the column total.
-constexpr int kScanThreads = 1024;
+// Pass 2: exclusive scan of one count column over one replica's chunks; grid
+// (kCols, replicas).  totals[r][col] receives the column total.
+constexpr int kMaxScanThreads = 1024;
 
-__global__ __launch_bounds__(kScanThreads) void scan_kernel(
-    const uint32_t* __restrict__ counts, uint32_t* __restrict__ prefix,
-    uint32_t* __restrict__ totals, uint32_t chunks)
+__global__ __launch_bounds__(kMaxScanThreads) void scan_kernel(Layout L, PlanBuffers B)
 {
-    __shared__ uint32_t s_wave[kScanThreads / kWave];
+    __shared__ uint32_t s_wave[kMaxScanThreads / kWave];
     const uint32_t col = blockIdx.x;
+    const uint32_t r = blockIdx.y;
+    const ReplicaDesc* d = L.desc + r;
+    const uint32_t n = d->chunks;
+    const uint32_t threads = blockDim.x;
+    const uint32_t waves = threads / kWave;
     const uint32_t t = threadIdx.x;
-    const uint32_t per = (chunks + kScanThreads - 1) / kScanThreads;
-    const uint32_t begin = min(t * per, chunks);
-    const uint32_t end = min(begin + per, chunks);
-    const uint32_t* src = counts + static_cast<uint64_t>(col) * chunks;
-    uint32_t* dst = prefix + static_cast<uint64_t>(col) * chunks;
+    const uint32_t per = (n + threads - 1) / threads;
+    const uint32_t begin = min(t * per, n);
+    const uint32_t end = min(begin + per, n);
+    const uint64_t base = static_cast<uint64_t>(col) * L.chunks + d->first_chunk;
+    const uint32_t* src = B.counts + base;
+    uint32_t* dst = B.prefix + base;
     uint32_t sum = 0;
     for (uint32_t k = begin; k < end; ++k) sum += src[k];
     const uint32_t incl = wave_inclusive_sum(sum);
@@ -137,10 +165,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(
     if (lane_id() == kWave - 1) s_wave[wave] = incl;
     __syncthreads();
     if (wave == 0) {
-        const uint32_t w = lane_id() < kScanThreads / kWave ? s_wave[lane_id()] : 0u;
+        const uint32_t w = lane_id() < waves ? s_wave[lane_id()] : 0u;
         const uint32_t wi = wave_inclusive_sum(w);
-        if (lane_id() < kScanThreads / kWave) s_wave[lane_id()] = wi - w;
-        if (lane_id() == kScanThreads / kWave - 1) totals[col] = wi;
+        if (lane_id() < waves) s_wave[lane_id()] = wi - w;
+        if (lane_id() == waves - 1) B.totals[static_cast<uint64_t>(r) * kTotals + col] = wi;
     }
     __syncthreads();
     uint32_t run = s_wave[wave] + incl - sum;
@@ -149,7 +177,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(
         dst[k] = run;
         run += v;
     }
-    if (col == 0 && t == 0) totals[kStatusWord] = 0u;
+    if (col == 0 && t == 0) B.totals[static_cast<uint64_t>(r) * kTotals + kStatusWord] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -157,41 +185,42 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(
 // order; for each of its send slots s (ascending) a particle gets the ghost
 // at receive block 25-s, position prefix[s][chunk] + rank in the chunk.
 // Writes the 13 ghost arrays, the registered fields and the owner -> ghost
-// map.  The capacity test is uniform over the grid, so an overflow leaves
-// every ghost slot untouched.
-__device__ __forceinline__ void copy_field_words(const FieldParams& fp, uint64_t from,
-                                                 uint64_t to)
+// map.  The capacity test is uniform over a replica, so an overflowing
+// replica leaves every one of its ghost slots untouched.
+__device__ __forceinline__ void copy_field_words(const FieldParams& fp, uint64_t from, uint64_t to)
 {
     for (uint32_t f = 0; f < fp.count; ++f) {
         const uint32_t w = fp.words[f];
-        uint32_t* d = static_cast<uint32_t*>(fp.data[f]);
-        for (uint32_t k = 0; k < w; ++k) d[to * w + k] = d[from * w + k];
+        uint32_t* dd = static_cast<uint32_t*>(fp.data[f]);
+        for (uint32_t k = 0; k < w; ++k) dd[to * w + k] = dd[from * w + k];
     }
 }
 
-__global__ __launch_bounds__(kBlock) void scatter_kernel(
-    dsdx_particles p, uint32_t owned, BoxParams bp, FieldParams fp,
-    const uint32_t* __restrict__ entries, const uint32_t* __restrict__ counts,
-    const uint32_t* __restrict__ prefix, uint32_t* __restrict__ totals,
-    uint32_t* __restrict__ owner_index, uint8_t* __restrict__ owner_code,
-    uint32_t* __restrict__ image_start, uint32_t* __restrict__ ghost_list,
-    uint32_t chunks, uint64_t capacity, uint64_t ghost_list_capacity)
+template <int ITEMS, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void scatter_kernel(dsdx_particles p, FieldParams fp,
+                                                                Layout L, PlanBuffers B)
 {
-    __shared__ uint32_t s_lut[64];
-    if (threadIdx.x < 64) s_lut[threadIdx.x] = bp.lut[threadIdx.x];
-    __syncthreads();
+    constexpr uint32_t K = ITEMS * kWave;
+    __shared__ uint32_t s_lut[WAVES][64];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t chunk = wave_chunk<WAVES>();
+    if (chunk >= L.chunks) return;
+    const uint32_t r = replica_of(L, chunk);
+    const ReplicaDesc* d = L.desc + r;
+    uint32_t* totals = B.totals + static_cast<uint64_t>(r) * kTotals;
+    s_lut[wave][lane] = d->bp.lut[lane];
 
-    const uint32_t ghosts = totals[kColImages];
+    const uint32_t owned = d->owned;
+    const uint32_t ghosts = uniform(totals[kColImages]);
     uint32_t overflow = 0;
-    if (static_cast<uint64_t>(owned) + ghosts > capacity) overflow |= kOverflowStorage;
-    if (ghosts > ghost_list_capacity) overflow |= kOverflowGhostList;
+    if (static_cast<uint64_t>(owned) + ghosts > d->capacity) overflow |= kOverflowStorage;
+    if (ghosts > d->ghost_list_capacity) overflow |= kOverflowGhostList;
     if (overflow) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) totals[kStatusWord] = overflow;
+        if (chunk == d->first_chunk && lane == 0) totals[kStatusWord] = overflow;
         return;
     }
-    const uint32_t lane = lane_id();
-    const uint32_t chunk = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    if (chunk >= chunks) return;
+    __builtin_amdgcn_wave_barrier();
 
     // Receive block m holds send slot 25-m: recv_base[m] = sum of totals of
     // slots 25..26-m.  Lane s ends with slot_base = recv_base[25-s] + prefix.
@@ -199,21 +228,26 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
     const uint32_t recv_base = wave_inclusive_sum(block_size) - block_size;
     const uint32_t mirrored = __shfl(recv_base, lane < kSlots ? kSlots - 1 - lane : 0, kWave);
     const uint32_t slot_base =
-        lane < kSlots ? mirrored + prefix[static_cast<uint64_t>(lane) * chunks + chunk] : 0u;
-    uint32_t slot_run = 0;   // lane s: ghosts of slot s already written in this chunk
+        lane < kSlots ? mirrored + B.prefix[static_cast<uint64_t>(lane) * L.chunks + chunk] : 0u;
+    uint32_t slot_run = 0;   // lane s: ghosts of slot s already written from this chunk
 
-    const uint32_t n_boundary = counts[static_cast<uint64_t>(kColOwners) * chunks + chunk];
-    const uint32_t owner_base = prefix[static_cast<uint64_t>(kColOwners) * chunks + chunk];
-    uint32_t image_run = prefix[static_cast<uint64_t>(kColImages) * chunks + chunk];
-    const uint64_t chunk_first = static_cast<uint64_t>(chunk) * kChunk;
+    const uint32_t n_boundary = uniform(B.counts[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
+    const uint32_t owner_base = uniform(B.prefix[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
+    uint32_t image_run = uniform(B.prefix[static_cast<uint64_t>(kColImages) * L.chunks + chunk]);
+    const uint64_t off = d->offset;
+    const uint32_t first = (chunk - d->first_chunk) * K;
+    const uint64_t ebase = static_cast<uint64_t>(chunk) * K;
+    const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * K + owner_base;
+    uint32_t* glist = B.ghost_list + d->ghost_list_offset;
 
     for (uint32_t e0 = 0; e0 < n_boundary; e0 += kWave) {
         const uint32_t e = e0 + lane;
         const bool valid = e < n_boundary;
-        const uint32_t entry = valid ? entries[chunk_first + e] : 0u;
+        const uint32_t entry = valid ? B.entries[ebase + e] : 0u;
         const uint32_t code = entry & 63u;
-        const uint64_t i = chunk_first + (entry >> 8);
-        const uint32_t slots = valid ? s_lut[code] : 0u;
+        const uint32_t il = first + (entry >> 8);
+        const uint64_t i = off + il;
+        const uint32_t slots = valid ? s_lut[wave][code] : 0u;
         const uint32_t images = static_cast<uint32_t>(__popc(slots));
         const uint32_t incl = wave_inclusive_sum(images);
         const uint32_t first_image = image_run + incl - images;
@@ -222,10 +256,10 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
         double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0, m = 0, q = 0;
         int32_t ty = 0, id = 0;
         if (valid) {
-            const uint32_t u = owner_base + e;
-            owner_index[u] = static_cast<uint32_t>(i);
-            owner_code[u] = static_cast<uint8_t>(code);
-            image_start[u] = first_image;
+            const uint64_t u = slot0 + e;
+            B.owner_index[u] = il;
+            B.owner_code[u] = static_cast<uint8_t>(code);
+            B.image_start[u] = first_image;
             px = p.x[i]; py = p.y[i]; pz = p.z[i];
             vx = p.vx[i]; vy = p.vy[i]; vz = p.vz[i];
             m = p.mass[i]; q = p.charge[i];
@@ -242,10 +276,10 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
                                   __builtin_amdgcn_readlane(slot_run, s);
             if (mine) {
                 const uint32_t rel = base + lanes_below(ballot);
-                const uint64_t j = static_cast<uint64_t>(owned) + rel;
-                p.x[j] = px + bp.shift[s][0];
-                p.y[j] = py + bp.shift[s][1];
-                p.z[j] = pz + bp.shift[s][2];
+                const uint64_t j = off + owned + rel;
+                p.x[j] = px + d->bp.shift[s][0];
+                p.y[j] = py + d->bp.shift[s][1];
+                p.z[j] = pz + d->bp.shift[s][2];
                 p.vx[j] = vx; p.vy[j] = vy; p.vz[j] = vz;
                 if (p.fx) p.fx[j] = 0.0;
                 if (p.fy) p.fy[j] = 0.0;
@@ -253,7 +287,7 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
                 p.mass[j] = m; p.charge[j] = q;
                 p.type[j] = ty; p.global_id[j] = id;
                 if (fp.count) copy_field_words(fp, i, j);
-                ghost_list[first_image + written] = rel;
+                glist[first_image + written] = rel;
                 ++written;
             }
             if (lane == s) slot_run += static_cast<uint32_t>(__popcll(ballot));
@@ -262,33 +296,54 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Owner-major kernels over the plan: thread per boundary owner, grid-stride.
-// Owners of the last build; zero when that build overflowed (its scatter
-// wrote no owner map), so the owner-major kernels then touch nothing.
-__device__ __forceinline__ uint32_t owner_count(const PlanView& v)
+// Owner-major kernels: one wave per chunk, one lane per boundary owner found
+// in that chunk.  A replica whose last build overflowed has no owner map and
+// is skipped.
+struct OwnerCursor {
+    const ReplicaDesc* d;
+    uint64_t slot0;      // owner slot of the chunk's first owner
+    uint32_t count;      // owners found in the chunk
+    bool live;
+};
+
+template <int ITEMS, int WAVES>
+__device__ __forceinline__ OwnerCursor owners_of_wave(const Layout& L, const PlanBuffers& B)
 {
-    return v.totals[kStatusWord] ? 0u : v.totals[kColOwners];
+    constexpr uint32_t K = ITEMS * kWave;
+    OwnerCursor c{nullptr, 0, 0, false};
+    const uint32_t chunk = wave_chunk<WAVES>();
+    if (chunk >= L.chunks) return c;
+    const uint32_t r = replica_of(L, chunk);
+    c.d = L.desc + r;
+    if (uniform(B.totals[static_cast<uint64_t>(r) * kTotals + kStatusWord])) return c;
+    c.count = uniform(B.counts[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
+    c.slot0 = static_cast<uint64_t>(c.d->first_chunk) * K +
+              uniform(B.prefix[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
+    c.live = true;
+    return c;
 }
 
 // reverse_halo: f[i] = ((f[i] + g_s1) + g_s2) + ... over the owner's send
 // slots s1 < s2 < ..., the order of the Host reduction (src/halo.cpp:1357-1370
-// for n ascending).  No atomics: each owner is summed by one thread.
-__global__ __launch_bounds__(kBlock) void reverse_forces_kernel(
-    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
-    uint32_t owned, PlanView v, BoxParams bp)
+// for n ascending).  No atomics: one lane sums one owner.
+template <int ITEMS, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void reverse_forces_kernel(
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz, Layout L,
+    PlanBuffers B)
 {
-    __shared__ uint32_t s_lut[64];
-    if (threadIdx.x < 64) s_lut[threadIdx.x] = bp.lut[threadIdx.x];
-    __syncthreads();
-    const uint32_t n_owners = owner_count(v);
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_owners;
-         u += gridDim.x * blockDim.x) {
-        const uint32_t i = v.owner_index[u];
-        const uint32_t first = v.image_start[u];
-        const uint32_t n_img = static_cast<uint32_t>(__popc(s_lut[v.owner_code[u]]));
+    const OwnerCursor c = owners_of_wave<ITEMS, WAVES>(L, B);
+    if (!c.live) return;
+    const uint64_t off = c.d->offset;
+    const uint64_t goff = off + c.d->owned;
+    const uint32_t* glist = B.ghost_list + c.d->ghost_list_offset;
+    for (uint32_t e = lane_id(); e < c.count; e += kWave) {
+        const uint64_t u = c.slot0 + e;
+        const uint64_t i = off + B.owner_index[u];
+        const uint32_t first = B.image_start[u];
+        const uint32_t n_img = static_cast<uint32_t>(__popc(c.d->bp.lut[B.owner_code[u]]));
         double ax = fx[i], ay = fy[i], az = fz[i];
         for (uint32_t k = 0; k < n_img; ++k) {
-            const uint64_t j = static_cast<uint64_t>(owned) + v.ghost_list[first + k];
+            const uint64_t j = goff + glist[first + k];
             ax = ax + fx[j];
             ay = ay + fy[j];
             az = az + fz[j];
@@ -309,60 +364,59 @@ __device__ __forceinline__ T reduce_value(T dst, T v, int32_t op)
 }
 
 template <class T>
-__device__ void reduce_field(T* data, uint32_t c, int32_t op, uint32_t i, uint32_t owned,
+__device__ void reduce_field(T* data, uint32_t c, int32_t op, uint64_t i, uint64_t goff,
                              const uint32_t* ghosts, uint32_t n_img)
 {
     for (uint32_t e = 0; e < c; ++e) {
-        T acc = data[static_cast<uint64_t>(i) * c + e];
-        for (uint32_t k = 0; k < n_img; ++k) {
-            const uint64_t j = static_cast<uint64_t>(owned) + ghosts[k];
-            acc = reduce_value<T>(acc, data[j * c + e], op);
-        }
-        data[static_cast<uint64_t>(i) * c + e] = acc;
+        T acc = data[i * c + e];
+        for (uint32_t k = 0; k < n_img; ++k) acc = reduce_value<T>(acc, data[(goff + ghosts[k]) * c + e], op);
+        data[i * c + e] = acc;
     }
 }
 
 // Integer sums wrap as unsigned (two's complement), like the Host's
 // destination += value on int32/int64.
-__global__ __launch_bounds__(kBlock) void reverse_fields_kernel(
-    TypedFields tf, uint32_t owned, PlanView v, BoxParams bp)
+template <int ITEMS, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void reverse_fields_kernel(TypedFields tf, Layout L,
+                                                                       PlanBuffers B)
 {
-    __shared__ uint32_t s_lut[64];
-    if (threadIdx.x < 64) s_lut[threadIdx.x] = bp.lut[threadIdx.x];
-    __syncthreads();
-    const uint32_t n_owners = owner_count(v);
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_owners;
-         u += gridDim.x * blockDim.x) {
-        const uint32_t i = v.owner_index[u];
-        const uint32_t* ghosts = v.ghost_list + v.image_start[u];
-        const uint32_t n_img = static_cast<uint32_t>(__popc(s_lut[v.owner_code[u]]));
+    const OwnerCursor c = owners_of_wave<ITEMS, WAVES>(L, B);
+    if (!c.live) return;
+    const uint64_t off = c.d->offset;
+    const uint64_t goff = off + c.d->owned;
+    const uint32_t* glist = B.ghost_list + c.d->ghost_list_offset;
+    for (uint32_t e = lane_id(); e < c.count; e += kWave) {
+        const uint64_t u = c.slot0 + e;
+        const uint64_t i = off + B.owner_index[u];
+        const uint32_t* ghosts = glist + B.image_start[u];
+        const uint32_t n_img = static_cast<uint32_t>(__popc(c.d->bp.lut[B.owner_code[u]]));
         for (uint32_t f = 0; f < tf.count; ++f) {
-            const uint32_t c = tf.components[f];
+            const uint32_t cc = tf.components[f];
             const int32_t op = tf.op[f];
             switch (tf.type[f]) {
             case DSDX_INT32:
                 if (op == DSDX_SUM)
-                    reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), c, op, i, owned, ghosts, n_img);
+                    reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
                 else
-                    reduce_field<int32_t>(static_cast<int32_t*>(tf.data[f]), c, op, i, owned, ghosts, n_img);
+                    reduce_field<int32_t>(static_cast<int32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
                 break;
             case DSDX_INT64:
                 if (op == DSDX_SUM)
-                    reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), c, op, i, owned, ghosts, n_img);
+                    reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
                 else
-                    reduce_field<int64_t>(static_cast<int64_t*>(tf.data[f]), c, op, i, owned, ghosts, n_img);
+                    reduce_field<int64_t>(static_cast<int64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
                 break;
             case DSDX_UINT32:
-                reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), c, op, i, owned, ghosts, n_img);
+                reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
                 break;
             case DSDX_UINT64:
-                reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), c, op, i, owned, ghosts, n_img);
+                reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
                 break;
             case DSDX_FLOAT32:
-                reduce_field<float>(static_cast<float*>(tf.data[f]), c, op, i, owned, ghosts, n_img);
+                reduce_field<float>(static_cast<float*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
                 break;
             default:
-                reduce_field<double>(static_cast<double*>(tf.data[f]), c, op, i, owned, ghosts, n_img);
+                reduce_field<double>(static_cast<double*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
                 break;
             }
         }
@@ -370,25 +424,26 @@ __global__ __launch_bounds__(kBlock) void reverse_fields_kernel(
 }
 
 // forward_exchange: owner values copied into each of its ghost slots.
-__global__ __launch_bounds__(kBlock) void forward_fields_kernel(
-    FieldParams fp, uint32_t owned, PlanView v, BoxParams bp)
+template <int ITEMS, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void forward_fields_kernel(FieldParams fp, Layout L,
+                                                                       PlanBuffers B)
 {
-    __shared__ uint32_t s_lut[64];
-    if (threadIdx.x < 64) s_lut[threadIdx.x] = bp.lut[threadIdx.x];
-    __syncthreads();
-    const uint32_t n_owners = owner_count(v);
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_owners;
-         u += gridDim.x * blockDim.x) {
-        const uint32_t i = v.owner_index[u];
-        const uint32_t* ghosts = v.ghost_list + v.image_start[u];
-        const uint32_t n_img = static_cast<uint32_t>(__popc(s_lut[v.owner_code[u]]));
+    const OwnerCursor c = owners_of_wave<ITEMS, WAVES>(L, B);
+    if (!c.live) return;
+    const uint64_t off = c.d->offset;
+    const uint64_t goff = off + c.d->owned;
+    const uint32_t* glist = B.ghost_list + c.d->ghost_list_offset;
+    for (uint32_t e = lane_id(); e < c.count; e += kWave) {
+        const uint64_t u = c.slot0 + e;
+        const uint64_t i = off + B.owner_index[u];
+        const uint32_t* ghosts = glist + B.image_start[u];
+        const uint32_t n_img = static_cast<uint32_t>(__popc(c.d->bp.lut[B.owner_code[u]]));
         for (uint32_t f = 0; f < fp.count; ++f) {
             const uint32_t w = fp.words[f];
-            uint32_t* d = static_cast<uint32_t*>(fp.data[f]);
+            uint32_t* dd = static_cast<uint32_t*>(fp.data[f]);
             for (uint32_t k = 0; k < w; ++k) {
-                const uint32_t value = d[static_cast<uint64_t>(i) * w + k];
-                for (uint32_t g = 0; g < n_img; ++g)
-                    d[(static_cast<uint64_t>(owned) + ghosts[g]) * w + k] = value;
+                const uint32_t value = dd[i * w + k];
+                for (uint32_t g = 0; g < n_img; ++g) dd[(goff + ghosts[g]) * w + k] = value;
             }
         }
     }
@@ -396,27 +451,29 @@ __global__ __launch_bounds__(kBlock) void forward_fields_kernel(
 
 // Reused-plan refresh: each ghost re-reads its owner; the slot of the k-th
 // ghost of an owner is the k-th set bit of its slot set.
-__global__ __launch_bounds__(kBlock) void refresh_kernel(
-    dsdx_particles p, FieldParams fp, uint32_t owned, int32_t positions_only,
-    PlanView v, BoxParams bp)
+template <int ITEMS, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(dsdx_particles p, FieldParams fp,
+                                                                int32_t positions_only, Layout L,
+                                                                PlanBuffers B)
 {
-    __shared__ uint32_t s_lut[64];
-    if (threadIdx.x < 64) s_lut[threadIdx.x] = bp.lut[threadIdx.x];
-    __syncthreads();
-    const uint32_t n_owners = owner_count(v);
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_owners;
-         u += gridDim.x * blockDim.x) {
-        const uint32_t i = v.owner_index[u];
-        const uint32_t* ghosts = v.ghost_list + v.image_start[u];
-        uint32_t slots = s_lut[v.owner_code[u]];
+    const OwnerCursor c = owners_of_wave<ITEMS, WAVES>(L, B);
+    if (!c.live) return;
+    const uint64_t off = c.d->offset;
+    const uint64_t goff = off + c.d->owned;
+    const uint32_t* glist = B.ghost_list + c.d->ghost_list_offset;
+    for (uint32_t e = lane_id(); e < c.count; e += kWave) {
+        const uint64_t u = c.slot0 + e;
+        const uint64_t i = off + B.owner_index[u];
+        const uint32_t* ghosts = glist + B.image_start[u];
+        uint32_t slots = c.d->bp.lut[B.owner_code[u]];
         const double px = p.x[i], py = p.y[i], pz = p.z[i];
         for (uint32_t k = 0; slots; ++k) {
             const uint32_t s = static_cast<uint32_t>(__builtin_ctz(slots));
             slots &= slots - 1u;
-            const uint64_t j = static_cast<uint64_t>(owned) + ghosts[k];
-            p.x[j] = px + bp.shift[s][0];
-            p.y[j] = py + bp.shift[s][1];
-            p.z[j] = pz + bp.shift[s][2];
+            const uint64_t j = goff + ghosts[k];
+            p.x[j] = px + c.d->bp.shift[s][0];
+            p.y[j] = py + c.d->bp.shift[s][1];
+            p.z[j] = pz + c.d->bp.shift[s][2];
             if (p.fx) p.fx[j] = 0.0;
             if (p.fy) p.fy[j] = 0.0;
             if (p.fz) p.fz[j] = 0.0;
@@ -430,19 +487,20 @@ __global__ __launch_bounds__(kBlock) void refresh_kernel(
     }
 }
 
-__global__ __launch_bounds__(kBlock) void ghost_owners_kernel(
-    int32_t* __restrict__ out, PlanView v, BoxParams bp)
+// Owner of every ghost: out[offset_r + rel] = owned index (replica-local).
+template <int ITEMS, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void ghost_owners_kernel(int32_t* __restrict__ out,
+                                                                     Layout L, PlanBuffers B)
 {
-    __shared__ uint32_t s_lut[64];
-    if (threadIdx.x < 64) s_lut[threadIdx.x] = bp.lut[threadIdx.x];
-    __syncthreads();
-    const uint32_t n_owners = owner_count(v);
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n_owners;
-         u += gridDim.x * blockDim.x) {
-        const uint32_t i = v.owner_index[u];
-        const uint32_t* ghosts = v.ghost_list + v.image_start[u];
-        const uint32_t n_img = static_cast<uint32_t>(__popc(s_lut[v.owner_code[u]]));
-        for (uint32_t k = 0; k < n_img; ++k) out[ghosts[k]] = static_cast<int32_t>(i);
+    const OwnerCursor c = owners_of_wave<ITEMS, WAVES>(L, B);
+    if (!c.live) return;
+    const uint32_t* glist = B.ghost_list + c.d->ghost_list_offset;
+    for (uint32_t e = lane_id(); e < c.count; e += kWave) {
+        const uint64_t u = c.slot0 + e;
+        const uint32_t i = B.owner_index[u];
+        const uint32_t* ghosts = glist + B.image_start[u];
+        const uint32_t n_img = static_cast<uint32_t>(__popc(c.d->bp.lut[B.owner_code[u]]));
+        for (uint32_t k = 0; k < n_img; ++k) out[c.d->offset + ghosts[k]] = static_cast<int32_t>(i);
     }
 }
 
@@ -450,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void ghost_owners_kernel(
 // migrate at P = 1: periodic wrap in place (src/migration.cpp:106-113):
 // v = x - lower; v -= floor(v / extent) * extent; v + lower, each operation
 // rounded on its own (explicit _rn intrinsics, never fused).  Only a changed
-// coordinate is stored.
+// coordinate is stored.  One wave per chunk, loads before compute.
 __device__ __forceinline__ double wrap(double value, double lower, double extent)
 {
     const double v = __dsub_rn(value, lower);
@@ -458,130 +516,177 @@ __device__ __forceinline__ double wrap(double value, double lower, double extent
     return __dadd_rn(__dsub_rn(v, __dmul_rn(q, extent)), lower);
 }
 
-constexpr int kWrapItems = 4;
-
-__global__ __launch_bounds__(kBlock) void migrate_wrap_kernel(
-    double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
-    uint64_t owned, BoxParams bp)
+template <int ITEMS, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void migrate_wrap_kernel(
+    double* __restrict__ x, double* __restrict__ y, double* __restrict__ z, Layout L)
 {
+    constexpr uint32_t K = ITEMS * kWave;
+    const uint32_t chunk = wave_chunk<WAVES>();
+    if (chunk >= L.chunks) return;
+    const ReplicaDesc* d = L.desc + replica_of(L, chunk);
+    const uint32_t owned = d->owned;
+    const uint64_t off = d->offset;
+    const uint32_t first = (chunk - d->first_chunk) * K;
     double* axis[3] = {x, y, z};
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t i0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-         i0 < owned; i0 += stride * kWrapItems) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (!bp.periodic[a]) continue;
-            double* d = axis[a];
-            double val[kWrapItems];
+    for (int a = 0; a < 3; ++a) {
+        if (!d->bp.periodic[a]) continue;
+        double* v = axis[a] + off;
+        const double lower = d->bp.lower[a];
+        const double extent = d->bp.extent[a];
+        double val[ITEMS];
 #pragma unroll
-            for (int k = 0; k < kWrapItems; ++k) {
-                const uint64_t i = i0 + k * stride;
-                val[k] = i < owned ? d[i] : 0.0;
-            }
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t i = first + k * kWave + lane_id();
+            val[k] = i < owned ? v[i] : 0.0;
+        }
 #pragma unroll
-            for (int k = 0; k < kWrapItems; ++k) {
-                const uint64_t i = i0 + k * stride;
-                const double w = wrap(val[k], bp.lower[a], bp.extent[a]);
-                if (i < owned && __double_as_longlong(w) != __double_as_longlong(val[k]))
-                    d[i] = w;
-            }
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t i = first + k * kWave + lane_id();
+            const double w = wrap(val[k], lower, extent);
+            if (i < owned && __double_as_longlong(w) != __double_as_longlong(val[k])) v[i] = w;
         }
     }
 }
 
+// ---------------------------------------------------------------------------
+// Shape dispatch: items per lane 4 / 8 / 16, waves per block 2 / 4 / 8.
+template <class F>
+hipError_t dispatch(const Shape& s, F&& f)
+{
+    auto by_waves = [&](auto items) -> hipError_t {
+        switch (s.waves) {
+        case 2: return f(items, std::integral_constant<int, 2>{});
+        case 4: return f(items, std::integral_constant<int, 4>{});
+        case 8: return f(items, std::integral_constant<int, 8>{});
+        default: return hipErrorInvalidValue;
+        }
+    };
+    switch (s.items) {
+    case 4: return by_waves(std::integral_constant<int, 4>{});
+    case 8: return by_waves(std::integral_constant<int, 8>{});
+    case 16: return by_waves(std::integral_constant<int, 16>{});
+    default: return hipErrorInvalidValue;
+    }
+}
+
+uint32_t wave_blocks(uint32_t chunks, int waves)
+{
+    return std::max<uint32_t>(1, (chunks + waves - 1) / waves);
+}
+
 }  // namespace
+
+bool shape_supported(const Shape& s)
+{
+    return (s.items == 4 || s.items == 8 || s.items == 16) &&
+           (s.waves == 2 || s.waves == 4 || s.waves == 8) &&
+           (s.scan_threads == 256 || s.scan_threads == 512 || s.scan_threads == 1024);
+}
 
 // ---------------------------------------------------------------------------
 // Launchers (called by the C ABI in abi.cpp).
-hipError_t launch_classify(const double* x, const double* y, const double* z, uint32_t owned,
-                           const BoxParams& bp, uint32_t* entries, uint32_t* counts,
-                           uint32_t chunks, hipStream_t stream)
+hipError_t launch_classify(const double* x, const double* y, const double* z, const Layout& L,
+                           const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
-    if (chunks == 0) return hipSuccess;
-    const uint32_t blocks = (chunks + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL(classify_kernel, dim3(blocks), dim3(kBlock), 0, stream, x, y, z, owned,
-                       bp, entries, counts, chunks);
+    if (L.chunks == 0) return hipSuccess;
+    return dispatch(s, [&](auto items, auto waves) {
+        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
+        hipLaunchKernelGGL((classify_kernel<I, W>), dim3(wave_blocks(L.chunks, W)), dim3(W * kWave), 0,
+                           stream, x, y, z, L, B);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_scan(const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream)
+{
+    if (L.replicas == 0) return hipSuccess;
+    hipLaunchKernelGGL(scan_kernel, dim3(kCols, L.replicas), dim3(s.scan_threads), 0, stream, L, B);
     return hipGetLastError();
 }
 
-hipError_t launch_scan(const uint32_t* counts, uint32_t* prefix, uint32_t* totals,
-                       uint32_t chunks, hipStream_t stream)
+hipError_t launch_scatter(const dsdx_particles& p, const FieldParams& fp, const Layout& L,
+                          const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
-    hipLaunchKernelGGL(scan_kernel, dim3(kCols), dim3(kScanThreads), 0, stream, counts, prefix,
-                       totals, chunks);
-    return hipGetLastError();
+    if (L.chunks == 0) return hipSuccess;
+    return dispatch(s, [&](auto items, auto waves) {
+        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
+        hipLaunchKernelGGL((scatter_kernel<I, W>), dim3(wave_blocks(L.chunks, W)), dim3(W * kWave), 0,
+                           stream, p, fp, L, B);
+        return hipGetLastError();
+    });
 }
 
-hipError_t launch_scatter(const dsdx_particles& p, uint32_t owned, const BoxParams& bp,
-                          const FieldParams& fp, const uint32_t* entries, const uint32_t* counts,
-                          const uint32_t* prefix, uint32_t* totals, uint32_t* owner_index,
-                          uint8_t* owner_code, uint32_t* image_start, uint32_t* ghost_list,
-                          uint32_t chunks, uint64_t capacity, uint64_t ghost_list_capacity,
-                          hipStream_t stream)
+hipError_t launch_reverse_forces(double* fx, double* fy, double* fz, const Layout& L,
+                                 const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
-    const uint32_t blocks = std::max<uint32_t>(1, (chunks + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL(scatter_kernel, dim3(blocks), dim3(kBlock), 0, stream, p, owned, bp, fp,
-                       entries, counts, prefix, totals, owner_index, owner_code, image_start,
-                       ghost_list, chunks, capacity, ghost_list_capacity);
-    return hipGetLastError();
+    if (L.chunks == 0) return hipSuccess;
+    return dispatch(s, [&](auto items, auto waves) {
+        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
+        hipLaunchKernelGGL((reverse_forces_kernel<I, W>), dim3(wave_blocks(L.chunks, W)),
+                           dim3(W * kWave), 0, stream, fx, fy, fz, L, B);
+        return hipGetLastError();
+    });
 }
 
-static uint32_t owner_grid(uint64_t owned)
+hipError_t launch_reverse_fields(const TypedFields& tf, const Layout& L, const PlanBuffers& B,
+                                 const Shape& s, hipStream_t stream)
 {
-    const uint64_t want = (owned + kBlock - 1) / kBlock;
-    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, 2048)));
+    if (L.chunks == 0) return hipSuccess;
+    return dispatch(s, [&](auto items, auto waves) {
+        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
+        hipLaunchKernelGGL((reverse_fields_kernel<I, W>), dim3(wave_blocks(L.chunks, W)),
+                           dim3(W * kWave), 0, stream, tf, L, B);
+        return hipGetLastError();
+    });
 }
 
-hipError_t launch_reverse_forces(double* fx, double* fy, double* fz, uint32_t owned,
-                                 const PlanView& v, const BoxParams& bp, hipStream_t stream)
+hipError_t launch_forward_fields(const FieldParams& fp, const Layout& L, const PlanBuffers& B,
+                                 const Shape& s, hipStream_t stream)
 {
-    hipLaunchKernelGGL(reverse_forces_kernel, dim3(owner_grid(owned)), dim3(kBlock), 0, stream,
-                       fx, fy, fz, owned, v, bp);
-    return hipGetLastError();
+    if (L.chunks == 0) return hipSuccess;
+    return dispatch(s, [&](auto items, auto waves) {
+        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
+        hipLaunchKernelGGL((forward_fields_kernel<I, W>), dim3(wave_blocks(L.chunks, W)),
+                           dim3(W * kWave), 0, stream, fp, L, B);
+        return hipGetLastError();
+    });
 }
 
-hipError_t launch_reverse_fields(const TypedFields& tf, uint32_t owned, const PlanView& v,
-                                 const BoxParams& bp, hipStream_t stream)
+hipError_t launch_refresh(const dsdx_particles& p, const FieldParams& fp, int32_t positions_only,
+                          const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
-    hipLaunchKernelGGL(reverse_fields_kernel, dim3(owner_grid(owned)), dim3(kBlock), 0, stream,
-                       tf, owned, v, bp);
-    return hipGetLastError();
+    if (L.chunks == 0) return hipSuccess;
+    return dispatch(s, [&](auto items, auto waves) {
+        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
+        hipLaunchKernelGGL((refresh_kernel<I, W>), dim3(wave_blocks(L.chunks, W)), dim3(W * kWave), 0,
+                           stream, p, fp, positions_only, L, B);
+        return hipGetLastError();
+    });
 }
 
-hipError_t launch_forward_fields(const FieldParams& fp, uint32_t owned, const PlanView& v,
-                                 const BoxParams& bp, hipStream_t stream)
+hipError_t launch_ghost_owners(int32_t* out, const Layout& L, const PlanBuffers& B, const Shape& s,
+                               hipStream_t stream)
 {
-    hipLaunchKernelGGL(forward_fields_kernel, dim3(owner_grid(owned)), dim3(kBlock), 0, stream,
-                       fp, owned, v, bp);
-    return hipGetLastError();
+    if (L.chunks == 0) return hipSuccess;
+    return dispatch(s, [&](auto items, auto waves) {
+        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
+        hipLaunchKernelGGL((ghost_owners_kernel<I, W>), dim3(wave_blocks(L.chunks, W)),
+                           dim3(W * kWave), 0, stream, out, L, B);
+        return hipGetLastError();
+    });
 }
 
-hipError_t launch_refresh(const dsdx_particles& p, const FieldParams& fp, uint32_t owned,
-                          int32_t positions_only, const PlanView& v, const BoxParams& bp,
-                          hipStream_t stream)
+hipError_t launch_migrate_wrap(double* x, double* y, double* z, const Layout& L, const Shape& s,
+                               hipStream_t stream)
 {
-    hipLaunchKernelGGL(refresh_kernel, dim3(owner_grid(owned)), dim3(kBlock), 0, stream, p, fp,
-                       owned, positions_only, v, bp);
-    return hipGetLastError();
-}
-
-hipError_t launch_ghost_owners(int32_t* out, uint32_t owned, const PlanView& v,
-                               const BoxParams& bp, hipStream_t stream)
-{
-    hipLaunchKernelGGL(ghost_owners_kernel, dim3(owner_grid(owned)), dim3(kBlock), 0, stream, out,
-                       v, bp);
-    return hipGetLastError();
-}
-
-hipError_t launch_migrate_wrap(double* x, double* y, double* z, uint64_t owned,
-                               const BoxParams& bp, hipStream_t stream)
-{
-    if (owned == 0) return hipSuccess;
-    const uint64_t per_block = static_cast<uint64_t>(kBlock) * kWrapItems;
-    const uint64_t blocks = std::min<uint64_t>((owned + per_block - 1) / per_block, 1u << 20);
-    hipLaunchKernelGGL(migrate_wrap_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0,
-                       stream, x, y, z, owned, bp);
-    return hipGetLastError();
+    if (L.chunks == 0) return hipSuccess;
+    return dispatch(s, [&](auto items, auto waves) {
+        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
+        hipLaunchKernelGGL((migrate_wrap_kernel<I, W>), dim3(wave_blocks(L.chunks, W)),
+                           dim3(W * kWave), 0, stream, x, y, z, L);
+        return hipGetLastError();
+    });
 }
 
 }  // namespace dsdx

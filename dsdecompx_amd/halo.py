@@ -149,9 +149,13 @@ class LoopbackHalo:
         g = C.c_uint64(0)
         s = _stream_ptr(stream)
         pc = p.c()
+        # the ghosts must fit the particle arrays and every field
+        capacity = min([p.capacity] + [int(f.shape[0]) for f in fields or ()])
         rc = _abi.lib().dsdx_halo_build(self._plan, C.byref(self._box), C.byref(pc), arr, n,
-                                        owned, p.capacity, s, C.byref(g) if wait else None)
+                                        owned, capacity, s, C.byref(g) if wait else None)
         if rc == _abi.ECAPACITY:
+            if any(int(f.shape[0]) < owned + g.value for f in fields or ()):
+                raise _abi.CapacityError("a registered field is too small for the ghosts")
             p.grow(owned + g.value, owned)
             pc = p.c()
             rc = _abi.lib().dsdx_halo_scatter(self._plan, C.byref(pc), arr, n, owned,

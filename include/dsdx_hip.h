@@ -183,6 +183,74 @@ int dsdx_reverse_exchange(const dsdx_plan* plan, const dsdx_field* fields,
 int dsdx_forward_exchange(const dsdx_plan* plan, const dsdx_field* fields,
                           uint32_t field_count, uint64_t owned, void* stream);
 
+/* Launch shape of the plan's kernels: lane items per chunk (4, 8, 16; a
+ * chunk is 64 x items particles, one wave each), waves per block (2, 4, 8)
+ * and scan block threads (256, 512, 1024).  Defaults 8 / 4 / 1024, or the
+ * environment variables DSDX_ITEMS, DSDX_WAVES, DSDX_SCAN_THREADS; the
+ * autotuner (dsdecompx_amd.autotune) picks them by measurement.  Changing
+ * items drops the plan. */
+int dsdx_plan_set_shape(dsdx_plan* plan, int32_t items, int32_t waves,
+                        int32_t scan_threads);
+int dsdx_plan_get_shape(const dsdx_plan* plan, int32_t* items, int32_t* waves,
+                        int32_t* scan_threads);
+
+/* ---- Batched form: many independent boxes ("replicas") in one set of SoA
+ * arrays, each phase one launch over all of them, nothing waiting on the
+ * host, so a whole step can be captured in a hipGraph.  Replica r owns the
+ * particle slots [offset, offset + capacity) in every particle array (and in
+ * every registered field): its owned particles first, its ghosts after them.
+ * Each replica's result is bit-identical to the single-box entry points on
+ * its own slots.  No counterpart in the reference (which has one box per
+ * rank); for ensembles of replica systems. */
+typedef struct dsdx_replica {
+    dsdx_box box;
+    uint64_t offset;
+    uint64_t capacity;
+    uint64_t owned;
+} dsdx_replica;
+
+typedef struct dsdx_batch dsdx_batch;
+
+/* Copies the replica table to the device and sizes every buffer (the ghost
+ * list to the sum of capacity - owned), so later calls allocate nothing.
+ * Regions must be ascending and disjoint. */
+int dsdx_batch_create(const dsdx_replica* replicas, uint32_t count,
+                      void* stream, dsdx_batch** batch);
+int dsdx_batch_destroy(dsdx_batch* batch);
+int dsdx_batch_set_shape(dsdx_batch* batch, int32_t items, int32_t waves,
+                         int32_t scan_threads, void* stream);
+int dsdx_batch_get_shape(const dsdx_batch* batch, int32_t* items,
+                         int32_t* waves, int32_t* scan_threads);
+int dsdx_batch_migrate_wrap(const dsdx_batch* batch, double* x, double* y,
+                            double* z, void* stream);
+/* A replica whose ghosts do not fit its capacity writes none of them and
+ * sets its status word (read by dsdx_batch_counts); the owner-major calls
+ * then skip it. */
+int dsdx_batch_halo_build(dsdx_batch* batch, const dsdx_particles* particles,
+                          const dsdx_field* fields, uint32_t field_count,
+                          void* stream);
+int dsdx_batch_refresh(dsdx_batch* batch, const dsdx_particles* particles,
+                       const dsdx_field* fields, uint32_t field_count,
+                       int32_t positions_only, void* stream);
+int dsdx_batch_reverse_halo(const dsdx_batch* batch, double* fx, double* fy,
+                            double* fz, void* stream);
+int dsdx_batch_reverse_exchange(const dsdx_batch* batch,
+                                const dsdx_field* fields, uint32_t field_count,
+                                void* stream);
+int dsdx_batch_forward_exchange(const dsdx_batch* batch,
+                                const dsdx_field* fields, uint32_t field_count,
+                                void* stream);
+/* owners[offset_r + g] = owned index (replica-local) of ghost g of replica r */
+int dsdx_batch_ghost_owners(const dsdx_batch* batch, int32_t* owners,
+                            void* stream);
+/* Waits; per replica: ghosts, distinct owners, status (0 or overflow bits). */
+int dsdx_batch_counts(const dsdx_batch* batch, uint32_t* ghosts,
+                      uint32_t* owners, uint32_t* status, void* stream);
+/* Device pointer to the per-replica counters, 32 u32 words per replica:
+ * [0..25] send count per slot, [26] owners, [27] ghosts, [28] status.  For
+ * kernels that must know a replica's ghost count without a host wait. */
+const uint32_t* dsdx_batch_device_totals(const dsdx_batch* batch);
+
 /* Per-kernel device timing, process-wide: when enabled, every kernel launch
  * is bracketed by two HIP events on its stream.  dsdx_timing_read waits for
  * the recorded events and returns the accumulated milliseconds and launch

@@ -1,0 +1,170 @@
+"""cfg5-style batching: many replica boxes of uneven size in one launch per
+phase.  Every replica's ghosts, positions and reverse sums are bit-exact with
+the oracle run on that replica alone and with the plain per-box loop; the
+full step captured in a hipGraph replays to the same bits as the eager step
+(several replays, eager work in between)."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.gpu_util import F64, I32, assert_bits_equal, host, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.set_stream(torch.cuda.Stream())
+    return torch
+
+
+def _specs(count, seed, nmin=2048, nmax=30000):
+    from dsdecompx_amd.batch import ReplicaSpec
+    from dsdecompx_amd.halo import BoxConfig
+    rng = np.random.default_rng(seed)
+    n = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count)).astype(int)
+    rho = rng.uniform(0.6, 1.2, count)
+    specs = []
+    for k in range(count):
+        L = float((n[k] / rho[k]) ** (1.0 / 3.0))
+        specs.append(ReplicaSpec(int(n[k]), BoxConfig(upper=(L, L, L), width=2.8)))
+    return specs
+
+
+def _fill(torch, b, seed):
+    rng = np.random.default_rng(seed)
+    for r, s in enumerate(b.specs):
+        sl = b.owned_slice(r)
+        L = s.box.upper[0]
+        for k in ("x", "y", "z"):
+            getattr(b, k)[sl] = torch.from_numpy(rng.uniform(0, L, s.owned)).cuda()
+        for k in ("vx", "vy", "vz", "charge"):
+            getattr(b, k)[sl] = torch.from_numpy(rng.standard_normal(s.owned)).cuda()
+        b.mass[sl] = 1.0
+        b.type[sl] = torch.from_numpy(rng.integers(0, 4, s.owned).astype(np.int32)).cuda()
+        b.global_id[sl] = torch.arange(s.owned, dtype=torch.int32, device="cuda")
+
+
+def _snapshot(b):
+    return {k: host(getattr(b, k)).copy() for k in F64 + I32}
+
+
+def test_batch_matches_oracle_and_loop(gpu):
+    torch = gpu
+    from dsdecompx_amd.batch import LoopedReplicas, ReplicaBatch
+    specs = _specs(12, 5)
+    say("batch", [s.owned for s in specs])
+    b = ReplicaBatch(specs)
+    _fill(torch, b, 6)
+    start = _snapshot(b)
+    rng = np.random.default_rng(9)
+    disp = rng.normal(0, 0.1, (3, b.total))
+    forces = rng.uniform(-1, 1, (3, b.total))
+
+    # batched, eager
+    for a, k in enumerate(("x", "y", "z")):
+        getattr(b, k).add_(torch.from_numpy(disp[a]).cuda())
+    b.migrate()
+    b.exchange_halo()
+    ghosts, owners, status = b.counts()
+    assert not any(status)
+    after_halo = _snapshot(b)
+    own_all = host(b.ghost_owners())
+    for a, k in enumerate(("fx", "fy", "fz")):
+        getattr(b, k).copy_(torch.from_numpy(forces[a]).cuda())
+    b.reverse_halo()
+    after_rev = _snapshot(b)
+
+    # oracle per replica
+    for r, s in enumerate(specs):
+        n, g, off = s.owned, ghosts[r], b.offsets[r]
+        box = po.Box(upper=s.box.upper, width=s.box.width)
+        o = po.Oracle(box)
+        p = po.Particles(b.capacities[r])
+        for k in F64 + I32:
+            getattr(p, k)[:n] = start[k][off:off + n]
+        for a, k in enumerate(("x", "y", "z")):
+            getattr(p, k)[:n] += disp[a][off:off + n]
+        o.migrate(p, n)
+        assert o.exchange_halo(p, n) == g, f"replica {r} ghost count"
+        for k in F64 + I32:
+            assert_bits_equal(after_halo[k][off:off + n + g], getattr(p, k)[:n + g], f"r{r} halo {k}")
+        assert_bits_equal(own_all[off:off + g], o.ghost_owners(), f"r{r} ghost owners")
+        assert owners[r] == len(np.unique(o.ghost_owners()))
+        for a, k in enumerate(("fx", "fy", "fz")):
+            getattr(p, k)[:n + g] = forces[a][off:off + n + g]
+        o.reverse_halo(p, n)
+        for k in ("fx", "fy", "fz"):
+            assert_bits_equal(after_rev[k][off:off + n], getattr(p, k)[:n], f"r{r} reverse {k}")
+
+    # the plain per-box loop on the same storage
+    for k in F64 + I32:
+        getattr(b, k).copy_(torch.from_numpy(start[k]).cuda())
+    loop = LoopedReplicas(b)
+    for a, k in enumerate(("x", "y", "z")):
+        getattr(b, k).add_(torch.from_numpy(disp[a]).cuda())
+    loop.migrate()
+    loop.exchange_halo()
+    assert loop.ghosts == ghosts
+    for r, s in enumerate(specs):
+        off, n, g = b.offsets[r], s.owned, ghosts[r]
+        for k in F64 + I32:
+            assert_bits_equal(host(getattr(b, k)[off:off + n + g]), after_halo[k][off:off + n + g],
+                              f"loop r{r} {k}")
+    for a, k in enumerate(("fx", "fy", "fz")):
+        getattr(b, k).copy_(torch.from_numpy(forces[a]).cuda())
+    loop.reverse_halo()
+    for r, s in enumerate(specs):
+        off, n = b.offsets[r], s.owned
+        for k in ("fx", "fy", "fz"):
+            assert_bits_equal(host(getattr(b, k)[off:off + n]), after_rev[k][off:off + n],
+                              f"loop r{r} reverse {k}")
+
+
+@pytest.mark.parametrize("shape", [(8, 4, 1024), (4, 2, 256), (16, 8, 512)])
+def test_graph_replays_match_eager(gpu, shape):
+    torch = gpu
+    from dsdecompx_amd.batch import ReplicaBatch, capture_step
+    specs = _specs(9, 11)
+    say("graph", shape)
+    b = ReplicaBatch(specs)
+    b.set_shape(*shape)
+    _fill(torch, b, 12)
+    start = _snapshot(b)
+    disp = [torch.randn((3, b.total), dtype=torch.float64, device="cuda",
+                        generator=torch.Generator(device="cuda").manual_seed(40 + k)) * 0.1
+            for k in range(3)]
+    frc = torch.rand((3, b.total), dtype=torch.float64, device="cuda",
+                     generator=torch.Generator(device="cuda").manual_seed(50)) * 2 - 1
+    step_disp = torch.zeros((3, b.total), dtype=torch.float64, device="cuda")
+
+    def app_halo():
+        b.x.add_(step_disp[0]); b.y.add_(step_disp[1]); b.z.add_(step_disp[2])
+
+    def app_rev():
+        b.fx.copy_(frc[0]); b.fy.copy_(frc[1]); b.fz.copy_(frc[2])
+
+    # eager reference: three steps
+    eager = []
+    for k in range(3):
+        step_disp.copy_(disp[k])
+        app_halo(); b.migrate(); b.exchange_halo(); app_rev(); b.reverse_halo()
+        eager.append(_snapshot(b))
+    # graph: restore, capture (warm-up steps run eagerly inside capture_step),
+    # then restore again and replay
+    for k_ in F64 + I32:
+        getattr(b, k_).copy_(torch.from_numpy(start[k_]).cuda())
+    step_disp.zero_()
+    graph = capture_step(b, app_halo, app_rev, warmup=1)
+    for k_ in F64 + I32:
+        getattr(b, k_).copy_(torch.from_numpy(start[k_]).cuda())
+    for k in range(3):
+        step_disp.copy_(disp[k])
+        graph.replay()
+        torch.cuda.synchronize()
+        got = _snapshot(b)
+        for name in F64 + I32:
+            assert_bits_equal(got[name], eager[k][name], f"replay {k} {name}")
