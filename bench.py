@@ -158,6 +158,30 @@ def measure_abi(args, n, L, w, sigma, periodic):
             "path": "C ABI from Python (dsdx_migrate_wrap, dsdx_halo_build, dsdx_reverse_halo)"}
 
 
+def tune_shape(args, n, L, w, periodic):
+    """Autotunes the launch shape on a synthetic box of the workload's size
+    (dsdecompx_amd.autotune, HIP-event timed) and returns (shape, report)."""
+    import torch
+    from dsdecompx_amd import autotune
+    from dsdecompx_amd.halo import BoxConfig, DeviceParticles, LoopbackHalo
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    torch.cuda.set_stream(torch.cuda.Stream(dev))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 1)
+    p = DeviceParticles(int(n * 1.12) + 4096, dev)
+    for k in ("x", "y", "z"):
+        getattr(p, k)[:n].uniform_(0.0, L, generator=gen)
+    p.global_id[:n] = torch.arange(n, dtype=torch.int32, device=dev)
+    halo = LoopbackHalo(BoxConfig(upper=(L, L, L), periodic=periodic, width=w))
+    report = autotune.tune_single(halo, p, n)
+    halo.close()
+    del p
+    torch.cuda.empty_cache()
+    return report["best"], report
+
+
 def measure_cpp(args, n, L, w, sigma, periodic):
     """The hot path through the drop-in C++ API (dsdecompx::Decomp), driven
     by the compiled driver dsdecompx_amd/bin/comm_bench."""
@@ -165,13 +189,20 @@ def measure_cpp(args, n, L, w, sigma, periodic):
     exe = os.path.join(ROOT, "dsdecompx_amd", "bin", "comm_bench")
     if not os.path.exists(exe):
         raise SystemExit(f"{exe} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    env = dict(os.environ)
+    tuned = None
+    if args.tune:
+        from dsdecompx_amd import autotune
+        shape, tuned = tune_shape(args, n, L, w, periodic)
+        env.update(autotune.env_for(shape))
     cmd = [exe, str(n), repr(L), repr(w), repr(sigma), "1" if periodic[2] else "0",
            str(args.steps), str(args.warmup), str(args.seed)]
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
     if r.returncode != 0:
         raise SystemExit(f"comm_bench failed ({r.returncode}): {r.stderr[-2000:]}")
     res = json.loads(r.stdout.strip().splitlines()[-1])
     res["path"] = "C++ API (dsdecompx::Decomp migrate / exchange_halo / reverse_halo)"
+    res["autotune"] = tuned
     return res
 
 
@@ -252,12 +283,17 @@ def measure_batched(args):
     restore()
     t_loop = timed(loop_step, steps)
     loop_state = {k: getattr(b, k).clone() for k in start}
+    tuned = None
+    if args.tune:
+        from dsdecompx_amd import autotune
+        restore()
+        tuned = autotune.tune_batch(b)
     restore()
     timed(eager_step, 1)
     restore()
     t_eager = timed(eager_step, steps)
     restore()
-    graph = capture_step(b, app_halo, app_rev, warmup=1)
+    graph = capture_step(b, app_halo, app_rev, warmup=1, fused=True)
     restore()
     t_graph = timed(graph.replay, steps)
     equal = all(torch.equal(getattr(b, k).view(torch.int64) if getattr(b, k).dtype == torch.float64
@@ -281,6 +317,9 @@ def measure_batched(args):
         "looped_ms_per_step": 1e3 * t_loop / steps,
         "bit_exact_graph_vs_looped": bool(equal),
         "shape": list(b.shape()),
+        "autotune": tuned,
+        "graph_form": "one hipGraph per step: displacement, fused migrate+classify, scan, "
+                      "scatter, force fill, reverse_halo",
         "timing": "wall clock over the steps, application displacement and force fill included "
                   "in every form",
     }
@@ -344,12 +383,30 @@ def run(args):
         "owners_per_step": U,
         "wall_ms_per_step": res["wall_ms_per_step"],
         "kernels": per_kernel,
+        "autotune": res.get("autotune"),
     }
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(name, n, L, w, sigma, periodic,
                                                                  args.cpu_steps)
     if not args.no_batched:
         out["batched"] = measure_batched(args)
+    if args.copy_calibration:
+        out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
     return out
+
+
+def copy_calibration(nbytes):
+    """One plain device-to-device copy of `nbytes` (torch's copy kernel),
+    after the measured steps: a kernel of known traffic (nbytes read, nbytes
+    written) against which rocprofv3's FETCH_SIZE / WRITE_SIZE counters of
+    the same run are scaled."""
+    import torch
+    n = int(nbytes) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    return 8 * n
 
 
 def _owners(halo):
@@ -374,6 +431,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the cfg5 batched comparison")
+    ap.add_argument("--copy-calibration", type=float, default=0,
+                    help="after the run, copy this many bytes once (counter calibration)")
+    ap.add_argument("--no-tune", dest="tune", action="store_false",
+                    help="keep the default launch shape instead of autotuning it")
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--traffic", type=float, default=None,

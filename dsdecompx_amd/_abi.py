@@ -79,6 +79,8 @@ def lib():
         "dsdx_migrate_wrap": ([C.POINTER(Box), P, P, P, u64, P], C.c_int),
         "dsdx_halo_build": ([P, C.POINTER(Box), C.POINTER(Particles), C.POINTER(Field), u32,
                              u64, u64, P, C.POINTER(u64)], C.c_int),
+        "dsdx_migrate_halo_build": ([P, C.POINTER(Box), C.POINTER(Particles), C.POINTER(Field),
+                                     u32, u64, u64, P, C.POINTER(u64)], C.c_int),
         "dsdx_halo_scatter": ([P, C.POINTER(Particles), C.POINTER(Field), u32, u64, u64, P],
                               C.c_int),
         "dsdx_halo_refresh": ([P, C.POINTER(Box), C.POINTER(Particles), C.POINTER(Field), u32,
@@ -97,6 +99,8 @@ def lib():
         "dsdx_batch_get_shape": ([P, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
         "dsdx_batch_migrate_wrap": ([P, P, P, P, P], C.c_int),
         "dsdx_batch_halo_build": ([P, C.POINTER(Particles), C.POINTER(Field), u32, P], C.c_int),
+        "dsdx_batch_migrate_halo_build": ([P, C.POINTER(Particles), C.POINTER(Field), u32, P],
+                                          C.c_int),
         "dsdx_batch_refresh": ([P, C.POINTER(Particles), C.POINTER(Field), u32, i32, P], C.c_int),
         "dsdx_batch_reverse_halo": ([P, P, P, P, P], C.c_int),
         "dsdx_batch_reverse_exchange": ([P, C.POINTER(Field), u32, P], C.c_int),
@@ -128,11 +132,13 @@ def exported_symbols():
             "dsdx_batch_migrate_wrap", "dsdx_batch_halo_build", "dsdx_batch_refresh",
             "dsdx_batch_reverse_halo", "dsdx_batch_reverse_exchange",
             "dsdx_batch_forward_exchange", "dsdx_batch_ghost_owners", "dsdx_batch_counts",
-            "dsdx_batch_device_totals"]
+            "dsdx_batch_device_totals", "dsdx_migrate_halo_build",
+            "dsdx_batch_migrate_halo_build"]
 
 
 KERNELS = ("migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "reverse_halo",
-           "reverse_exchange", "forward_exchange", "halo_refresh", "ghost_owners")
+           "reverse_exchange", "forward_exchange", "halo_refresh", "ghost_owners",
+           "migrate_classify")
 
 
 def timing_enable(on: bool = True) -> None:

@@ -66,9 +66,11 @@ __global__ void place(double* x, double* y, double* z, double* vx, double* vy, d
     id[i] = static_cast<int>(i);
 }
 
-// Box-Muller displacement N(0, sigma) per axis
+// Box-Muller displacement N(0, sigma) per axis; with z non-periodic the
+// application reflects z at the walls 0 and L (the library only wraps
+// periodic axes).
 __global__ void move(double* x, double* y, double* z, uint64_t n, double sigma, uint64_t seed,
-                     uint64_t step)
+                     uint64_t step, int zper, double L)
 {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) return;
@@ -76,7 +78,12 @@ __global__ void move(double* x, double* y, double* z, uint64_t n, double sigma, 
     for (int a = 0; a < 3; ++a) {
         const double u1 = 1.0 - unit(seed, 16 + 2 * a + 8 * step, i);
         const double u2 = unit(seed, 17 + 2 * a + 8 * step, i);
-        axis[a][i] += sigma * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        double v = axis[a][i] + sigma * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+        if (a == 2 && !zper) {
+            if (v < 0.0) v = -v;
+            if (v > L) v = 2.0 * L - v;
+        }
+        axis[a][i] = v;
     }
 }
 
@@ -153,7 +160,7 @@ int main(int argc, char** argv)
             wall0 = std::chrono::steady_clock::now();
         }
         hipLaunchKernelGGL(move, dim3(grid(n)), dim3(256), 0, s, p.x_data(), p.y_data(), p.z_data(),
-                           n, sigma, seed, static_cast<uint64_t>(step));
+                           n, sigma, seed, static_cast<uint64_t>(step), zper ? 1 : 0, L);
         if (timed) ok(hipEventRecord(ev[0], s), "record");
         decomp.migrate(p, s);
         const HaloResult r = decomp.exchange_halo(p, {true, s});

@@ -117,6 +117,16 @@ class ReplicaBatch:
         _abi.check(_abi.lib().dsdx_batch_halo_build(self._batch, C.byref(p), arr, len(fields),
                                                     self._stream(stream)), "batch halo")
 
+    def migrate_exchange_halo(self, fields=(), stream=None) -> None:
+        """migrate then exchange_halo, fused into one pass over x, y, z."""
+        arr = (_abi.Field * max(1, len(fields)))()
+        for k, f in enumerate(fields):
+            arr[k] = field_struct(f)
+        p = self.particles()
+        _abi.check(_abi.lib().dsdx_batch_migrate_halo_build(self._batch, C.byref(p), arr,
+                                                            len(fields), self._stream(stream)),
+                   "batch migrate + halo")
+
     def refresh(self, positions_only=True, stream=None) -> None:
         p = self.particles()
         _abi.check(_abi.lib().dsdx_batch_refresh(self._batch, C.byref(p), None, 0,
@@ -209,10 +219,11 @@ class LoopedReplicas:
 
 
 def capture_step(batch: ReplicaBatch, app_before_halo=None, app_before_reverse=None,
-                 warmup: int = 2):
+                 warmup: int = 2, fused: bool = False):
     """Captures migrate -> exchange_halo -> [app] -> reverse_halo of every
     replica in one hipGraph and returns the torch.cuda.CUDAGraph.  The
-    application callbacks (displacement, forces) are captured in place."""
+    application callbacks (displacement, forces) are captured in place.  With
+    `fused` migrate and the rebuild share one pass over the positions."""
     _abi.timing_enable(False)
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
@@ -220,8 +231,11 @@ def capture_step(batch: ReplicaBatch, app_before_halo=None, app_before_reverse=N
     def body():
         if app_before_halo:
             app_before_halo()
-        batch.migrate()
-        batch.exchange_halo()
+        if fused:
+            batch.migrate_exchange_halo()
+        else:
+            batch.migrate()
+            batch.exchange_halo()
         if app_before_reverse:
             app_before_reverse()
         batch.reverse_halo()

@@ -279,13 +279,14 @@ int read_totals(Plan* plan, hipStream_t stream)
     return DSDX_OK;
 }
 
-int run_classify(Plan* plan, const dsdx_particles* p, hipStream_t s)
+int run_classify(Plan* plan, const dsdx_particles* p, bool wrap_first, hipStream_t s)
 {
     const Layout L = plan->layout();
     const PlanBuffers B = plan->buffers();
     {
-        timing::Scope timed(timing::kClassify, s);
-        DSDX_HIP(launch_classify(p->x, p->y, p->z, L, B, plan->shape, s), "launch halo classify");
+        timing::Scope timed(wrap_first ? timing::kMigrateClassify : timing::kClassify, s);
+        DSDX_HIP(launch_classify(p->x, p->y, p->z, wrap_first, L, B, plan->shape, s),
+                 "launch halo classify");
     }
     {
         timing::Scope timed(timing::kScan, s);
@@ -348,6 +349,10 @@ bool single_matches(const Plan* plan, uint64_t owned)
 }
 
 }  // namespace
+
+static int halo_build(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles* particles,
+                      const dsdx_field* fields, uint32_t field_count, uint64_t owned,
+                      uint64_t capacity, void* stream, uint64_t* ghosts, bool wrap_first);
 
 extern "C" {
 
@@ -451,6 +456,24 @@ int dsdx_halo_build(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles* 
                     const dsdx_field* fields, uint32_t field_count, uint64_t owned, uint64_t capacity,
                     void* stream, uint64_t* ghosts)
 {
+    return halo_build(plan, box, particles, fields, field_count, owned, capacity, stream, ghosts,
+                      false);
+}
+
+int dsdx_migrate_halo_build(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles* particles,
+                            const dsdx_field* fields, uint32_t field_count, uint64_t owned,
+                            uint64_t capacity, void* stream, uint64_t* ghosts)
+{
+    return halo_build(plan, box, particles, fields, field_count, owned, capacity, stream, ghosts,
+                      true);
+}
+
+}  // extern "C"
+
+static int halo_build(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles* particles,
+                      const dsdx_field* fields, uint32_t field_count, uint64_t owned,
+                      uint64_t capacity, void* stream, uint64_t* ghosts, bool wrap_first)
+{
     if (!plan) return fail(DSDX_EINVAL, "null plan");
     if (!stream) return fail(DSDX_EINVAL, "device halo requires an explicit stream");
     if (owned > kMaxOwned) return fail(DSDX_EINVAL, "too many particles for one device plan");
@@ -465,7 +488,7 @@ int dsdx_halo_build(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles* 
     plan->classified = plan->built = plan->totals_cached = false;
     const uint64_t guess = std::max<uint64_t>(plan->ghost_list_capacity, owned + 1024);
     if (int rc = set_single(plan, bp, owned, capacity, guess, s)) return rc;
-    if (int rc = run_classify(plan, particles, s)) return rc;
+    if (int rc = run_classify(plan, particles, wrap_first, s)) return rc;
     if (int rc = run_scatter(plan, particles, fp, s)) return rc;
     if (!ghosts) {
         plan->built = true;
@@ -482,6 +505,8 @@ int dsdx_halo_build(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles* 
     if (status & kOverflowStorage) return fail(DSDX_ECAPACITY, "device halo output capacity is too small");
     return rescatter(plan, particles, fp, capacity, s, g);
 }
+
+extern "C" {
 
 int dsdx_halo_scatter(dsdx_plan* plan, const dsdx_particles* particles, const dsdx_field* fields,
                       uint32_t field_count, uint64_t owned, uint64_t capacity, void* stream)
@@ -697,8 +722,8 @@ int dsdx_batch_migrate_wrap(const dsdx_batch* batch, double* x, double* y, doubl
     return DSDX_OK;
 }
 
-int dsdx_batch_halo_build(dsdx_batch* batch, const dsdx_particles* particles, const dsdx_field* fields,
-                          uint32_t field_count, void* stream)
+static int batch_build(dsdx_batch* batch, const dsdx_particles* particles, const dsdx_field* fields,
+                       uint32_t field_count, void* stream, bool wrap_first)
 {
     if (!batch) return fail(DSDX_EINVAL, "null batch");
     if (!stream) return fail(DSDX_EINVAL, "device halo requires an explicit stream");
@@ -707,10 +732,22 @@ int dsdx_batch_halo_build(dsdx_batch* batch, const dsdx_particles* particles, co
     if (int rc = make_field_params(fields, field_count, &fp)) return rc;
     const hipStream_t s = as_stream(stream);
     batch->totals_cached = false;
-    if (int rc = run_classify(batch, particles, s)) return rc;
+    if (int rc = run_classify(batch, particles, wrap_first, s)) return rc;
     if (int rc = run_scatter(batch, particles, fp, s)) return rc;
     batch->built = true;
     return DSDX_OK;
+}
+
+int dsdx_batch_halo_build(dsdx_batch* batch, const dsdx_particles* particles, const dsdx_field* fields,
+                          uint32_t field_count, void* stream)
+{
+    return batch_build(batch, particles, fields, field_count, stream, false);
+}
+
+int dsdx_batch_migrate_halo_build(dsdx_batch* batch, const dsdx_particles* particles,
+                                  const dsdx_field* fields, uint32_t field_count, void* stream)
+{
+    return batch_build(batch, particles, fields, field_count, stream, true);
 }
 
 int dsdx_batch_refresh(dsdx_batch* batch, const dsdx_particles* particles, const dsdx_field* fields,

@@ -96,7 +96,7 @@ int upload_table(Plan* plan, hipStream_t stream);
 int set_single(Plan* plan, const BoxParams& bp, uint64_t owned, uint64_t capacity,
                uint64_t ghost_list_elems, hipStream_t stream);
 int read_totals(Plan* plan, hipStream_t stream);
-int run_classify(Plan* plan, const dsdx_particles* p, hipStream_t s);
+int run_classify(Plan* plan, const dsdx_particles* p, bool wrap_first, hipStream_t s);
 int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipStream_t s);
 int shape_from_env(Shape* s);
 

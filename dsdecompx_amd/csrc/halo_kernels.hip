@@ -67,16 +67,40 @@ __device__ __forceinline__ uint32_t slab_code(double px, double py, double pz, c
            (static_cast<uint32_t>(pz > bp.hi_thr[2]) << 5);
 }
 
+// Periodic wrap of migrate at P = 1 (src/migration.cpp:106-113):
+// v = x - lower; v -= floor(v / extent) * extent; v + lower, each operation
+// rounded on its own (explicit _rn intrinsics, never fused).
+__device__ __forceinline__ double wrap(double value, double lower, double extent)
+{
+    const double v = __dsub_rn(value, lower);
+    const double q = floor(__ddiv_rn(v, extent));
+    return __dadd_rn(__dsub_rn(v, __dmul_rn(q, extent)), lower);
+}
+
+// Wraps the periodic axes of one lane's value and stores it back when it
+// changed (used by the fused migrate + classify pass).
+__device__ __forceinline__ void wrap_store(double& v, double* dst, bool valid, int periodic,
+                                           double lower, double extent)
+{
+    if (!periodic) return;
+    const double w = wrap(v, lower, extent);
+    if (valid && __double_as_longlong(w) != __double_as_longlong(v)) *dst = w;
+    v = w;
+}
+
 // ---------------------------------------------------------------------------
 // Pass 1: one wave per chunk.  Slab-classifies every owned particle,
 // compacts the boundary ones (non-empty slot set) into the chunk's entry
 // region in owned order, and counts per send slot through an LDS histogram of
 // slab codes (one LDS atomic per boundary particle).  Reads 24 B per
 // particle, writes 4 B per boundary particle and 28 counters per chunk.
-template <int ITEMS, int WAVES>
+// With WRAP the pass first applies migrate's periodic wrap to the positions
+// it loaded (storing the changed ones) and classifies the wrapped values:
+// migrate followed by a rebuild with one read of x, y, z.
+template <int ITEMS, int WAVES, bool WRAP>
 __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
-    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
-    Layout L, PlanBuffers B)
+    double* __restrict__ x, double* __restrict__ y, double* __restrict__ z, Layout L,
+    PlanBuffers B)
 {
     constexpr uint32_t K = ITEMS * kWave;
     __shared__ uint32_t s_lut[WAVES][64];
@@ -106,6 +130,16 @@ __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
             px[k] = valid ? x[off + i] : 0.0;
             py[k] = valid ? y[off + i] : 0.0;
             pz[k] = valid ? z[off + i] : 0.0;
+        }
+        if (WRAP) {
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k) {
+                const uint32_t i = first + k * kWave + lane;
+                const bool valid = i < owned;
+                wrap_store(px[k], x + off + i, valid, d->bp.periodic[0], d->bp.lower[0], d->bp.extent[0]);
+                wrap_store(py[k], y + off + i, valid, d->bp.periodic[1], d->bp.lower[1], d->bp.extent[1]);
+                wrap_store(pz[k], z + off + i, valid, d->bp.periodic[2], d->bp.lower[2], d->bp.extent[2]);
+            }
         }
         const uint64_t ebase = static_cast<uint64_t>(chunk) * K;
 #pragma unroll
@@ -505,16 +539,8 @@ __global__ __launch_bounds__(WAVES * kWave) void ghost_owners_kernel(int32_t* __
 }
 
 // ---------------------------------------------------------------------------
-// migrate at P = 1: periodic wrap in place (src/migration.cpp:106-113):
-// v = x - lower; v -= floor(v / extent) * extent; v + lower, each operation
-// rounded on its own (explicit _rn intrinsics, never fused).  Only a changed
-// coordinate is stored.  One wave per chunk, loads before compute.
-__device__ __forceinline__ double wrap(double value, double lower, double extent)
-{
-    const double v = __dsub_rn(value, lower);
-    const double q = floor(__ddiv_rn(v, extent));
-    return __dadd_rn(__dsub_rn(v, __dmul_rn(q, extent)), lower);
-}
+// migrate at P = 1 (the wrap itself is defined above classify).  One wave per
+// chunk, loads before compute; only a changed coordinate is stored.
 
 template <int ITEMS, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void migrate_wrap_kernel(
@@ -586,14 +612,18 @@ bool shape_supported(const Shape& s)
 
 // ---------------------------------------------------------------------------
 // Launchers (called by the C ABI in abi.cpp).
-hipError_t launch_classify(const double* x, const double* y, const double* z, const Layout& L,
+hipError_t launch_classify(double* x, double* y, double* z, bool wrap_first, const Layout& L,
                            const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
     if (L.chunks == 0) return hipSuccess;
     return dispatch(s, [&](auto items, auto waves) {
         constexpr int I = decltype(items)::value, W = decltype(waves)::value;
-        hipLaunchKernelGGL((classify_kernel<I, W>), dim3(wave_blocks(L.chunks, W)), dim3(W * kWave), 0,
-                           stream, x, y, z, L, B);
+        if (wrap_first)
+            hipLaunchKernelGGL((classify_kernel<I, W, true>), dim3(wave_blocks(L.chunks, W)),
+                               dim3(W * kWave), 0, stream, x, y, z, L, B);
+        else
+            hipLaunchKernelGGL((classify_kernel<I, W, false>), dim3(wave_blocks(L.chunks, W)),
+                               dim3(W * kWave), 0, stream, x, y, z, L, B);
         return hipGetLastError();
     });
 }

@@ -9,7 +9,8 @@ namespace dsdx {
 
 bool shape_supported(const Shape& s);
 
-hipError_t launch_classify(const double* x, const double* y, const double* z, const Layout& L,
+// wrap_first: apply migrate's periodic wrap in the same pass (fused step).
+hipError_t launch_classify(double* x, double* y, double* z, bool wrap_first, const Layout& L,
                            const PlanBuffers& B, const Shape& s, hipStream_t stream);
 hipError_t launch_scan(const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream);
 hipError_t launch_scatter(const dsdx_particles& p, const FieldParams& fp, const Layout& L,

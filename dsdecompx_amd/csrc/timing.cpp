@@ -13,7 +13,8 @@ namespace {
 
 const char* const kNames[kKernelCount] = {"migrate_wrap", "halo_classify", "halo_scan",
                                           "halo_scatter", "reverse_halo", "reverse_exchange",
-                                          "forward_exchange", "halo_refresh", "ghost_owners"};
+                                          "forward_exchange", "halo_refresh", "ghost_owners",
+                                          "migrate_classify"};
 
 struct Pending {
     Kernel kernel;

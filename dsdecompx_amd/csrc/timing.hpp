@@ -18,6 +18,7 @@ enum Kernel {
     kForward,
     kRefresh,
     kGhostOwners,
+    kMigrateClassify,
     kKernelCount
 };
 

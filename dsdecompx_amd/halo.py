@@ -165,6 +165,27 @@ class LoopbackHalo:
         self.ghosts = int(g.value) if wait else -1
         return self.ghosts
 
+    def migrate_exchange_halo(self, p: DeviceParticles, owned: int, fields=(), stream=None) -> int:
+        """migrate then exchange_halo(rebuild), fused into one pass over x, y, z."""
+        arr, n = _field_array(fields)
+        g = C.c_uint64(0)
+        s = _stream_ptr(stream)
+        pc = p.c()
+        capacity = min([p.capacity] + [int(f.shape[0]) for f in fields or ()])
+        rc = _abi.lib().dsdx_migrate_halo_build(self._plan, C.byref(self._box), C.byref(pc), arr,
+                                                n, owned, capacity, s, C.byref(g))
+        if rc == _abi.ECAPACITY:
+            if any(int(f.shape[0]) < owned + g.value for f in fields or ()):
+                raise _abi.CapacityError("a registered field is too small for the ghosts")
+            p.grow(owned + g.value, owned)
+            pc = p.c()
+            rc = _abi.lib().dsdx_halo_scatter(self._plan, C.byref(pc), arr, n, owned,
+                                              p.capacity, s)
+        _abi.check(rc, "migrate_exchange_halo")
+        self.owned = owned
+        self.ghosts = int(g.value)
+        return self.ghosts
+
     def refresh(self, p: DeviceParticles, owned: int, fields=(), positions_only=False,
                 stream=None) -> None:
         arr, n = _field_array(fields)

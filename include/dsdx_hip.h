@@ -128,6 +128,18 @@ int dsdx_halo_build(dsdx_plan* plan, const dsdx_box* box,
                     uint32_t field_count, uint64_t owned, uint64_t capacity,
                     void* stream, uint64_t* ghosts);
 
+/* migrate (dsdx_migrate_wrap) followed by dsdx_halo_build in one pass over
+ * the positions: the classification pass wraps the periodic coordinates it
+ * loaded, stores the changed ones and classifies the wrapped values.  Same
+ * results as the two calls; reads x, y, z once instead of twice.  For
+ * callers that rebuild right after migrating (the reference's
+ * communication_test step, examples/communication_test/main.cu:222-236). */
+int dsdx_migrate_halo_build(dsdx_plan* plan, const dsdx_box* box,
+                            const dsdx_particles* particles,
+                            const dsdx_field* fields, uint32_t field_count,
+                            uint64_t owned, uint64_t capacity, void* stream,
+                            uint64_t* ghosts);
+
 /* Writes the ghosts of the plan's last classification (after a
  * DSDX_ECAPACITY, with storage of at least owned + ghosts). */
 int dsdx_halo_scatter(dsdx_plan* plan, const dsdx_particles* particles,
@@ -229,6 +241,11 @@ int dsdx_batch_migrate_wrap(const dsdx_batch* batch, double* x, double* y,
 int dsdx_batch_halo_build(dsdx_batch* batch, const dsdx_particles* particles,
                           const dsdx_field* fields, uint32_t field_count,
                           void* stream);
+/* dsdx_batch_migrate_wrap + dsdx_batch_halo_build in one pass over x, y, z. */
+int dsdx_batch_migrate_halo_build(dsdx_batch* batch,
+                                  const dsdx_particles* particles,
+                                  const dsdx_field* fields,
+                                  uint32_t field_count, void* stream);
 int dsdx_batch_refresh(dsdx_batch* batch, const dsdx_particles* particles,
                        const dsdx_field* fields, uint32_t field_count,
                        int32_t positions_only, void* stream);
@@ -256,7 +273,7 @@ const uint32_t* dsdx_batch_device_totals(const dsdx_batch* batch);
  * the recorded events and returns the accumulated milliseconds and launch
  * count of one kernel: "migrate_wrap", "halo_classify", "halo_scan",
  * "halo_scatter", "reverse_halo", "reverse_exchange", "forward_exchange",
- * "halo_refresh" or "ghost_owners".  The GPU-event counterpart of the
+ * "halo_refresh", "ghost_owners" or "migrate_classify" (the fused pass).  The GPU-event counterpart of the
  * reference's host-clock timings (include/decomposition.hpp:30-38,
  * src/halo.cpp:316-323). */
 int dsdx_timing_enable(int32_t enable);

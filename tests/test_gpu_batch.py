@@ -124,8 +124,9 @@ def test_batch_matches_oracle_and_loop(gpu):
                               f"loop r{r} reverse {k}")
 
 
-@pytest.mark.parametrize("shape", [(8, 4, 1024), (4, 2, 256), (16, 8, 512)])
-def test_graph_replays_match_eager(gpu, shape):
+@pytest.mark.parametrize("shape,fused", [((8, 4, 1024), False), ((4, 2, 256), True),
+                                         ((16, 8, 512), False), ((8, 4, 1024), True)])
+def test_graph_replays_match_eager(gpu, shape, fused):
     torch = gpu
     from dsdecompx_amd.batch import ReplicaBatch, capture_step
     specs = _specs(9, 11)
@@ -158,7 +159,7 @@ def test_graph_replays_match_eager(gpu, shape):
     for k_ in F64 + I32:
         getattr(b, k_).copy_(torch.from_numpy(start[k_]).cuda())
     step_disp.zero_()
-    graph = capture_step(b, app_halo, app_rev, warmup=1)
+    graph = capture_step(b, app_halo, app_rev, warmup=1, fused=fused)
     for k_ in F64 + I32:
         getattr(b, k_).copy_(torch.from_numpy(start[k_]).cuda())
     for k in range(3):

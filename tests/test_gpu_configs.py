@@ -1,0 +1,126 @@
+"""The workloads of the benchmark at their real sizes, bit-exact against the
+oracle on identical inputs:
+
+- cfg2: 1,048,576 particles in 64 Gaussian clusters (sigma 4, L = 101.6),
+  displacement N(0, 0.5) per step (particles cross the box faces and wrap),
+  two registered
+  fields (density f64, vec3 f64) travelling with the ghosts and returned by
+  reverse_exchange(Sum); three steps.
+- cfg3: 16,777,216 particles in L = 256, periodic and with z non-periodic
+  (reflecting walls applied by the application); one full step.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from tests.gpu_util import F64, I32, assert_bits_equal, compare_particles, host, say, synth, \
+    to_device, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.set_stream(torch.cuda.Stream())
+    return torch
+
+
+def test_cfg2_clusters_fields(gpu):
+    torch = gpu
+    from dsdecompx_amd import _abi
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    n, L, w = 1_048_576, 101.6, 2.8
+    say("cfg2", n)
+    d = synth(n, L, 2002, clusters=64, sigma=4.0)
+    rng = np.random.default_rng(2003)
+    cap = int(n * 1.6)
+    dens = np.zeros(cap)
+    dens[:n] = rng.uniform(0, 2, n)
+    vec = np.zeros((cap, 3))
+    vec[:n] = rng.uniform(-1, 1, (n, 3))
+    ora = po.Oracle(po.Box(upper=(L,) * 3, width=w))
+    op = to_oracle(d, cap)
+    dev = to_device(d, cap)
+    td = torch.from_numpy(dens.copy()).cuda()
+    tv = torch.from_numpy(vec.copy()).cuda()
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
+    for step in range(3):
+        say("cfg2 step", step)
+        disp = rng.normal(0, 0.5, (3, n))
+        for a, k in enumerate(("x", "y", "z")):
+            getattr(op, k)[:n] += disp[a]
+            getattr(dev, k)[:n] += torch.from_numpy(disp[a]).cuda()
+        crossed = int(((op.x[:n] < 0) | (op.x[:n] >= L) | (op.y[:n] < 0) | (op.y[:n] >= L) |
+                       (op.z[:n] < 0) | (op.z[:n] >= L)).sum())
+        assert crossed > 0, crossed   # particles leave the box and are wrapped back
+        ora.migrate(op, n)
+        halo.migrate(dev, n)
+        g = ora.exchange_halo(op, n, fields=[po.Field(dens), po.Field(vec)])
+        assert halo.exchange_halo(dev, n, fields=[td, tv]) == g
+        compare_particles(dev, op, n + g, f"cfg2 step {step}")
+        assert_bits_equal(host(td[:n + g]), dens[:n + g], "density")
+        assert_bits_equal(host(tv[:n + g]).reshape(-1), vec[:n + g].reshape(-1), "vec")
+        contrib = rng.uniform(0, 1, g)
+        vc = rng.uniform(-1, 1, (g, 3))
+        dens[n:n + g] = contrib
+        vec[n:n + g] = vc
+        td[n:n + g] = torch.from_numpy(contrib).cuda()
+        tv[n:n + g] = torch.from_numpy(vc).cuda()
+        ora.reverse_exchange(n, [po.Field(dens, po.SUM), po.Field(vec, po.SUM)])
+        halo.reverse_exchange([(td, _abi.SUM), (tv, _abi.SUM)], n)
+        assert_bits_equal(host(td[:n]), dens[:n], "density reverse")
+        assert_bits_equal(host(tv[:n]).reshape(-1), vec[:n].reshape(-1), "vec reverse")
+        f = rng.uniform(-1, 1, (3, n + g))
+        for a, k in enumerate(("fx", "fy", "fz")):
+            getattr(op, k)[:n + g] = f[a]
+            getattr(dev, k)[:n + g] = torch.from_numpy(f[a]).cuda()
+        ora.reverse_halo(op, n)
+        halo.reverse_halo(dev, n)
+        for k in ("fx", "fy", "fz"):
+            assert_bits_equal(host(getattr(dev, k)[:n]), getattr(op, k)[:n], f"cfg2 reverse {k}")
+
+
+@pytest.mark.parametrize("zper", [True, False], ids=["periodic", "z_walls"])
+def test_cfg3_full_size_step(gpu, zper):
+    torch = gpu
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    n, L, w = 16_777_216, 256.0, 2.8
+    per = (True, True, zper)
+    say("cfg3 full size", per)
+    rng = np.random.default_rng(303 if zper else 304)
+    cap = int(n * 1.1)
+    d = {k: rng.uniform(0, L, n) for k in ("x", "y", "z")}
+    for k in ("vx", "vy", "vz", "charge", "fx", "fy", "fz"):
+        d[k] = rng.standard_normal(n)
+    d["mass"] = np.ones(n)
+    d["type"] = np.zeros(n, np.int32)
+    d["global_id"] = np.arange(n, dtype=np.int32)
+    disp = rng.normal(0, 0.1, (3, n))
+    for a, k in enumerate(("x", "y", "z")):
+        d[k] = d[k] + disp[a]
+    if not zper:   # reflecting walls, applied by the application
+        z = d["z"]
+        z[z < 0] = -z[z < 0]
+        z[z > L] = 2 * L - z[z > L]
+    op = to_oracle(d, cap)
+    dev = to_device(d, cap)
+    ora = po.Oracle(po.Box(upper=(L,) * 3, width=w, periodic=per))
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w, periodic=per))
+    ora.migrate(op, n)
+    halo.migrate(dev, n)
+    g = ora.exchange_halo(op, n)
+    assert halo.exchange_halo(dev, n) == g
+    expect = ((L + 2 * w) ** 2 * (L + (2 * w if zper else 0)) - L ** 3) / L ** 3 * n
+    assert abs(g - expect) < 0.01 * expect, (g, expect)
+    compare_particles(dev, op, n + g, "cfg3 halo")
+    f = rng.uniform(-1, 1, (3, n + g))
+    for a, k in enumerate(("fx", "fy", "fz")):
+        getattr(op, k)[:n + g] = f[a]
+        getattr(dev, k)[:n + g] = torch.from_numpy(f[a]).cuda()
+    ora.reverse_halo(op, n)
+    halo.reverse_halo(dev, n)
+    for k in ("fx", "fy", "fz"):
+        assert_bits_equal(host(getattr(dev, k)[:n]), getattr(op, k)[:n], f"cfg3 reverse {k}")
