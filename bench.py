@@ -8,7 +8,10 @@ before reverse_halo) runs too but is not part of the measured hot path: HIP
 events on the launch stream bracket migrate + exchange_halo and reverse_halo
 separately, and their sum per step is `ms_per_step`.  The wall-clock time of
 the whole loop (application work included) is reported as
-`wall_ms_per_step`.
+`wall_ms_per_step`.  Per-kernel durations (`kernels`, `roofline`) come from
+a second pass of the same number of steps in which every dispatch of the
+library carries its own HIP start / stop events; those events delay each
+dispatch by a few microseconds, so they are off in the timed pass.
 
 Prints ONE JSON line (rank 0).  See --help for the workloads.
 """
@@ -38,13 +41,15 @@ def algorithmic_bytes(n, g, u, periodic_axes):
     """Bytes each kernel must move by the path's definition: migrate reads
     and writes x,y,z on every periodic axis (16 N per axis); the rebuild
     reads x,y,z (24 N) in the classify pass and, per ghost, reads the owner's
-    72-byte record and writes the 96-byte ghost record in the scatter pass;
-    reverse_halo reads 24 B per ghost and reads + writes 48 B per owner."""
+    72-byte record and writes the 96-byte ghost record in the copy pass (the
+    scan and the route pass move plan bookkeeping only); reverse_halo reads
+    24 B per ghost and reads + writes 48 B per owner."""
     return {
         "migrate_wrap": 16.0 * n * periodic_axes,
         "halo_classify": 24.0 * n,
         "halo_scan": 0.0,
-        "halo_scatter": 168.0 * g,
+        "halo_scatter": 0.0,
+        "halo_copy": 168.0 * g,
         "reverse_halo": 24.0 * g + 48.0 * u,
     }
 
@@ -138,8 +143,7 @@ def measure_abi(args, n, L, w, sigma, periodic):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    _abi.timing_reset()
-    _abi.timing_enable(True)
+    # timed pass with the per-kernel timers off, then a pass with them on
     timers = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
     ghosts, owners = [], []
     torch.cuda.synchronize()
@@ -149,11 +153,17 @@ def measure_abi(args, n, L, w, sigma, periodic):
         owners.append(_owners(halo))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_wall
+    hot_ms = sum(t[0].elapsed_time(t[1]) + t[2].elapsed_time(t[3]) for t in timers)
+    _abi.timing_reset()
+    _abi.timing_enable(True)
+    for s in range(args.steps):
+        step()
+    torch.cuda.synchronize()
     _abi.timing_enable(False)
     kern = _abi.timing_read()
-    hot_ms = sum(t[0].elapsed_time(t[1]) + t[2].elapsed_time(t[3]) for t in timers)
     return {"hot_ms_per_step": hot_ms / args.steps, "wall_ms_per_step": 1e3 * wall / args.steps,
             "ghosts_per_step": sum(ghosts) / len(ghosts), "owners": sum(owners) / len(owners),
+            "kernel_steps": args.steps,
             "kernels": {k: {"total_ms": v[0], "launches": v[1]} for k, v in kern.items()},
             "path": "C ABI from Python (dsdx_migrate_wrap, dsdx_halo_build, dsdx_reverse_halo)"}
 
@@ -196,7 +206,7 @@ def measure_cpp(args, n, L, w, sigma, periodic):
         shape, tuned = tune_shape(args, n, L, w, periodic)
         env.update(autotune.env_for(shape))
     cmd = [exe, str(n), repr(L), repr(w), repr(sigma), "1" if periodic[2] else "0",
-           str(args.steps), str(args.warmup), str(args.seed)]
+           str(args.steps), str(args.warmup), str(args.seed), "0", str(args.steps)]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
     if r.returncode != 0:
         raise SystemExit(f"comm_bench failed ({r.returncode}): {r.stderr[-2000:]}")
@@ -382,6 +392,7 @@ def run(args):
         "ghosts_per_step": G,
         "owners_per_step": U,
         "wall_ms_per_step": res["wall_ms_per_step"],
+        "kernel_pass_ms_per_step": res.get("kernel_pass_hot_ms_per_step"),
         "kernels": per_kernel,
         "autotune": res.get("autotune"),
     }
@@ -389,24 +400,7 @@ def run(args):
                                                                  args.cpu_steps)
     if not args.no_batched:
         out["batched"] = measure_batched(args)
-    if args.copy_calibration:
-        out["copy_calibration_bytes"] = copy_calibration(args.copy_calibration)
     return out
-
-
-def copy_calibration(nbytes):
-    """One plain device-to-device copy of `nbytes` (torch's copy kernel),
-    after the measured steps: a kernel of known traffic (nbytes read, nbytes
-    written) against which rocprofv3's FETCH_SIZE / WRITE_SIZE counters of
-    the same run are scaled."""
-    import torch
-    n = int(nbytes) // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    b.copy_(a)
-    torch.cuda.synchronize()
-    return 8 * n
 
 
 def _owners(halo):
@@ -431,8 +425,6 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the cfg5 batched comparison")
-    ap.add_argument("--copy-calibration", type=float, default=0,
-                    help="after the run, copy this many bytes once (counter calibration)")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="keep the default launch shape instead of autotuning it")
     ap.add_argument("--replicas", type=int, default=512)

@@ -136,9 +136,9 @@ def exported_symbols():
             "dsdx_batch_migrate_halo_build"]
 
 
-KERNELS = ("migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "reverse_halo",
-           "reverse_exchange", "forward_exchange", "halo_refresh", "ghost_owners",
-           "migrate_classify")
+KERNELS = ("migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "halo_copy",
+           "reverse_halo", "reverse_exchange", "forward_exchange", "halo_refresh",
+           "ghost_owners", "migrate_classify")
 
 
 def timing_enable(on: bool = True) -> None:

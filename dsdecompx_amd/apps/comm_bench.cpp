@@ -5,11 +5,15 @@
 // -> reverse_halo through dsdecompx::Decomp on a HIP stream.
 //
 // HIP events on that stream bracket migrate + exchange_halo and reverse_halo;
-// their sum is the hot-path time.  The per-kernel HIP-event timers of the
-// library (dsdx_timing_*) give each kernel's average duration.  Prints one
-// JSON object on stdout.
+// their sum over `steps` steps is the hot-path time.  A second pass of
+// `kernel_steps` steps runs with the library's per-kernel timers on
+// (dsdx_timing_*: each dispatch carries its own start / stop events) and
+// gives each kernel's average duration.  An optional plain copy of
+// `calibration_bytes` at the end is the reference dispatch for rocprofv3's
+// FETCH_SIZE / WRITE_SIZE counters.  Prints one JSON object on stdout.
 //
 // usage: comm_bench particles L width sigma z_periodic steps warmup seed
+//                   [calibration_bytes [kernel_steps]]
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -112,6 +116,15 @@ __global__ void count_boundary(const double* x, const double* y, const double* z
     if ((threadIdx.x & 63) == 0 && ballot) atomicAdd(out, static_cast<unsigned long long>(__popcll(ballot)));
 }
 
+// A plain copy of known traffic (n doubles read, n written): rocprofv3's
+// FETCH_SIZE / WRITE_SIZE of this dispatch scale the counters of the halo
+// kernels in the same run to bytes.
+__global__ void calibration_copy(const double* __restrict__ src, double* __restrict__ dst, uint64_t n)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 unsigned grid(uint64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
 }  // namespace
@@ -119,7 +132,10 @@ unsigned grid(uint64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 int main(int argc, char** argv)
 {
     if (argc < 9) {
-        std::fprintf(stderr, "usage: %s particles L width sigma z_periodic steps warmup seed\n", argv[0]);
+        std::fprintf(stderr,
+                     "usage: %s particles L width sigma z_periodic steps warmup seed [calibration_bytes "
+                     "[kernel_steps]]\n",
+                     argv[0]);
         return 2;
     }
     const uint64_t n = std::strtoull(argv[1], nullptr, 10);
@@ -130,6 +146,8 @@ int main(int argc, char** argv)
     const int steps = std::atoi(argv[6]);
     const int warmup = std::atoi(argv[7]);
     const uint64_t seed = std::strtoull(argv[8], nullptr, 10);
+    const uint64_t calibration = argc > 9 ? std::strtoull(argv[9], nullptr, 10) / 8 : 0;
+    const int kernel_steps = argc > 10 ? std::atoi(argv[10]) : steps;
 
     Config config;
     config.domain.upper = {L, L, L};
@@ -147,44 +165,57 @@ int main(int argc, char** argv)
 
     hipEvent_t ev[4];
     for (auto& e : ev) ok(hipEventCreate(&e), "event");
-    double hot_ms = 0.0;
-    double ghosts_sum = 0.0;
+    uint64_t step = 0;
     uint64_t last_ghosts = 0;
-    std::chrono::steady_clock::time_point wall0;
-    for (int step = 0; step < warmup + steps; ++step) {
-        const bool timed = step >= warmup;
-        if (step == warmup) {
-            ok(hipStreamSynchronize(s), "sync");
-            dsdx_timing_reset();
-            dsdx_timing_enable(1);
-            wall0 = std::chrono::steady_clock::now();
+    // One pass of `count` steps; returns the hot-path milliseconds summed over
+    // the steps and adds each step's ghost count to *ghosts.
+    auto pass = [&](int count, bool timed, double* ghosts) {
+        double hot = 0.0;
+        for (int k = 0; k < count; ++k, ++step) {
+            hipLaunchKernelGGL(move, dim3(grid(n)), dim3(256), 0, s, p.x_data(), p.y_data(), p.z_data(),
+                               n, sigma, seed, step, zper ? 1 : 0, L);
+            if (timed) ok(hipEventRecord(ev[0], s), "record");
+            decomp.migrate(p, s);
+            const HaloResult r = decomp.exchange_halo(p, {true, s});
+            if (timed) ok(hipEventRecord(ev[1], s), "record");
+            const uint64_t total = n + r.ghost_count;
+            hipLaunchKernelGGL(fill_forces, dim3(grid(total)), dim3(256), 0, s, p.fx_data(), p.fy_data(),
+                               p.fz_data(), total, seed, step);
+            if (timed) ok(hipEventRecord(ev[2], s), "record");
+            decomp.reverse_halo(p, s);
+            if (timed) {
+                ok(hipEventRecord(ev[3], s), "record");
+                ok(hipEventSynchronize(ev[3]), "sync");
+                float a = 0.0f, b = 0.0f;
+                ok(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
+                ok(hipEventElapsedTime(&b, ev[2], ev[3]), "elapsed");
+                hot += a + b;
+            }
+            if (ghosts) *ghosts += static_cast<double>(r.ghost_count);
+            last_ghosts = r.ghost_count;
         }
-        hipLaunchKernelGGL(move, dim3(grid(n)), dim3(256), 0, s, p.x_data(), p.y_data(), p.z_data(),
-                           n, sigma, seed, static_cast<uint64_t>(step), zper ? 1 : 0, L);
-        if (timed) ok(hipEventRecord(ev[0], s), "record");
-        decomp.migrate(p, s);
-        const HaloResult r = decomp.exchange_halo(p, {true, s});
-        if (timed) ok(hipEventRecord(ev[1], s), "record");
-        const uint64_t total = n + r.ghost_count;
-        hipLaunchKernelGGL(fill_forces, dim3(grid(total)), dim3(256), 0, s, p.fx_data(), p.fy_data(),
-                           p.fz_data(), total, seed, static_cast<uint64_t>(step));
-        if (timed) ok(hipEventRecord(ev[2], s), "record");
-        decomp.reverse_halo(p, s);
-        if (timed) {
-            ok(hipEventRecord(ev[3], s), "record");
-            ok(hipEventSynchronize(ev[3]), "sync");
-            float a = 0.0f, b = 0.0f;
-            ok(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
-            ok(hipEventElapsedTime(&b, ev[2], ev[3]), "elapsed");
-            hot_ms += a + b;
-            ghosts_sum += static_cast<double>(r.ghost_count);
-        }
-        last_ghosts = r.ghost_count;
-    }
+        return hot;
+    };
+
+    pass(warmup, false, nullptr);
+    ok(hipStreamSynchronize(s), "sync");
+    // Timed pass: the library's per-kernel timers stay off (a timed dispatch
+    // is preceded by several microseconds of idle time).
+    double ghosts_sum = 0.0;
+    const auto wall0 = std::chrono::steady_clock::now();
+    const double hot_ms = pass(steps, true, &ghosts_sum);
     ok(hipStreamSynchronize(s), "sync");
     const double wall_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
-    dsdx_timing_enable(0);
+    // Kernel pass: the same steps again with every kernel's dispatch timed.
+    double kernel_pass_hot_ms = 0.0;
+    if (kernel_steps > 0) {
+        dsdx_timing_reset();
+        dsdx_timing_enable(1);
+        kernel_pass_hot_ms = pass(kernel_steps, true, nullptr);
+        ok(hipStreamSynchronize(s), "sync");
+        dsdx_timing_enable(0);
+    }
 
     unsigned long long* d_u = nullptr;
     ok(hipMalloc(&d_u, sizeof(*d_u)), "malloc");
@@ -195,14 +226,26 @@ int main(int argc, char** argv)
     ok(hipMemcpyAsync(&owners, d_u, sizeof(owners), hipMemcpyDeviceToHost, s), "copy");
     ok(hipStreamSynchronize(s), "sync");
     ok(hipFree(d_u), "free");
+    if (calibration) {
+        double *src = nullptr, *dst = nullptr;
+        ok(hipMalloc(&src, calibration * 8), "malloc");
+        ok(hipMalloc(&dst, calibration * 8), "malloc");
+        hipLaunchKernelGGL(calibration_copy, dim3(grid(calibration)), dim3(256), 0, s, src, dst, calibration);
+        ok(hipStreamSynchronize(s), "calibration copy");
+        ok(hipFree(src), "free");
+        ok(hipFree(dst), "free");
+    }
 
-    const char* kernels[] = {"migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "reverse_halo"};
+    const char* kernels[] = {"migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "halo_copy",
+                             "reverse_halo"};
     std::printf("{\"particles\": %llu, \"steps\": %d, \"warmup\": %d, \"hot_ms_per_step\": %.6f, "
                 "\"wall_ms_per_step\": %.6f, \"ghosts_per_step\": %.1f, \"last_ghosts\": %llu, "
-                "\"owners\": %llu, \"kernels\": {",
+                "\"owners\": %llu, \"kernel_steps\": %d, \"kernel_pass_hot_ms_per_step\": %.6f, "
+                "\"kernels\": {",
                 static_cast<unsigned long long>(n), steps, warmup, hot_ms / steps, wall_ms / steps,
-                ghosts_sum / steps, static_cast<unsigned long long>(last_ghosts), owners);
-    for (int k = 0; k < 5; ++k) {
+                ghosts_sum / steps, static_cast<unsigned long long>(last_ghosts), owners, kernel_steps,
+                kernel_steps > 0 ? kernel_pass_hot_ms / kernel_steps : 0.0);
+    for (int k = 0; k < 6; ++k) {
         double ms = 0.0;
         uint64_t launches = 0;
         dsdx_timing_read(kernels[k], &ms, &launches);

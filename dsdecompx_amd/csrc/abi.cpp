@@ -181,14 +181,18 @@ int reserve_buffers(Plan* plan, uint64_t ghost_list_elems)
 {
     const uint64_t c = std::max<uint64_t>(1, plan->chunks);
     const uint64_t slots = c * plan->chunk_size();
+    // entry indices (chunk * K + rank) are 32-bit
+    if (slots > UINT32_MAX) return fail(DSDX_EINVAL, "too many particles for one device plan");
     const uint64_t replicas = std::max<size_t>(1, plan->desc.size());
     DSDX_HIP(plan->entries.reserve(slots * 4), "hipMalloc");
+    DSDX_HIP(plan->bpos.reserve(slots * 3 * sizeof(double)), "hipMalloc");
     DSDX_HIP(plan->counts.reserve(c * kCols * 4), "hipMalloc");
     DSDX_HIP(plan->prefix.reserve(c * kCols * 4), "hipMalloc");
     DSDX_HIP(plan->totals.reserve(replicas * kTotals * 4), "hipMalloc");
     DSDX_HIP(plan->owner_index.reserve(slots * 4), "hipMalloc");
     DSDX_HIP(plan->owner_code.reserve(slots), "hipMalloc");
     DSDX_HIP(plan->image_start.reserve(slots * 4), "hipMalloc");
+    DSDX_HIP(plan->owner_entry.reserve(slots * 4), "hipMalloc");
     if (ghost_list_elems > plan->ghost_list_capacity) {
         DSDX_HIP(plan->ghost_list.reserve(ghost_list_elems * 4), "hipMalloc");
         plan->ghost_list_capacity = plan->ghost_list.bytes / 4;
@@ -197,11 +201,17 @@ int reserve_buffers(Plan* plan, uint64_t ghost_list_elems)
     if (words > plan->host_totals_words) {
         if (plan->host_totals) (void)hipHostFree(plan->host_totals);
         plan->host_totals = nullptr;
+        plan->host_totals_dev = nullptr;
         plan->host_totals_words = 0;
         void* host = nullptr;
-        DSDX_HIP(hipHostMalloc(&host, words * 4, hipHostMallocDefault), "hipHostMalloc");
+        // mapped and coherent: the scan kernel stores the counts straight into it
+        DSDX_HIP(hipHostMalloc(&host, words * 4, hipHostMallocMapped | hipHostMallocCoherent),
+                 "hipHostMalloc");
         plan->host_totals = static_cast<uint32_t*>(host);
         plan->host_totals_words = words;
+        void* dev = nullptr;
+        DSDX_HIP(hipHostGetDevicePointer(&dev, host, 0), "hipHostGetDevicePointer");
+        plan->host_totals_dev = static_cast<uint32_t*>(dev);
     }
     return DSDX_OK;
 }
@@ -225,6 +235,26 @@ void number_chunks(Plan* plan)
     plan->chunks = next;
 }
 
+// Owner tiles of every replica: ceil(bound_r / kOwnerTile).  A replica that
+// builds without overflow has at most min(owned, ghosts) boundary owners and
+// at most min(capacity - owned, ghost list capacity) ghosts.
+void number_tiles(Plan* plan)
+{
+    uint32_t next = 0;
+    plan->tile_replica.clear();
+    for (size_t r = 0; r < plan->desc.size(); ++r) {
+        ReplicaDesc& d = plan->desc[r];
+        const uint64_t bound = std::min<uint64_t>(
+            {static_cast<uint64_t>(d.owned), d.capacity - d.owned, d.ghost_list_capacity});
+        const uint32_t tiles = static_cast<uint32_t>((bound + kOwnerTile - 1) / kOwnerTile);
+        d.first_tile = next;
+        next += tiles;
+        if (plan->desc.size() > 1)
+            plan->tile_replica.insert(plan->tile_replica.end(), tiles, static_cast<uint32_t>(r));
+    }
+    plan->tiles = next;
+}
+
 // Copies the table to the device and waits (the host copy may change after).
 int upload_table(Plan* plan, hipStream_t stream)
 {
@@ -239,6 +269,12 @@ int upload_table(Plan* plan, hipStream_t stream)
             DSDX_HIP(hipMemcpyAsync(plan->d_chunk_replica.ptr, plan->chunk_replica.data(),
                                     plan->chunk_replica.size() * 4, hipMemcpyHostToDevice, stream),
                      "upload chunk table");
+        DSDX_HIP(plan->d_tile_replica.reserve(std::max<size_t>(1, plan->tile_replica.size()) * 4),
+                 "hipMalloc");
+        if (!plan->tile_replica.empty())
+            DSDX_HIP(hipMemcpyAsync(plan->d_tile_replica.ptr, plan->tile_replica.data(),
+                                    plan->tile_replica.size() * 4, hipMemcpyHostToDevice, stream),
+                     "upload tile table");
     }
     DSDX_HIP(hipStreamSynchronize(stream), "wait for the replica table");
     plan->desc_on_device = true;
@@ -263,9 +299,29 @@ int set_single(Plan* plan, const BoxParams& bp, uint64_t owned, uint64_t capacit
     if (ghost_list_elems) {
         if (int rc = reserve_buffers(plan, ghost_list_elems)) return rc;
         plan->desc[0].ghost_list_capacity = plan->ghost_list_capacity;
+        number_tiles(plan);
     }
     if (!plan->desc_on_device || std::memcmp(&before, &plan->desc[0], sizeof(ReplicaDesc)) != 0)
         return upload_table(plan, stream);
+    return DSDX_OK;
+}
+
+// Single-box plans: the scan also writes the column totals into pinned host
+// memory, and the host waits for the scan only, never for the ghost records
+// (the reference's rebuild likewise waits for the classification counts
+// alone, src/halo.cpp:482-491).  The overflow status is derived here exactly
+// as the route pass derives it on the device.
+int wait_counts(Plan* plan)
+{
+    DSDX_HIP(hipEventSynchronize(plan->counted), "wait for halo counts");
+    uint32_t* t = plan->host_totals;
+    const ReplicaDesc& d = plan->desc[0];
+    const uint64_t ghosts = t[kColImages];
+    uint32_t status = 0;
+    if (d.owned + ghosts > d.capacity) status |= kOverflowStorage;
+    if (ghosts > d.ghost_list_capacity) status |= kOverflowGhostList;
+    t[kStatusWord] = status;
+    plan->totals_cached = true;
     return DSDX_OK;
 }
 
@@ -292,15 +348,27 @@ int run_classify(Plan* plan, const dsdx_particles* p, bool wrap_first, hipStream
         timing::Scope timed(timing::kScan, s);
         DSDX_HIP(launch_scan(L, B, plan->shape, s), "launch halo scan");
     }
+    if (plan->desc.size() == 1) {
+        if (!plan->counted)
+            DSDX_HIP(hipEventCreateWithFlags(&plan->counted, hipEventDisableTiming), "hipEventCreate");
+        DSDX_HIP(hipEventRecord(plan->counted, s), "record halo counts");
+    }
     plan->classified = true;
     return DSDX_OK;
 }
 
 int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipStream_t s)
 {
-    timing::Scope timed(timing::kScatter, s);
-    DSDX_HIP(launch_scatter(*p, fp, plan->layout(), plan->buffers(), plan->shape, s),
-             "launch halo scatter");
+    const Layout L = plan->layout();
+    const PlanBuffers B = plan->buffers();
+    {
+        timing::Scope timed(timing::kScatter, s);
+        DSDX_HIP(launch_route(L, B, plan->shape, s), "launch halo route");
+    }
+    {
+        timing::Scope timed(timing::kCopy, s);
+        DSDX_HIP(launch_copy(*p, fp, L, B, plan->shape, s), "launch halo copy");
+    }
     return DSDX_OK;
 }
 
@@ -309,6 +377,7 @@ int shape_from_env(Shape* s)
     if (const char* v = std::getenv("DSDX_ITEMS")) s->items = std::atoi(v);
     if (const char* v = std::getenv("DSDX_WAVES")) s->waves = std::atoi(v);
     if (const char* v = std::getenv("DSDX_SCAN_THREADS")) s->scan_threads = std::atoi(v);
+    if (const char* v = std::getenv("DSDX_XCD_RUN")) s->xcd_run = std::atoi(v);
     return shape_supported(*s) ? DSDX_OK : fail(DSDX_EINVAL, "unsupported launch shape");
 }
 
@@ -337,7 +406,7 @@ int rescatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, uint64
     if (int rc = set_single(plan, bp, owned, capacity, ghosts + ghosts / 4 + 1024, stream)) return rc;
     DSDX_HIP(hipMemsetAsync(plan->totals.as<uint32_t>() + kStatusWord, 0, 4, stream), "clear halo status");
     if (int rc = run_scatter(plan, p, fp, stream)) return rc;
-    if (int rc = read_totals(plan, stream)) return rc;
+    if (int rc = wait_counts(plan)) return rc;
     if (plan->host_totals[kStatusWord] != 0) return fail(DSDX_EDEVICE, "halo scatter overflow after growth");
     plan->built = true;
     return DSDX_OK;
@@ -401,7 +470,7 @@ int dsdx_plan_reserve(dsdx_plan* plan, uint64_t owned, uint64_t ghosts)
 int dsdx_plan_set_shape(dsdx_plan* plan, int32_t items, int32_t waves, int32_t scan_threads)
 {
     if (!plan) return fail(DSDX_EINVAL, "null plan");
-    const Shape s{items, waves, scan_threads};
+    const Shape s{items, waves, scan_threads, plan->shape.xcd_run};
     if (!shape_supported(s)) return fail(DSDX_EINVAL, "unsupported launch shape");
     if (s.items != plan->shape.items) {
         plan->classified = plan->built = plan->totals_cached = false;
@@ -494,7 +563,7 @@ static int halo_build(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles
         plan->built = true;
         return DSDX_OK;
     }
-    if (int rc = read_totals(plan, s)) return rc;
+    if (int rc = wait_counts(plan)) return rc;
     const uint64_t g = plan->host_totals[kColImages];
     const uint32_t status = plan->host_totals[kStatusWord];
     *ghosts = g;
@@ -519,7 +588,7 @@ int dsdx_halo_scatter(dsdx_plan* plan, const dsdx_particles* particles, const ds
     FieldParams fp;
     if (int rc = make_field_params(fields, field_count, &fp)) return rc;
     const hipStream_t s = as_stream(stream);
-    if (int rc = read_totals(plan, s)) return rc;
+    if (int rc = wait_counts(plan)) return rc;
     return rescatter(plan, particles, fp, capacity, s, plan->host_totals[kColImages]);
 }
 
@@ -558,7 +627,7 @@ int dsdx_plan_query(const dsdx_plan* plan, uint32_t* send_counts, uint32_t* rece
         // A null stream returns the counts of the last waited build without
         // touching the device.
         if (stream) {
-            if (int rc = read_totals(const_cast<dsdx_plan*>(plan), as_stream(stream))) return rc;
+            if (int rc = wait_counts(const_cast<dsdx_plan*>(plan))) return rc;
         } else if (!plan->totals_cached) {
             return fail(DSDX_EINVAL, "plan query without a stream needs a waited build");
         }
@@ -669,6 +738,7 @@ int dsdx_batch_create(const dsdx_replica* replicas, uint32_t count, void* stream
     }
     if (!rc) {
         number_chunks(b);
+        number_tiles(b);
         rc = reserve_buffers(b, std::max<uint64_t>(1, ghost_list));
     }
     if (!rc) rc = upload_table(b, as_stream(stream));
@@ -691,7 +761,7 @@ int dsdx_batch_set_shape(dsdx_batch* batch, int32_t items, int32_t waves, int32_
 {
     if (!batch) return fail(DSDX_EINVAL, "null batch");
     if (!stream) return fail(DSDX_EINVAL, "batch setup requires an explicit stream");
-    const Shape s{items, waves, scan_threads};
+    const Shape s{items, waves, scan_threads, batch->shape.xcd_run};
     if (!shape_supported(s)) return fail(DSDX_EINVAL, "unsupported launch shape");
     const bool relayout = s.items != batch->shape.items;
     batch->shape = s;

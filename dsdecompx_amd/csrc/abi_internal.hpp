@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -42,19 +43,24 @@ struct Plan {
     Shape shape;
     std::vector<ReplicaDesc> desc;        // host copy of the table on the device
     std::vector<uint32_t> chunk_replica;  // host copy (R > 1 only)
-    DeviceBuffer d_desc, d_chunk_replica;
+    std::vector<uint32_t> tile_replica;   // host copy (R > 1 only)
+    DeviceBuffer d_desc, d_chunk_replica, d_tile_replica;
     bool desc_on_device = false;
     uint32_t chunks = 0;
     uint32_t max_replica_chunks = 0;
+    uint32_t tiles = 0;
 
     bool classified = false;   // classify + scan of the current table are on the device
     bool built = false;        // ghosts and owner map written: reverse/forward usable
     bool totals_cached = false;
 
-    DeviceBuffer entries, counts, prefix, totals, owner_index, owner_code, image_start, ghost_list;
+    DeviceBuffer entries, bpos, counts, prefix, totals, owner_index, owner_code, image_start,
+        ghost_list, owner_entry;
     uint64_t ghost_list_capacity = 0;   // elements
     uint32_t* host_totals = nullptr;    // pinned copy of totals (R x kTotals)
+    uint32_t* host_totals_dev = nullptr;  // the same memory, as the device addresses it
     size_t host_totals_words = 0;
+    hipEvent_t counted = nullptr;       // recorded after the scan of a single-box classify
 
     Plan() = default;
     Plan(const Plan&) = delete;
@@ -62,6 +68,7 @@ struct Plan {
     ~Plan()
     {
         if (host_totals) (void)hipHostFree(host_totals);
+        if (counted) (void)hipEventDestroy(counted);
     }
 
     uint32_t chunk_size() const { return static_cast<uint32_t>(shape.items) * kWave; }
@@ -71,18 +78,24 @@ struct Plan {
         Layout L;
         L.desc = d_desc.as<ReplicaDesc>();
         L.chunk_replica = desc.size() > 1 ? d_chunk_replica.as<uint32_t>() : nullptr;
+        L.tile_replica = desc.size() > 1 ? d_tile_replica.as<uint32_t>() : nullptr;
         L.replicas = static_cast<uint32_t>(desc.size());
         L.chunks = chunks;
         L.max_replica_chunks = max_replica_chunks;
         L.chunk_size = chunk_size();
+        L.tiles = tiles;
+        L.xcd_run = static_cast<uint32_t>(shape.xcd_run);
         return L;
     }
 
     PlanBuffers buffers() const
     {
-        return {entries.as<uint32_t>(), counts.as<uint32_t>(), prefix.as<uint32_t>(),
+        return {entries.as<uint32_t>(), bpos.as<double>(),
+                static_cast<uint64_t>(std::max<uint32_t>(1, chunks)) * chunk_size(),
+                counts.as<uint32_t>(), prefix.as<uint32_t>(),
                 totals.as<uint32_t>(), owner_index.as<uint32_t>(), owner_code.as<uint8_t>(),
-                image_start.as<uint32_t>(), ghost_list.as<uint32_t>()};
+                image_start.as<uint32_t>(), ghost_list.as<uint32_t>(),
+                owner_entry.as<uint32_t>(), desc.size() == 1 ? host_totals_dev : nullptr};
     }
 };
 
@@ -92,10 +105,12 @@ int make_typed_fields(const dsdx_field* fields, uint32_t count, TypedFields* tf)
 int check_particles(const dsdx_particles* p, uint64_t owned);
 int reserve_buffers(Plan* plan, uint64_t ghost_list_elems);
 void number_chunks(Plan* plan);
+void number_tiles(Plan* plan);
 int upload_table(Plan* plan, hipStream_t stream);
 int set_single(Plan* plan, const BoxParams& bp, uint64_t owned, uint64_t capacity,
                uint64_t ghost_list_elems, hipStream_t stream);
 int read_totals(Plan* plan, hipStream_t stream);
+int wait_counts(Plan* plan);
 int run_classify(Plan* plan, const dsdx_particles* p, bool wrap_first, hipStream_t s);
 int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipStream_t s);
 int shape_from_env(Shape* s);

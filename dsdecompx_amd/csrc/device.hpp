@@ -10,6 +10,8 @@
 // Plan buffers in HBM (u32 unless noted), C = total chunks, K = chunk size:
 //   entries[C * K]          per chunk, its boundary particles compacted in
 //                           owned order: (index in chunk << 8) | slab code
+//   bpos[3][C * K] (f64)    the positions of those entries (written by the
+//                           classify pass, read densely by the scatter)
 //   counts[kCols][C]        per chunk: send count of each of the 26 slots,
 //                           boundary owners, images (column-major)
 //   prefix[kCols][C]        exclusive scan of counts over each replica's chunks
@@ -20,6 +22,7 @@
 //   ghost_list[...]         per replica at ghost_list_offset_r: per owner its
 //                           ghosts (index relative to the replica's owned
 //                           count) in ascending send-slot order
+//   owner_entry[C * K]      ... -> its entry index (chunk * K + rank)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -63,7 +66,7 @@ struct ReplicaDesc {
     uint32_t owned;
     uint32_t first_chunk;
     uint32_t chunks;
-    uint32_t pad;
+    uint32_t first_tile;           // first owner tile (kOwnerTile owner slots each)
 };
 
 // Launch shape of the classify / scatter / owner-major kernels: items per
@@ -72,19 +75,31 @@ struct Shape {
     int items = 8;
     int waves = 4;
     int scan_threads = 1024;
+    int xcd_run = 8;    // consecutive blocks per XCD turn (0: hardware order)
 };
+
+// Owner tiles: the owner slots [0, bound_r) of replica r (relative to
+// first_chunk_r * K), bound_r = min(owned_r, capacity_r - owned_r,
+// ghost_list_capacity_r) >= its boundary owners, cut into tiles of
+// kOwnerTile; one wave of the copy pass per tile.
+constexpr uint32_t kOwnerTile = kWave;
 
 struct Layout {
     const ReplicaDesc* desc;        // device, one per replica
     const uint32_t* chunk_replica;  // device, replica of each chunk; null for one replica
+    const uint32_t* tile_replica;   // device, replica of each owner tile; null for one replica
     uint32_t replicas;
     uint32_t chunks;                // total over all replicas
     uint32_t max_replica_chunks;    // largest chunks_r (scan work per block)
-    uint32_t chunk_size;            // kWave * items
+    uint32_t chunk_size;            // kWave * items (a power of two)
+    uint32_t tiles;                 // owner tiles over all replicas
+    uint32_t xcd_run;               // block renumbering (see xcd_block)
 };
 
 struct PlanBuffers {
     uint32_t* entries;
+    double* bpos;          // x, y, z of each entry (3 planes of bpos_stride doubles)
+    uint64_t bpos_stride;  // C * K
     uint32_t* counts;
     uint32_t* prefix;
     uint32_t* totals;
@@ -92,6 +107,8 @@ struct PlanBuffers {
     uint8_t* owner_code;
     uint32_t* image_start;
     uint32_t* ghost_list;
+    uint32_t* owner_entry;   // owner slot -> its entry index (chunk * K + rank)
+    uint32_t* host_totals;   // null, or pinned host memory the scan also writes (single box)
 };
 
 struct FieldParams {

@@ -6,10 +6,13 @@
 // Every kernel walks a batch of replicas (device/halo layout in device.hpp);
 // one wave owns one chunk of one replica, so replica parameters are
 // wave-uniform and come in through scalar loads.
+#include <hip/hip_ext.h>
+
 #include <type_traits>
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "timing.hpp"
 
 namespace dsdx {
 namespace {
@@ -43,16 +46,39 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v)
     return v;
 }
 
-// The chunk of this wave and its replica (both wave-uniform).
-template <int WAVES>
-__device__ __forceinline__ uint32_t wave_chunk()
+// MI355X places block b of a launch on XCD b % 8, and each XCD has its own
+// L2.  Blocks are renumbered so that each XCD works through runs of `run`
+// consecutive blocks (the XCDs take the runs in turn): neighbouring chunks
+// share cache lines (compacted entries, owner maps, ghost runs, count
+// columns), and a line touched by one XCD only is fetched and written back
+// once.  The last partial round of runs keeps the hardware order; run 0
+// keeps it everywhere.  A bijection on [0, gridDim.x).
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_block(uint32_t run)
 {
-    return uniform(blockIdx.x * WAVES + threadIdx.x / kWave);
+    const uint32_t b = blockIdx.x;
+    if (run == 0) return b;
+    const uint32_t group = kXcds * run;
+    if (b >= gridDim.x - gridDim.x % group) return b;
+    const uint32_t l = b % group;
+    return b - l + (l % kXcds) * run + l / kXcds;
 }
 
-__device__ __forceinline__ uint32_t replica_of(const Layout& L, uint32_t chunk)
+// The chunk (or tile) of this wave (wave-uniform).
+template <int WAVES>
+__device__ __forceinline__ uint32_t wave_chunk(uint32_t run)
 {
-    return L.chunk_replica ? uniform(L.chunk_replica[chunk]) : 0u;
+    return uniform(xcd_block(run) * WAVES + threadIdx.x / kWave);
+}
+
+// Replica of a chunk.  The descriptor table and the chunk -> replica map
+// reach every kernel as __restrict__ parameters: nothing the kernel stores
+// aliases them, so their wave-uniform reads compile to scalar loads.
+__device__ __forceinline__ uint32_t replica_of(const uint32_t* __restrict__ chunk_replica,
+                                               uint32_t chunk)
+{
+    return chunk_replica ? uniform(chunk_replica[chunk]) : 0u;
 }
 
 __device__ __forceinline__ uint32_t slab_code(double px, double py, double pz, const BoxParams& bp)
@@ -73,6 +99,11 @@ __device__ __forceinline__ uint32_t slab_code(double px, double py, double pz, c
 __device__ __forceinline__ double wrap(double value, double lower, double extent)
 {
     const double v = __dsub_rn(value, lower);
+    // 0 < v < extent: the rounded quotient v / extent lies in (0, 1) (a
+    // quotient just below 1 rounds to at most 1 - 2^-53), so floor gives +0,
+    // 0 * extent is +0 and v - 0 == v exactly.  The result is then v + lower,
+    // bit for bit, without the division.
+    if (v > 0.0 && v < extent) return __dadd_rn(v, lower);
     const double q = floor(__ddiv_rn(v, extent));
     return __dadd_rn(__dsub_rn(v, __dmul_rn(q, extent)), lower);
 }
@@ -88,86 +119,129 @@ __device__ __forceinline__ void wrap_store(double& v, double* dst, bool valid, i
     v = w;
 }
 
+// Loads v[k] = src[first + 64 k + lane] for the chunk's items (0 past
+// `owned`); a chunk entirely below `owned` takes the unpredicated loads.
+template <int ITEMS>
+__device__ __forceinline__ void load_items(const double* __restrict__ src, uint32_t first,
+                                           uint32_t owned, bool full, double (&v)[ITEMS])
+{
+    const uint32_t lane = lane_id();
+    if (full) {
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) v[k] = src[first + k * kWave + lane];
+    } else {
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t i = first + k * kWave + lane;
+            v[k] = i < owned ? src[i] : 0.0;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
-// Pass 1: one wave per chunk.  Slab-classifies every owned particle,
+// Pass 1: one wave per chunk.  Slab-classifies every owned particle and
 // compacts the boundary ones (non-empty slot set) into the chunk's entry
-// region in owned order, and counts per send slot through an LDS histogram of
-// slab codes (one LDS atomic per boundary particle).  Reads 24 B per
-// particle, writes 4 B per boundary particle and 28 counters per chunk.
+// region in owned order, with their positions; counts the sends of each slot
+// with one LDS atomic per (boundary particle, slot).  Every LDS array is
+// per wave, so no block barrier is needed.  The slot-set table lut[64] sits one
+// entry per lane and is read with a cross-lane permute.  Reads 24 B per
+// particle, writes 28 B per boundary particle and 28 counters per chunk.
 // With WRAP the pass first applies migrate's periodic wrap to the positions
 // it loaded (storing the changed ones) and classifies the wrapped values:
 // migrate followed by a rebuild with one read of x, y, z.
 template <int ITEMS, int WAVES, bool WRAP>
 __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
-    double* __restrict__ x, double* __restrict__ y, double* __restrict__ z, Layout L,
+    double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+    const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ chunk_replica, Layout L,
     PlanBuffers B)
 {
     constexpr uint32_t K = ITEMS * kWave;
-    __shared__ uint32_t s_lut[WAVES][64];
-    __shared__ uint32_t s_hist[WAVES][64];
+    constexpr uint32_t kRing = 2 * kWave;
+    __shared__ uint32_t s_count[WAVES][kWave];
+    __shared__ uint32_t s_ring_e[WAVES][kRing];
+    __shared__ double s_ring_p[WAVES][3][kRing];
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t chunk = wave_chunk<WAVES>();
-    const bool live = chunk < L.chunks;
-    const ReplicaDesc* d = nullptr;
-    if (live) {
-        d = L.desc + replica_of(L, chunk);
-        s_lut[wave][lane] = d->bp.lut[lane];
-    }
-    s_hist[wave][lane] = 0u;
-    __syncthreads();
-
-    uint32_t running = 0;
-    if (live) {
-        const uint32_t owned = d->owned;
-        const uint64_t off = d->offset;
-        const uint32_t first = (chunk - d->first_chunk) * K;
-        double px[ITEMS], py[ITEMS], pz[ITEMS];
+    const uint32_t chunk = wave_chunk<WAVES>(L.xcd_run);
+    if (chunk >= L.chunks) return;
+    const ReplicaDesc* d = desc + replica_of(chunk_replica, chunk);
+    const uint32_t lut = d->bp.lut[lane];
+    s_count[wave][lane] = 0u;
+    const uint32_t owned = d->owned;
+    const uint64_t off = d->offset;
+    const uint32_t first = (chunk - d->first_chunk) * K;
+    const bool full = first + K <= owned;
+    double px[ITEMS], py[ITEMS], pz[ITEMS];
+    load_items<ITEMS>(x + off, first, owned, full, px);
+    load_items<ITEMS>(y + off, first, owned, full, py);
+    load_items<ITEMS>(z + off, first, owned, full, pz);
+    if (WRAP) {
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
             const uint32_t i = first + k * kWave + lane;
             const bool valid = i < owned;
-            px[k] = valid ? x[off + i] : 0.0;
-            py[k] = valid ? y[off + i] : 0.0;
-            pz[k] = valid ? z[off + i] : 0.0;
-        }
-        if (WRAP) {
-#pragma unroll
-            for (int k = 0; k < ITEMS; ++k) {
-                const uint32_t i = first + k * kWave + lane;
-                const bool valid = i < owned;
-                wrap_store(px[k], x + off + i, valid, d->bp.periodic[0], d->bp.lower[0], d->bp.extent[0]);
-                wrap_store(py[k], y + off + i, valid, d->bp.periodic[1], d->bp.lower[1], d->bp.extent[1]);
-                wrap_store(pz[k], z + off + i, valid, d->bp.periodic[2], d->bp.lower[2], d->bp.extent[2]);
-            }
-        }
-        const uint64_t ebase = static_cast<uint64_t>(chunk) * K;
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const uint32_t i = first + k * kWave + lane;
-            const uint32_t code = i < owned ? slab_code(px[k], py[k], pz[k], d->bp) : 0u;
-            const bool boundary = s_lut[wave][code] != 0u;
-            const uint64_t ballot = __ballot(boundary);
-            if (boundary) {
-                B.entries[ebase + running + lanes_below(ballot)] =
-                    ((static_cast<uint32_t>(k) * kWave + lane) << 8) | code;
-                atomicAdd(&s_hist[wave][code], 1u);
-            }
-            running += static_cast<uint32_t>(__popcll(ballot));
+            wrap_store(px[k], x + off + i, valid, d->bp.periodic[0], d->bp.lower[0], d->bp.extent[0]);
+            wrap_store(py[k], y + off + i, valid, d->bp.periodic[1], d->bp.lower[1], d->bp.extent[1]);
+            wrap_store(pz[k], z + off + i, valid, d->bp.periodic[2], d->bp.lower[2], d->bp.extent[2]);
         }
     }
-    __syncthreads();
-    if (live) {
-        // lane s < 26: send count of slot s; lane 26: owners; lane 27: images
-        uint32_t acc = 0;
-        for (int c = 1; c < 64; ++c) {
-            const uint32_t h = s_hist[wave][c];
-            const uint32_t m = s_lut[wave][c];
-            if (lane < kSlots) acc += ((m >> lane) & 1u) ? h : 0u;
-            else if (lane == kColImages) acc += h * static_cast<uint32_t>(__popc(m));
+    // Boundary particles go through a per-wave LDS ring of 2 * 64 slots and
+    // leave it 64 at a time, so the entry and position stores are full-wave
+    // and contiguous.  The ring never holds more than 127 entries: it is
+    // drained whenever 64 are waiting, and one item adds at most 64.
+    const uint64_t ebase = static_cast<uint64_t>(chunk) * K;
+    auto drain = [&](uint32_t from, bool active) {
+        // LDS operations of one wave complete in order: the ring writes of
+        // the other lanes are visible here without a barrier.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (active) {
+            const uint32_t q = (from + lane) & (kRing - 1);
+            const uint64_t e = ebase + from + lane;
+            B.entries[e] = s_ring_e[wave][q];
+            // the scatter reads these densely instead of gathering x, y, z
+            B.bpos[e] = s_ring_p[wave][0][q];
+            B.bpos[B.bpos_stride + e] = s_ring_p[wave][1][q];
+            B.bpos[2 * B.bpos_stride + e] = s_ring_p[wave][2][q];
         }
-        if (lane == kColOwners) acc = running;
-        if (lane < kCols) B.counts[static_cast<uint64_t>(lane) * L.chunks + chunk] = acc;
+        __builtin_amdgcn_wave_barrier();
+    };
+    uint32_t running = 0;
+    uint32_t drained = 0;
+    uint32_t images = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t i = first + k * kWave + lane;
+        const uint32_t code = i < owned ? slab_code(px[k], py[k], pz[k], d->bp) : 0u;
+        const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut), static_cast<int>(code), kWave));
+        const bool boundary = slots != 0u;
+        const uint64_t ballot = __ballot(boundary);
+        if (boundary) {
+            const uint32_t q = (running + lanes_below(ballot)) & (kRing - 1);
+            s_ring_e[wave][q] = ((static_cast<uint32_t>(k) * kWave + lane) << 8) | code;
+            s_ring_p[wave][0][q] = px[k];
+            s_ring_p[wave][1][q] = py[k];
+            s_ring_p[wave][2][q] = pz[k];
+            images += static_cast<uint32_t>(__popc(slots));
+            for (uint32_t m = slots; m; m &= m - 1u) atomicAdd(&s_count[wave][__builtin_ctz(m)], 1u);
+        }
+        running += static_cast<uint32_t>(__popcll(ballot));
+        if (running - drained >= kWave) {
+            drain(drained, true);
+            drained += kWave;
+        }
+    }
+    if (running > drained) drain(drained, lane < running - drained);
+    const uint32_t image_total = __builtin_amdgcn_readlane(wave_inclusive_sum(images), kWave - 1);
+    // LDS operations of one wave complete in order: the atomics above are
+    // visible to the reads below without a barrier.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < kCols) {
+        // lane s < 26: send count of slot s; lane 26: owners; lane 27: images
+        const uint32_t acc = lane < kSlots ? s_count[wave][lane]
+                             : lane == kColOwners ? running : image_total;
+        B.counts[static_cast<uint64_t>(lane) * L.chunks + chunk] = acc;
     }
 }
 
@@ -175,52 +249,127 @@ __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
 // Pass 2: exclusive scan of one count column over one replica's chunks; grid
 // (kCols, replicas).  totals[r][col] receives the column total.
 constexpr int kMaxScanThreads = 1024;
+constexpr int kScanPer = 8;                               // values per thread per tile
+constexpr int kScanTile = kMaxScanThreads * kScanPer;
 
-__global__ __launch_bounds__(kMaxScanThreads) void scan_kernel(Layout L, PlanBuffers B)
+// LDS index of tile element p: one pad word per 32 keeps the thread-
+// contiguous reads (stride kScanPer) off a single bank.
+__device__ __forceinline__ uint32_t scan_pad(uint32_t p) { return p + (p >> 5); }
+
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* s_wave, uint32_t waves,
+                                                    uint32_t* total)
 {
-    __shared__ uint32_t s_wave[kMaxScanThreads / kWave];
-    const uint32_t col = blockIdx.x;
-    const uint32_t r = blockIdx.y;
-    const ReplicaDesc* d = L.desc + r;
-    const uint32_t n = d->chunks;
-    const uint32_t threads = blockDim.x;
-    const uint32_t waves = threads / kWave;
     const uint32_t t = threadIdx.x;
-    const uint32_t per = (n + threads - 1) / threads;
-    const uint32_t begin = min(t * per, n);
-    const uint32_t end = min(begin + per, n);
-    const uint64_t base = static_cast<uint64_t>(col) * L.chunks + d->first_chunk;
-    const uint32_t* src = B.counts + base;
-    uint32_t* dst = B.prefix + base;
-    uint32_t sum = 0;
-    for (uint32_t k = begin; k < end; ++k) sum += src[k];
-    const uint32_t incl = wave_inclusive_sum(sum);
     const uint32_t wave = t / kWave;
+    const uint32_t incl = wave_inclusive_sum(v);
     if (lane_id() == kWave - 1) s_wave[wave] = incl;
     __syncthreads();
     if (wave == 0) {
         const uint32_t w = lane_id() < waves ? s_wave[lane_id()] : 0u;
         const uint32_t wi = wave_inclusive_sum(w);
         if (lane_id() < waves) s_wave[lane_id()] = wi - w;
-        if (lane_id() == waves - 1) B.totals[static_cast<uint64_t>(r) * kTotals + col] = wi;
+        if (lane_id() == waves - 1) s_wave[kMaxScanThreads / kWave] = wi;
     }
     __syncthreads();
-    uint32_t run = s_wave[wave] + incl - sum;
-    for (uint32_t k = begin; k < end; ++k) {
-        const uint32_t v = src[k];
-        dst[k] = run;
-        run += v;
+    const uint32_t out = s_wave[wave] + incl - v;
+    *total = s_wave[kMaxScanThreads / kWave];
+    __syncthreads();
+    return out;
+}
+
+// Tiles of T * kScanPer counts go through LDS: coalesced loads, a scan of
+// kScanPer consecutive values per thread, coalesced stores.
+__global__ __launch_bounds__(kMaxScanThreads) void scan_kernel(const ReplicaDesc* __restrict__ desc,
+                                                               Layout L, PlanBuffers B)
+{
+    __shared__ uint32_t s_tile[kScanTile + kScanTile / 32];
+    __shared__ uint32_t s_wave[kMaxScanThreads / kWave + 1];
+    const uint32_t col = blockIdx.x;
+    const uint32_t r = blockIdx.y;
+    const ReplicaDesc* d = desc + r;
+    const uint32_t n = d->chunks;
+    const uint32_t T = blockDim.x;
+    const uint32_t t = threadIdx.x;
+    const uint32_t waves = T / kWave;
+    const uint64_t base = static_cast<uint64_t>(col) * L.chunks + d->first_chunk;
+    const uint32_t* src = B.counts + base;
+    uint32_t* dst = B.prefix + base;
+    uint32_t carry = 0;
+    for (uint32_t tile = 0; tile < n; tile += T * kScanPer) {
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            const uint32_t idx = k * T + t;
+            s_tile[scan_pad(idx)] = tile + idx < n ? src[tile + idx] : 0u;
+        }
+        __syncthreads();
+        uint32_t v[kScanPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) {
+            v[j] = s_tile[scan_pad(t * kScanPer + j)];
+            sum += v[j];
+        }
+        uint32_t total = 0;
+        uint32_t run = carry + block_exclusive(sum, s_wave, waves, &total);
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) {
+            s_tile[scan_pad(t * kScanPer + j)] = run;
+            run += v[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            const uint32_t idx = k * T + t;
+            if (tile + idx < n) dst[tile + idx] = s_tile[scan_pad(idx)];
+        }
+        carry += total;
+        __syncthreads();
     }
-    if (col == 0 && t == 0) B.totals[static_cast<uint64_t>(r) * kTotals + kStatusWord] = 0u;
+    if (t == 0) {
+        B.totals[static_cast<uint64_t>(r) * kTotals + col] = carry;
+        if (col == 0) B.totals[static_cast<uint64_t>(r) * kTotals + kStatusWord] = 0u;
+        if (B.host_totals) B.host_totals[static_cast<uint64_t>(r) * kTotals + col] = carry;
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Pass 3: one wave per chunk.  Walks the chunk's boundary particles in owned
-// order; for each of its send slots s (ascending) a particle gets the ghost
-// at receive block 25-s, position prefix[s][chunk] + rank in the chunk.
-// Writes the 13 ghost arrays, the registered fields and the owner -> ghost
-// map.  The capacity test is uniform over a replica, so an overflowing
-// replica leaves every one of its ghost slots untouched.
+// Owner-major kernels: one wave per owner tile, one lane per boundary owner
+// (the owner slots of a replica are dense, in owned order).  A replica whose
+// last build overflowed has no owner map and is skipped.
+struct OwnerLane {
+    const ReplicaDesc* d;
+    uint64_t u;      // owner slot
+    bool live;       // this lane holds an owner
+};
+
+template <int WAVES>
+__device__ __forceinline__ OwnerLane owner_of_lane(const ReplicaDesc* __restrict__ desc,
+                                                   const uint32_t* __restrict__ tile_replica,
+                                                   const Layout& L, const PlanBuffers& B)
+{
+    OwnerLane o{nullptr, 0, false};
+    const uint32_t tile = wave_chunk<WAVES>(L.xcd_run);
+    if (tile >= L.tiles) return o;
+    const uint32_t r = tile_replica ? uniform(tile_replica[tile]) : 0u;
+    o.d = desc + r;
+    const uint32_t* totals = B.totals + static_cast<uint64_t>(r) * kTotals;
+    if (uniform(totals[kStatusWord])) return o;
+    const uint32_t owners = uniform(totals[kColOwners]);
+    const uint32_t e = (tile - o.d->first_tile) * kOwnerTile + lane_id();
+    if (e >= owners) return o;
+    o.u = static_cast<uint64_t>(o.d->first_chunk) * L.chunk_size + e;
+    o.live = true;
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// Pass 3 (route): one wave per chunk.  Walks the chunk's boundary particles
+// in owned order; for each of its send slots s (ascending) a particle gets
+// the ghost at receive block 25-s, position prefix[s][chunk] + rank in the
+// chunk.  Writes the owner -> ghost map (owner_index, owner_code,
+// image_start, owner_entry, ghost_list), from which the copy pass fills the
+// ghost records.  The capacity test is uniform over a replica: an
+// overflowing replica gets its status word set and no route.
 __device__ __forceinline__ void copy_field_words(const FieldParams& fp, uint64_t from, uint64_t to)
 {
     for (uint32_t f = 0; f < fp.count; ++f) {
@@ -231,19 +380,18 @@ __device__ __forceinline__ void copy_field_words(const FieldParams& fp, uint64_t
 }
 
 template <int ITEMS, int WAVES>
-__global__ __launch_bounds__(WAVES * kWave) void scatter_kernel(dsdx_particles p, FieldParams fp,
-                                                                Layout L, PlanBuffers B)
+__global__ __launch_bounds__(WAVES * kWave) void route_kernel(
+    const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ chunk_replica, Layout L,
+    PlanBuffers B)
 {
     constexpr uint32_t K = ITEMS * kWave;
-    __shared__ uint32_t s_lut[WAVES][64];
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t chunk = wave_chunk<WAVES>();
+    const uint32_t chunk = wave_chunk<WAVES>(L.xcd_run);
     if (chunk >= L.chunks) return;
-    const uint32_t r = replica_of(L, chunk);
-    const ReplicaDesc* d = L.desc + r;
+    const uint32_t r = replica_of(chunk_replica, chunk);
+    const ReplicaDesc* d = desc + r;
     uint32_t* totals = B.totals + static_cast<uint64_t>(r) * kTotals;
-    s_lut[wave][lane] = d->bp.lut[lane];
+    const uint32_t lut = d->bp.lut[lane];
 
     const uint32_t owned = d->owned;
     const uint32_t ghosts = uniform(totals[kColImages]);
@@ -254,7 +402,6 @@ __global__ __launch_bounds__(WAVES * kWave) void scatter_kernel(dsdx_particles p
         if (chunk == d->first_chunk && lane == 0) totals[kStatusWord] = overflow;
         return;
     }
-    __builtin_amdgcn_wave_barrier();
 
     // Receive block m holds send slot 25-m: recv_base[m] = sum of totals of
     // slots 25..26-m.  Lane s ends with slot_base = recv_base[25-s] + prefix.
@@ -263,14 +410,13 @@ __global__ __launch_bounds__(WAVES * kWave) void scatter_kernel(dsdx_particles p
     const uint32_t mirrored = __shfl(recv_base, lane < kSlots ? kSlots - 1 - lane : 0, kWave);
     const uint32_t slot_base =
         lane < kSlots ? mirrored + B.prefix[static_cast<uint64_t>(lane) * L.chunks + chunk] : 0u;
-    uint32_t slot_run = 0;   // lane s: ghosts of slot s already written from this chunk
+    uint32_t slot_run = 0;   // lane s: ghosts of slot s already routed from this chunk
 
     const uint32_t n_boundary = uniform(B.counts[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
     const uint32_t owner_base = uniform(B.prefix[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
     uint32_t image_run = uniform(B.prefix[static_cast<uint64_t>(kColImages) * L.chunks + chunk]);
-    const uint64_t off = d->offset;
     const uint32_t first = (chunk - d->first_chunk) * K;
-    const uint64_t ebase = static_cast<uint64_t>(chunk) * K;
+    const uint32_t ebase = chunk * K;
     const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * K + owner_base;
     uint32_t* glist = B.ghost_list + d->ghost_list_offset;
 
@@ -279,25 +425,17 @@ __global__ __launch_bounds__(WAVES * kWave) void scatter_kernel(dsdx_particles p
         const bool valid = e < n_boundary;
         const uint32_t entry = valid ? B.entries[ebase + e] : 0u;
         const uint32_t code = entry & 63u;
-        const uint32_t il = first + (entry >> 8);
-        const uint64_t i = off + il;
-        const uint32_t slots = valid ? s_lut[wave][code] : 0u;
+        const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut), static_cast<int>(code), kWave));
         const uint32_t images = static_cast<uint32_t>(__popc(slots));
         const uint32_t incl = wave_inclusive_sum(images);
         const uint32_t first_image = image_run + incl - images;
         image_run += uniform(__shfl(incl, kWave - 1, kWave));
-
-        double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0, m = 0, q = 0;
-        int32_t ty = 0, id = 0;
         if (valid) {
             const uint64_t u = slot0 + e;
-            B.owner_index[u] = il;
+            B.owner_index[u] = first + (entry >> 8);
             B.owner_code[u] = static_cast<uint8_t>(code);
             B.image_start[u] = first_image;
-            px = p.x[i]; py = p.y[i]; pz = p.z[i];
-            vx = p.vx[i]; vy = p.vy[i]; vz = p.vz[i];
-            m = p.mass[i]; q = p.charge[i];
-            ty = p.type[i]; id = p.global_id[i];
+            B.owner_entry[u] = ebase + e;
         }
         uint32_t present = uniform(wave_or(slots));
         uint32_t written = 0;
@@ -310,17 +448,6 @@ __global__ __launch_bounds__(WAVES * kWave) void scatter_kernel(dsdx_particles p
                                   __builtin_amdgcn_readlane(slot_run, s);
             if (mine) {
                 const uint32_t rel = base + lanes_below(ballot);
-                const uint64_t j = off + owned + rel;
-                p.x[j] = px + d->bp.shift[s][0];
-                p.y[j] = py + d->bp.shift[s][1];
-                p.z[j] = pz + d->bp.shift[s][2];
-                p.vx[j] = vx; p.vy[j] = vy; p.vz[j] = vz;
-                if (p.fx) p.fx[j] = 0.0;
-                if (p.fy) p.fy[j] = 0.0;
-                if (p.fz) p.fz[j] = 0.0;
-                p.mass[j] = m; p.charge[j] = q;
-                p.type[j] = ty; p.global_id[j] = id;
-                if (fp.count) copy_field_words(fp, i, j);
                 glist[first_image + written] = rel;
                 ++written;
             }
@@ -329,61 +456,74 @@ __global__ __launch_bounds__(WAVES * kWave) void scatter_kernel(dsdx_particles p
     }
 }
 
-// ---------------------------------------------------------------------------
-// Owner-major kernels: one wave per chunk, one lane per boundary owner found
-// in that chunk.  A replica whose last build overflowed has no owner map and
-// is skipped.
-struct OwnerCursor {
-    const ReplicaDesc* d;
-    uint64_t slot0;      // owner slot of the chunk's first owner
-    uint32_t count;      // owners found in the chunk
-    bool live;
-};
-
-template <int ITEMS, int WAVES>
-__device__ __forceinline__ OwnerCursor owners_of_wave(const Layout& L, const PlanBuffers& B)
+// Pass 4 (copy): one wave per owner tile, one lane per boundary owner, in
+// owned order (one ascending sweep over the owner arrays).  Each owner's
+// ghosts get position + shift[slot] (the owner's position comes from the
+// classify pass, densely), velocity, mass, charge, type, id and the
+// registered fields of the owner, forces zero (src/halo.cpp:1232-1262).
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
+    dsdx_particles p, FieldParams fp, const ReplicaDesc* __restrict__ desc,
+    const uint32_t* __restrict__ tile_replica, Layout L, PlanBuffers B)
 {
-    constexpr uint32_t K = ITEMS * kWave;
-    OwnerCursor c{nullptr, 0, 0, false};
-    const uint32_t chunk = wave_chunk<WAVES>();
-    if (chunk >= L.chunks) return c;
-    const uint32_t r = replica_of(L, chunk);
-    c.d = L.desc + r;
-    if (uniform(B.totals[static_cast<uint64_t>(r) * kTotals + kStatusWord])) return c;
-    c.count = uniform(B.counts[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
-    c.slot0 = static_cast<uint64_t>(c.d->first_chunk) * K +
-              uniform(B.prefix[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
-    c.live = true;
-    return c;
+    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
+    if (!o.live) return;
+    const ReplicaDesc* d = o.d;
+    const uint64_t u = o.u;
+    const uint64_t off = d->offset;
+    const uint64_t i = off + B.owner_index[u];
+    const uint32_t E = B.owner_entry[u];
+    const uint32_t first = B.image_start[u];
+    const uint32_t slots = d->bp.lut[B.owner_code[u]];
+    const double px = B.bpos[E];
+    const double py = B.bpos[B.bpos_stride + E];
+    const double pz = B.bpos[2 * B.bpos_stride + E];
+    const double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
+    const double m = p.mass[i], q = p.charge[i];
+    const int32_t ty = p.type[i], id = p.global_id[i];
+    const uint32_t* glist = B.ghost_list + d->ghost_list_offset;
+    const uint64_t goff = off + d->owned;
+    uint32_t k = 0;
+    for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k) {
+        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
+        const uint64_t j = goff + glist[first + k];
+        p.x[j] = px + d->bp.shift[s][0];
+        p.y[j] = py + d->bp.shift[s][1];
+        p.z[j] = pz + d->bp.shift[s][2];
+        p.vx[j] = vx; p.vy[j] = vy; p.vz[j] = vz;
+        if (p.fx) p.fx[j] = 0.0;
+        if (p.fy) p.fy[j] = 0.0;
+        if (p.fz) p.fz[j] = 0.0;
+        p.mass[j] = m; p.charge[j] = q;
+        p.type[j] = ty; p.global_id[j] = id;
+        if (fp.count) copy_field_words(fp, i, j);
+    }
 }
 
 // reverse_halo: f[i] = ((f[i] + g_s1) + g_s2) + ... over the owner's send
 // slots s1 < s2 < ..., the order of the Host reduction (src/halo.cpp:1357-1370
 // for n ascending).  No atomics: one lane sums one owner.
-template <int ITEMS, int WAVES>
+template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void reverse_forces_kernel(
-    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz, Layout L,
+    double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
+    const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ tile_replica, Layout L,
     PlanBuffers B)
 {
-    const OwnerCursor c = owners_of_wave<ITEMS, WAVES>(L, B);
-    if (!c.live) return;
-    const uint64_t off = c.d->offset;
-    const uint64_t goff = off + c.d->owned;
-    const uint32_t* glist = B.ghost_list + c.d->ghost_list_offset;
-    for (uint32_t e = lane_id(); e < c.count; e += kWave) {
-        const uint64_t u = c.slot0 + e;
-        const uint64_t i = off + B.owner_index[u];
-        const uint32_t first = B.image_start[u];
-        const uint32_t n_img = static_cast<uint32_t>(__popc(c.d->bp.lut[B.owner_code[u]]));
-        double ax = fx[i], ay = fy[i], az = fz[i];
-        for (uint32_t k = 0; k < n_img; ++k) {
-            const uint64_t j = goff + glist[first + k];
-            ax = ax + fx[j];
-            ay = ay + fy[j];
-            az = az + fz[j];
-        }
-        fx[i] = ax; fy[i] = ay; fz[i] = az;
+    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
+    if (!o.live) return;
+    const uint64_t off = o.d->offset;
+    const uint64_t goff = off + o.d->owned;
+    const uint32_t* glist = B.ghost_list + o.d->ghost_list_offset + B.image_start[o.u];
+    const uint64_t i = off + B.owner_index[o.u];
+    const uint32_t n_img = static_cast<uint32_t>(__popc(o.d->bp.lut[B.owner_code[o.u]]));
+    double ax = fx[i], ay = fy[i], az = fz[i];
+    for (uint32_t k = 0; k < n_img; ++k) {
+        const uint64_t j = goff + glist[k];
+        ax = ax + fx[j];
+        ay = ay + fy[j];
+        az = az + fz[j];
     }
+    fx[i] = ax; fy[i] = ay; fz[i] = az;
 }
 
 template <class T>
@@ -410,132 +550,120 @@ __device__ void reduce_field(T* data, uint32_t c, int32_t op, uint64_t i, uint64
 
 // Integer sums wrap as unsigned (two's complement), like the Host's
 // destination += value on int32/int64.
-template <int ITEMS, int WAVES>
-__global__ __launch_bounds__(WAVES * kWave) void reverse_fields_kernel(TypedFields tf, Layout L,
-                                                                       PlanBuffers B)
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void reverse_fields_kernel(
+    TypedFields tf, const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ tile_replica,
+    Layout L, PlanBuffers B)
 {
-    const OwnerCursor c = owners_of_wave<ITEMS, WAVES>(L, B);
-    if (!c.live) return;
-    const uint64_t off = c.d->offset;
-    const uint64_t goff = off + c.d->owned;
-    const uint32_t* glist = B.ghost_list + c.d->ghost_list_offset;
-    for (uint32_t e = lane_id(); e < c.count; e += kWave) {
-        const uint64_t u = c.slot0 + e;
-        const uint64_t i = off + B.owner_index[u];
-        const uint32_t* ghosts = glist + B.image_start[u];
-        const uint32_t n_img = static_cast<uint32_t>(__popc(c.d->bp.lut[B.owner_code[u]]));
-        for (uint32_t f = 0; f < tf.count; ++f) {
-            const uint32_t cc = tf.components[f];
-            const int32_t op = tf.op[f];
-            switch (tf.type[f]) {
-            case DSDX_INT32:
-                if (op == DSDX_SUM)
-                    reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
-                else
-                    reduce_field<int32_t>(static_cast<int32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
-                break;
-            case DSDX_INT64:
-                if (op == DSDX_SUM)
-                    reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
-                else
-                    reduce_field<int64_t>(static_cast<int64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
-                break;
-            case DSDX_UINT32:
+    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
+    if (!o.live) return;
+    const uint64_t goff = o.d->offset + o.d->owned;
+    const uint64_t u = o.u;
+    const uint64_t i = o.d->offset + B.owner_index[u];
+    const uint32_t* ghosts = B.ghost_list + o.d->ghost_list_offset + B.image_start[u];
+    const uint32_t n_img = static_cast<uint32_t>(__popc(o.d->bp.lut[B.owner_code[u]]));
+    for (uint32_t f = 0; f < tf.count; ++f) {
+        const uint32_t cc = tf.components[f];
+        const int32_t op = tf.op[f];
+        switch (tf.type[f]) {
+        case DSDX_INT32:
+            if (op == DSDX_SUM)
                 reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
-                break;
-            case DSDX_UINT64:
+            else
+                reduce_field<int32_t>(static_cast<int32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+            break;
+        case DSDX_INT64:
+            if (op == DSDX_SUM)
                 reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
-                break;
-            case DSDX_FLOAT32:
-                reduce_field<float>(static_cast<float*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
-                break;
-            default:
-                reduce_field<double>(static_cast<double*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
-                break;
-            }
+            else
+                reduce_field<int64_t>(static_cast<int64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+            break;
+        case DSDX_UINT32:
+            reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+            break;
+        case DSDX_UINT64:
+            reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+            break;
+        case DSDX_FLOAT32:
+            reduce_field<float>(static_cast<float*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+            break;
+        default:
+            reduce_field<double>(static_cast<double*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+            break;
         }
     }
 }
 
 // forward_exchange: owner values copied into each of its ghost slots.
-template <int ITEMS, int WAVES>
-__global__ __launch_bounds__(WAVES * kWave) void forward_fields_kernel(FieldParams fp, Layout L,
-                                                                       PlanBuffers B)
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void forward_fields_kernel(
+    FieldParams fp, const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ tile_replica,
+    Layout L, PlanBuffers B)
 {
-    const OwnerCursor c = owners_of_wave<ITEMS, WAVES>(L, B);
-    if (!c.live) return;
-    const uint64_t off = c.d->offset;
-    const uint64_t goff = off + c.d->owned;
-    const uint32_t* glist = B.ghost_list + c.d->ghost_list_offset;
-    for (uint32_t e = lane_id(); e < c.count; e += kWave) {
-        const uint64_t u = c.slot0 + e;
-        const uint64_t i = off + B.owner_index[u];
-        const uint32_t* ghosts = glist + B.image_start[u];
-        const uint32_t n_img = static_cast<uint32_t>(__popc(c.d->bp.lut[B.owner_code[u]]));
-        for (uint32_t f = 0; f < fp.count; ++f) {
-            const uint32_t w = fp.words[f];
-            uint32_t* dd = static_cast<uint32_t*>(fp.data[f]);
-            for (uint32_t k = 0; k < w; ++k) {
-                const uint32_t value = dd[i * w + k];
-                for (uint32_t g = 0; g < n_img; ++g) dd[(goff + ghosts[g]) * w + k] = value;
-            }
+    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
+    if (!o.live) return;
+    const uint64_t goff = o.d->offset + o.d->owned;
+    const uint64_t u = o.u;
+    const uint64_t i = o.d->offset + B.owner_index[u];
+    const uint32_t* ghosts = B.ghost_list + o.d->ghost_list_offset + B.image_start[u];
+    const uint32_t n_img = static_cast<uint32_t>(__popc(o.d->bp.lut[B.owner_code[u]]));
+    for (uint32_t f = 0; f < fp.count; ++f) {
+        const uint32_t w = fp.words[f];
+        uint32_t* dd = static_cast<uint32_t*>(fp.data[f]);
+        for (uint32_t k = 0; k < w; ++k) {
+            const uint32_t value = dd[i * w + k];
+            for (uint32_t g = 0; g < n_img; ++g) dd[(goff + ghosts[g]) * w + k] = value;
         }
     }
 }
 
 // Reused-plan refresh: each ghost re-reads its owner; the slot of the k-th
 // ghost of an owner is the k-th set bit of its slot set.
-template <int ITEMS, int WAVES>
-__global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(dsdx_particles p, FieldParams fp,
-                                                                int32_t positions_only, Layout L,
-                                                                PlanBuffers B)
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(
+    dsdx_particles p, FieldParams fp, int32_t positions_only, const ReplicaDesc* __restrict__ desc,
+    const uint32_t* __restrict__ tile_replica, Layout L, PlanBuffers B)
 {
-    const OwnerCursor c = owners_of_wave<ITEMS, WAVES>(L, B);
-    if (!c.live) return;
-    const uint64_t off = c.d->offset;
-    const uint64_t goff = off + c.d->owned;
-    const uint32_t* glist = B.ghost_list + c.d->ghost_list_offset;
-    for (uint32_t e = lane_id(); e < c.count; e += kWave) {
-        const uint64_t u = c.slot0 + e;
-        const uint64_t i = off + B.owner_index[u];
-        const uint32_t* ghosts = glist + B.image_start[u];
-        uint32_t slots = c.d->bp.lut[B.owner_code[u]];
-        const double px = p.x[i], py = p.y[i], pz = p.z[i];
-        for (uint32_t k = 0; slots; ++k) {
-            const uint32_t s = static_cast<uint32_t>(__builtin_ctz(slots));
-            slots &= slots - 1u;
-            const uint64_t j = goff + ghosts[k];
-            p.x[j] = px + c.d->bp.shift[s][0];
-            p.y[j] = py + c.d->bp.shift[s][1];
-            p.z[j] = pz + c.d->bp.shift[s][2];
-            if (p.fx) p.fx[j] = 0.0;
-            if (p.fy) p.fy[j] = 0.0;
-            if (p.fz) p.fz[j] = 0.0;
-            if (!positions_only) {
-                p.vx[j] = p.vx[i]; p.vy[j] = p.vy[i]; p.vz[j] = p.vz[i];
-                p.mass[j] = p.mass[i]; p.charge[j] = p.charge[i];
-                p.type[j] = p.type[i]; p.global_id[j] = p.global_id[i];
-                if (fp.count) copy_field_words(fp, i, j);
-            }
+    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
+    if (!o.live) return;
+    const ReplicaDesc* d = o.d;
+    const uint64_t goff = d->offset + d->owned;
+    const uint64_t u = o.u;
+    const uint64_t i = d->offset + B.owner_index[u];
+    const uint32_t* ghosts = B.ghost_list + d->ghost_list_offset + B.image_start[u];
+    uint32_t slots = d->bp.lut[B.owner_code[u]];
+    const double px = p.x[i], py = p.y[i], pz = p.z[i];
+    for (uint32_t k = 0; slots; ++k) {
+        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(slots));
+        slots &= slots - 1u;
+        const uint64_t j = goff + ghosts[k];
+        p.x[j] = px + d->bp.shift[s][0];
+        p.y[j] = py + d->bp.shift[s][1];
+        p.z[j] = pz + d->bp.shift[s][2];
+        if (p.fx) p.fx[j] = 0.0;
+        if (p.fy) p.fy[j] = 0.0;
+        if (p.fz) p.fz[j] = 0.0;
+        if (!positions_only) {
+            p.vx[j] = p.vx[i]; p.vy[j] = p.vy[i]; p.vz[j] = p.vz[i];
+            p.mass[j] = p.mass[i]; p.charge[j] = p.charge[i];
+            p.type[j] = p.type[i]; p.global_id[j] = p.global_id[i];
+            if (fp.count) copy_field_words(fp, i, j);
         }
     }
 }
 
 // Owner of every ghost: out[offset_r + rel] = owned index (replica-local).
-template <int ITEMS, int WAVES>
-__global__ __launch_bounds__(WAVES * kWave) void ghost_owners_kernel(int32_t* __restrict__ out,
-                                                                     Layout L, PlanBuffers B)
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void ghost_owners_kernel(
+    int32_t* __restrict__ out, const ReplicaDesc* __restrict__ desc,
+    const uint32_t* __restrict__ tile_replica, Layout L, PlanBuffers B)
 {
-    const OwnerCursor c = owners_of_wave<ITEMS, WAVES>(L, B);
-    if (!c.live) return;
-    const uint32_t* glist = B.ghost_list + c.d->ghost_list_offset;
-    for (uint32_t e = lane_id(); e < c.count; e += kWave) {
-        const uint64_t u = c.slot0 + e;
-        const uint32_t i = B.owner_index[u];
-        const uint32_t* ghosts = glist + B.image_start[u];
-        const uint32_t n_img = static_cast<uint32_t>(__popc(c.d->bp.lut[B.owner_code[u]]));
-        for (uint32_t k = 0; k < n_img; ++k) out[c.d->offset + ghosts[k]] = static_cast<int32_t>(i);
-    }
+    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
+    if (!o.live) return;
+    const uint32_t i = B.owner_index[o.u];
+    const uint32_t* ghosts = B.ghost_list + o.d->ghost_list_offset + B.image_start[o.u];
+    const uint32_t n_img = static_cast<uint32_t>(__popc(o.d->bp.lut[B.owner_code[o.u]]));
+    for (uint32_t k = 0; k < n_img; ++k) out[o.d->offset + ghosts[k]] = static_cast<int32_t>(i);
 }
 
 // ---------------------------------------------------------------------------
@@ -544,33 +672,32 @@ __global__ __launch_bounds__(WAVES * kWave) void ghost_owners_kernel(int32_t* __
 
 template <int ITEMS, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void migrate_wrap_kernel(
-    double* __restrict__ x, double* __restrict__ y, double* __restrict__ z, Layout L)
+    double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+    const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ chunk_replica, Layout L)
 {
     constexpr uint32_t K = ITEMS * kWave;
-    const uint32_t chunk = wave_chunk<WAVES>();
+    const uint32_t chunk = wave_chunk<WAVES>(L.xcd_run);
     if (chunk >= L.chunks) return;
-    const ReplicaDesc* d = L.desc + replica_of(L, chunk);
+    const ReplicaDesc* d = desc + replica_of(chunk_replica, chunk);
     const uint32_t owned = d->owned;
     const uint64_t off = d->offset;
     const uint32_t first = (chunk - d->first_chunk) * K;
-    double* axis[3] = {x, y, z};
+    const bool full = first + K <= owned;
+    double* axis[3] = {x + off, y + off, z + off};
+    double val[3][ITEMS];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (d->bp.periodic[a]) load_items<ITEMS>(axis[a], first, owned, full, val[a]);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (!d->bp.periodic[a]) continue;
-        double* v = axis[a] + off;
         const double lower = d->bp.lower[a];
         const double extent = d->bp.extent[a];
-        double val[ITEMS];
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
             const uint32_t i = first + k * kWave + lane_id();
-            val[k] = i < owned ? v[i] : 0.0;
-        }
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const uint32_t i = first + k * kWave + lane_id();
-            const double w = wrap(val[k], lower, extent);
-            if (i < owned && __double_as_longlong(w) != __double_as_longlong(val[k])) v[i] = w;
+            const double w = wrap(val[a][k], lower, extent);
+            if (i < owned && __double_as_longlong(w) != __double_as_longlong(val[a][k])) axis[a][i] = w;
         }
     }
 }
@@ -611,7 +738,36 @@ bool shape_supported(const Shape& s)
 }
 
 // ---------------------------------------------------------------------------
-// Launchers (called by the C ABI in abi.cpp).
+// Launchers (called by the C ABI in abi.cpp).  Every kernel goes through
+// launch(): when a timing region is open on this thread the dispatch itself
+// carries the region's start / stop events.
+namespace {
+
+template <typename... Params, typename... Args>
+hipError_t launch(void (*kernel)(Params...), dim3 grid, dim3 block, hipStream_t stream, Args... args)
+{
+    hipEvent_t start = nullptr, stop = nullptr;
+    timing::claim(&start, &stop);
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, start, stop, 0, args...);
+    return hipGetLastError();
+}
+
+dim3 chunk_grid(const Layout& L, int waves) { return dim3(wave_blocks(L.chunks, waves)); }
+dim3 tile_grid(const Layout& L, int waves) { return dim3(wave_blocks(L.tiles, waves)); }
+
+template <class F>
+hipError_t dispatch_waves(const Shape& s, F&& f)
+{
+    switch (s.waves) {
+    case 2: return f(std::integral_constant<int, 2>{});
+    case 4: return f(std::integral_constant<int, 4>{});
+    case 8: return f(std::integral_constant<int, 8>{});
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
 hipError_t launch_classify(double* x, double* y, double* z, bool wrap_first, const Layout& L,
                            const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
@@ -619,91 +775,92 @@ hipError_t launch_classify(double* x, double* y, double* z, bool wrap_first, con
     return dispatch(s, [&](auto items, auto waves) {
         constexpr int I = decltype(items)::value, W = decltype(waves)::value;
         if (wrap_first)
-            hipLaunchKernelGGL((classify_kernel<I, W, true>), dim3(wave_blocks(L.chunks, W)),
-                               dim3(W * kWave), 0, stream, x, y, z, L, B);
-        else
-            hipLaunchKernelGGL((classify_kernel<I, W, false>), dim3(wave_blocks(L.chunks, W)),
-                               dim3(W * kWave), 0, stream, x, y, z, L, B);
-        return hipGetLastError();
+            return launch(classify_kernel<I, W, true>, chunk_grid(L, W), dim3(W * kWave), stream,
+                          x, y, z, L.desc, L.chunk_replica, L, B);
+        return launch(classify_kernel<I, W, false>, chunk_grid(L, W), dim3(W * kWave), stream,
+                      x, y, z, L.desc, L.chunk_replica, L, B);
     });
 }
 
 hipError_t launch_scan(const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
     if (L.replicas == 0) return hipSuccess;
-    hipLaunchKernelGGL(scan_kernel, dim3(kCols, L.replicas), dim3(s.scan_threads), 0, stream, L, B);
-    return hipGetLastError();
+    return launch(scan_kernel, dim3(kCols, L.replicas), dim3(s.scan_threads), stream, L.desc, L, B);
 }
 
-hipError_t launch_scatter(const dsdx_particles& p, const FieldParams& fp, const Layout& L,
-                          const PlanBuffers& B, const Shape& s, hipStream_t stream)
+hipError_t launch_route(const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
     if (L.chunks == 0) return hipSuccess;
     return dispatch(s, [&](auto items, auto waves) {
         constexpr int I = decltype(items)::value, W = decltype(waves)::value;
-        hipLaunchKernelGGL((scatter_kernel<I, W>), dim3(wave_blocks(L.chunks, W)), dim3(W * kWave), 0,
-                           stream, p, fp, L, B);
-        return hipGetLastError();
+        return launch(route_kernel<I, W>, chunk_grid(L, W), dim3(W * kWave), stream, L.desc,
+                      L.chunk_replica, L, B);
+    });
+}
+
+hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Layout& L,
+                       const PlanBuffers& B, const Shape& s, hipStream_t stream)
+{
+    if (L.tiles == 0) return hipSuccess;
+    return dispatch_waves(s, [&](auto waves) {
+        constexpr int W = decltype(waves)::value;
+        return launch(copy_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, p, fp, L.desc,
+                      L.tile_replica, L, B);
     });
 }
 
 hipError_t launch_reverse_forces(double* fx, double* fy, double* fz, const Layout& L,
                                  const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
-    if (L.chunks == 0) return hipSuccess;
-    return dispatch(s, [&](auto items, auto waves) {
-        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
-        hipLaunchKernelGGL((reverse_forces_kernel<I, W>), dim3(wave_blocks(L.chunks, W)),
-                           dim3(W * kWave), 0, stream, fx, fy, fz, L, B);
-        return hipGetLastError();
+    if (L.tiles == 0) return hipSuccess;
+    return dispatch_waves(s, [&](auto waves) {
+        constexpr int W = decltype(waves)::value;
+        return launch(reverse_forces_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, fx, fy, fz,
+                      L.desc, L.tile_replica, L, B);
     });
 }
 
 hipError_t launch_reverse_fields(const TypedFields& tf, const Layout& L, const PlanBuffers& B,
                                  const Shape& s, hipStream_t stream)
 {
-    if (L.chunks == 0) return hipSuccess;
-    return dispatch(s, [&](auto items, auto waves) {
-        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
-        hipLaunchKernelGGL((reverse_fields_kernel<I, W>), dim3(wave_blocks(L.chunks, W)),
-                           dim3(W * kWave), 0, stream, tf, L, B);
-        return hipGetLastError();
+    if (L.tiles == 0) return hipSuccess;
+    return dispatch_waves(s, [&](auto waves) {
+        constexpr int W = decltype(waves)::value;
+        return launch(reverse_fields_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, tf,
+                      L.desc, L.tile_replica, L, B);
     });
 }
 
 hipError_t launch_forward_fields(const FieldParams& fp, const Layout& L, const PlanBuffers& B,
                                  const Shape& s, hipStream_t stream)
 {
-    if (L.chunks == 0) return hipSuccess;
-    return dispatch(s, [&](auto items, auto waves) {
-        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
-        hipLaunchKernelGGL((forward_fields_kernel<I, W>), dim3(wave_blocks(L.chunks, W)),
-                           dim3(W * kWave), 0, stream, fp, L, B);
-        return hipGetLastError();
+    if (L.tiles == 0) return hipSuccess;
+    return dispatch_waves(s, [&](auto waves) {
+        constexpr int W = decltype(waves)::value;
+        return launch(forward_fields_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, fp,
+                      L.desc, L.tile_replica, L, B);
     });
 }
 
 hipError_t launch_refresh(const dsdx_particles& p, const FieldParams& fp, int32_t positions_only,
                           const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
-    if (L.chunks == 0) return hipSuccess;
-    return dispatch(s, [&](auto items, auto waves) {
-        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
-        hipLaunchKernelGGL((refresh_kernel<I, W>), dim3(wave_blocks(L.chunks, W)), dim3(W * kWave), 0,
-                           stream, p, fp, positions_only, L, B);
-        return hipGetLastError();
+    if (L.tiles == 0) return hipSuccess;
+    return dispatch_waves(s, [&](auto waves) {
+        constexpr int W = decltype(waves)::value;
+        return launch(refresh_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, p, fp,
+                      positions_only, L.desc, L.tile_replica, L, B);
     });
 }
 
 hipError_t launch_ghost_owners(int32_t* out, const Layout& L, const PlanBuffers& B, const Shape& s,
                                hipStream_t stream)
 {
-    if (L.chunks == 0) return hipSuccess;
-    return dispatch(s, [&](auto items, auto waves) {
-        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
-        hipLaunchKernelGGL((ghost_owners_kernel<I, W>), dim3(wave_blocks(L.chunks, W)),
-                           dim3(W * kWave), 0, stream, out, L, B);
-        return hipGetLastError();
+    if (L.tiles == 0) return hipSuccess;
+    return dispatch_waves(s, [&](auto waves) {
+        constexpr int W = decltype(waves)::value;
+        return launch(ghost_owners_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, out, L.desc,
+                      L.tile_replica, L, B);
     });
 }
 
@@ -713,9 +870,8 @@ hipError_t launch_migrate_wrap(double* x, double* y, double* z, const Layout& L,
     if (L.chunks == 0) return hipSuccess;
     return dispatch(s, [&](auto items, auto waves) {
         constexpr int I = decltype(items)::value, W = decltype(waves)::value;
-        hipLaunchKernelGGL((migrate_wrap_kernel<I, W>), dim3(wave_blocks(L.chunks, W)),
-                           dim3(W * kWave), 0, stream, x, y, z, L);
-        return hipGetLastError();
+        return launch(migrate_wrap_kernel<I, W>, chunk_grid(L, W), dim3(W * kWave), stream,
+                      x, y, z, L.desc, L.chunk_replica, L);
     });
 }
 

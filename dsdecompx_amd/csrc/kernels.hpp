@@ -13,8 +13,9 @@ bool shape_supported(const Shape& s);
 hipError_t launch_classify(double* x, double* y, double* z, bool wrap_first, const Layout& L,
                            const PlanBuffers& B, const Shape& s, hipStream_t stream);
 hipError_t launch_scan(const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream);
-hipError_t launch_scatter(const dsdx_particles& p, const FieldParams& fp, const Layout& L,
-                          const PlanBuffers& B, const Shape& s, hipStream_t stream);
+hipError_t launch_route(const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream);
+hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Layout& L,
+                       const PlanBuffers& B, const Shape& s, hipStream_t stream);
 hipError_t launch_reverse_forces(double* fx, double* fy, double* fz, const Layout& L,
                                  const PlanBuffers& B, const Shape& s, hipStream_t stream);
 hipError_t launch_reverse_fields(const TypedFields& tf, const Layout& L, const PlanBuffers& B,

@@ -14,7 +14,7 @@ namespace {
 const char* const kNames[kKernelCount] = {"migrate_wrap", "halo_classify", "halo_scan",
                                           "halo_scatter", "reverse_halo", "reverse_exchange",
                                           "forward_exchange", "halo_refresh", "ghost_owners",
-                                          "migrate_classify"};
+                                          "migrate_classify", "halo_copy"};
 
 struct Pending {
     Kernel kernel;
@@ -67,28 +67,58 @@ int resolve()
 
 bool enabled() { return g_enabled.load(std::memory_order_relaxed); }
 
-Scope::Scope(Kernel kernel, hipStream_t stream) : kernel_(kernel), stream_(stream)
+namespace {
+thread_local Scope* t_open = nullptr;
+
+void give_back(hipEvent_t e)
+{
+    if (!e) return;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    g_pool.push_back(e);
+}
+}  // namespace
+
+Scope::Scope(Kernel kernel, hipStream_t stream) : kernel_(kernel)
 {
     if (!enabled()) return;
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &capture) != hipSuccess || capture != hipStreamCaptureStatusNone)
+        return;
     start_ = take_event();
-    if (start_ && hipEventRecord(start_, stream_) != hipSuccess) {
-        std::lock_guard<std::mutex> lock(g_mutex);
-        g_pool.push_back(start_);
-        start_ = nullptr;
+    stop_ = take_event();
+    if (!start_ || !stop_) {
+        give_back(start_);
+        give_back(stop_);
+        start_ = stop_ = nullptr;
+        return;
     }
+    outer_ = t_open;
+    t_open = this;
 }
 
 Scope::~Scope()
 {
     if (!start_) return;
-    hipEvent_t stop = take_event();
-    std::lock_guard<std::mutex> lock(g_mutex);
-    if (stop && hipEventRecord(stop, stream_) == hipSuccess) {
-        g_pending.push_back({kernel_, start_, stop});
+    t_open = outer_;
+    if (launched_) {
+        std::lock_guard<std::mutex> lock(g_mutex);
+        g_pending.push_back({kernel_, start_, stop_});
     } else {
-        g_pool.push_back(start_);
-        if (stop) g_pool.push_back(stop);
+        give_back(start_);
+        give_back(stop_);
     }
+}
+
+void claim(hipEvent_t* start, hipEvent_t* stop)
+{
+    Scope* s = t_open;
+    if (!s || s->launched_) {
+        *start = *stop = nullptr;
+        return;
+    }
+    s->launched_ = true;
+    *start = s->start_;
+    *stop = s->stop_;
 }
 
 }  // namespace dsdx::timing

@@ -1,7 +1,10 @@
-// timing.hpp -- optional per-kernel device timing: a pair of HIP events
-// recorded on the launching stream around every kernel, resolved on demand
-// (the device-side counterpart of the reference's host-clock
-// HaloExchangeTimings, include/decomposition.hpp:30-38).
+// timing.hpp -- optional per-kernel device timing: the start and stop
+// timestamps of each kernel's own dispatch (hipExtLaunchKernel events),
+// resolved on demand (the device-side counterpart of the reference's
+// host-clock HaloExchangeTimings, include/decomposition.hpp:30-38).  A timed
+// dispatch starts several microseconds after the previous kernel ends, so a
+// throughput measurement runs with timing off and times kernels in a pass of
+// its own.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,13 +22,16 @@ enum Kernel {
     kRefresh,
     kGhostOwners,
     kMigrateClassify,
+    kCopy,
     kKernelCount
 };
 
 bool enabled();
 
-// Records a start event on construction and a stop event on destruction when
-// timing is enabled; does nothing otherwise.
+// Opens a timed region on this thread when timing is enabled and the stream
+// is not being captured: the first kernel launched inside it (through
+// claim()) carries the region's event pair.  A region that launches nothing
+// records nothing.
 class Scope {
 public:
     Scope(Kernel kernel, hipStream_t stream);
@@ -34,9 +40,16 @@ public:
     Scope& operator=(const Scope&) = delete;
 
 private:
+    friend void claim(hipEvent_t* start, hipEvent_t* stop);
     Kernel kernel_;
-    hipStream_t stream_;
     hipEvent_t start_ = nullptr;
+    hipEvent_t stop_ = nullptr;
+    bool launched_ = false;
+    Scope* outer_ = nullptr;
 };
+
+// The event pair of the innermost open Scope on this thread, once; null
+// events otherwise (the launch is then untimed).
+void claim(hipEvent_t* start, hipEvent_t* stop);
 
 }  // namespace dsdx::timing

@@ -268,14 +268,16 @@ int dsdx_batch_counts(const dsdx_batch* batch, uint32_t* ghosts,
  * kernels that must know a replica's ghost count without a host wait. */
 const uint32_t* dsdx_batch_device_totals(const dsdx_batch* batch);
 
-/* Per-kernel device timing, process-wide: when enabled, every kernel launch
- * is bracketed by two HIP events on its stream.  dsdx_timing_read waits for
+/* Per-kernel device timing, process-wide: when enabled, every kernel
+ * dispatch carries its own start / stop HIP events (a timed dispatch starts a
+ * few microseconds later than an untimed one).  dsdx_timing_read waits for
  * the recorded events and returns the accumulated milliseconds and launch
  * count of one kernel: "migrate_wrap", "halo_classify", "halo_scan",
- * "halo_scatter", "reverse_halo", "reverse_exchange", "forward_exchange",
- * "halo_refresh", "ghost_owners" or "migrate_classify" (the fused pass).  The GPU-event counterpart of the
- * reference's host-clock timings (include/decomposition.hpp:30-38,
- * src/halo.cpp:316-323). */
+ * "halo_scatter" (the route pass: owner map and ghost routes), "halo_copy"
+ * (the ghost records), "reverse_halo", "reverse_exchange",
+ * "forward_exchange", "halo_refresh", "ghost_owners" or "migrate_classify"
+ * (the fused pass).  The GPU-event counterpart of the reference's host-clock
+ * timings (include/decomposition.hpp:30-38, src/halo.cpp:316-323). */
 int dsdx_timing_enable(int32_t enable);
 int dsdx_timing_reset(void);
 int dsdx_timing_read(const char* kernel, double* total_ms, uint64_t* launches);
