@@ -192,27 +192,39 @@ def tune_shape(args, n, L, w, periodic):
     return report["best"], report
 
 
-def measure_cpp(args, n, L, w, sigma, periodic):
-    """The hot path through the drop-in C++ API (dsdecompx::Decomp), driven
-    by the compiled driver dsdecompx_amd/bin/comm_bench."""
+def run_comm_bench(args, n, L, w, sigma, periodic, env, fused=False):
     import subprocess
     exe = os.path.join(ROOT, "dsdecompx_amd", "bin", "comm_bench")
     if not os.path.exists(exe):
         raise SystemExit(f"{exe} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    cmd = [exe, str(n), repr(L), repr(w), repr(sigma), "1" if periodic[2] else "0",
+           str(args.steps), str(args.warmup), str(args.seed), "0", str(args.steps)]
+    if fused:
+        cmd.append("fused")
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
+    if r.returncode != 0:
+        raise SystemExit(f"comm_bench failed ({r.returncode}): {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def measure_cpp(args, n, L, w, sigma, periodic):
+    """The hot path through the drop-in C++ API (dsdecompx::Decomp), driven
+    by the compiled driver dsdecompx_amd/bin/comm_bench: migrate, then
+    exchange_halo, then reverse_halo.  The same loop with
+    Decomp::migrate_and_exchange_halo (the fused extension) runs after it
+    and is reported under `fused`."""
     env = dict(os.environ)
     tuned = None
     if args.tune:
         from dsdecompx_amd import autotune
         shape, tuned = tune_shape(args, n, L, w, periodic)
         env.update(autotune.env_for(shape))
-    cmd = [exe, str(n), repr(L), repr(w), repr(sigma), "1" if periodic[2] else "0",
-           str(args.steps), str(args.warmup), str(args.seed), "0", str(args.steps)]
-    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
-    if r.returncode != 0:
-        raise SystemExit(f"comm_bench failed ({r.returncode}): {r.stderr[-2000:]}")
-    res = json.loads(r.stdout.strip().splitlines()[-1])
+    res = run_comm_bench(args, n, L, w, sigma, periodic, env)
+    res["env"] = env
     res["path"] = "C++ API (dsdecompx::Decomp migrate / exchange_halo / reverse_halo)"
     res["autotune"] = tuned
+    if args.fused:
+        res["fused_run"] = run_comm_bench(args, n, L, w, sigma, periodic, env, fused=True)
     return res
 
 
@@ -294,7 +306,7 @@ def measure_batched(args):
     t_loop = timed(loop_step, steps)
     loop_state = {k: getattr(b, k).clone() for k in start}
     tuned = None
-    if args.tune:
+    if args.batch_tune:
         from dsdecompx_amd import autotune
         restore()
         tuned = autotune.tune_batch(b)
@@ -333,6 +345,73 @@ def measure_batched(args):
         "timing": "wall clock over the steps, application displacement and force fill included "
                   "in every form",
     }
+
+
+# Library kernel name -> device kernel (as rocprofv3 names it).
+DEVICE_KERNELS = {"migrate_wrap": "migrate_wrap_kernel", "halo_classify": "classify_kernel",
+                  "halo_scan": "scan_kernel", "halo_scatter": "route_kernel",
+                  "halo_copy": "copy_kernel", "reverse_halo": "reverse_forces_kernel"}
+CALIBRATION_BYTES = 1 << 30
+
+
+def _counter_means(path):
+    """Mean counter value per dispatch, by plain kernel name."""
+    import csv
+    sums = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            name = row["Kernel_Name"].replace("void ", "").replace("(anonymous namespace)::", "")
+            name = name.replace("dsdx::", "").split("(")[0].split("<")[0].strip()
+            total, count = sums.get(name, (0.0, 0))
+            sums[name] = (total + float(row["Counter_Value"]), count + 1)
+    return {k: t / c for k, (t, c) in sums.items()}
+
+
+def measure_traffic(n, L, w, sigma, periodic, env, seed):
+    """HBM bytes per dispatch of every kernel of the step, from two rocprofv3
+    counter runs of the C++ driver on the same workload (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
+    for the driver's plain copy of CALIBRATION_BYTES in the same run.
+    None when rocprofv3 is not on PATH."""
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    exe = os.path.join(ROOT, "dsdecompx_amd", "bin", "comm_bench")
+    env = dict(env)
+    env["TMPDIR"] = "/tmp"
+    tmp = tempfile.mkdtemp(prefix="dsdx_pmc_", dir="/tmp")
+    means, scale = {}, {}
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out_dir = os.path.join(tmp, counter)
+            cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv",
+                   "-d", out_dir, "-o", "run", "--", exe, str(n), repr(L), repr(w), repr(sigma),
+                   "1" if periodic[2] else "0", "5", "2", str(seed), str(CALIBRATION_BYTES), "0"]
+            r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                               env=env, cwd="/tmp")
+            files = glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True)
+            if r.returncode != 0 or not files:
+                return {"error": f"rocprofv3 --pmc {counter} exited {r.returncode}: "
+                                 f"{r.stderr.strip()[-300:]}"}
+            means[counter] = _counter_means(files[0])
+            scale[counter] = means[counter]["calibration_copy"] * 1024.0 / CALIBRATION_BYTES
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    per_kernel = {}
+    for lib_name, dev in DEVICE_KERNELS.items():
+        if dev in means["FETCH_SIZE"] and dev in means["WRITE_SIZE"]:
+            rd = means["FETCH_SIZE"][dev] * 1024.0 / scale["FETCH_SIZE"]
+            wr = means["WRITE_SIZE"][dev] * 1024.0 / scale["WRITE_SIZE"]
+            per_kernel[lib_name] = {"read_bytes": rd, "write_bytes": wr, "bytes": rd + wr}
+    return {"kernels": per_kernel,
+            "scale": {"FETCH_SIZE": scale["FETCH_SIZE"], "WRITE_SIZE": scale["WRITE_SIZE"]},
+            "method": f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, on comm_bench (5 steps after "
+                      f"2 warm-up steps); KiB per dispatch x 1024 / (counter of a "
+                      f"{CALIBRATION_BYTES}-byte plain copy x 1024 / {CALIBRATION_BYTES})"}
 
 
 def run(args):
@@ -382,7 +461,8 @@ def run(args):
         "roofline": {
             "bound": "hbm", "kernel": dominant,
             "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": dom_gbs / HBM_PEAK_GBS, "traffic": args.traffic,
+            "frac": dom_gbs / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes_per_launch": ab[dominant],
         },
         "step_hbm": {
             "algorithmic_bytes": step_bytes,
@@ -396,6 +476,38 @@ def run(args):
         "kernels": per_kernel,
         "autotune": res.get("autotune"),
     }
+    if args.traffic and args.path == "cpp":
+        t = measure_traffic(n, L, w, sigma, periodic, res.get("env", dict(os.environ)), args.seed)
+        if t and "kernels" in t:
+            for k, v in t["kernels"].items():
+                if k in per_kernel:
+                    per_kernel[k]["hbm_bytes"] = v["bytes"]
+                    per_kernel[k]["hbm_read_bytes"] = v["read_bytes"]
+                    per_kernel[k]["hbm_write_bytes"] = v["write_bytes"]
+                    if per_kernel[k].get("avg_ms"):
+                        per_kernel[k]["hbm_GBps"] = v["bytes"] / (per_kernel[k]["avg_ms"] * 1e-3) / 1e9
+            dom = t["kernels"].get(dominant)
+            if dom:
+                out["roofline"]["traffic"] = dom["bytes"]
+                out["roofline"]["traffic_read"] = dom["read_bytes"]
+                out["roofline"]["traffic_write"] = dom["write_bytes"]
+            out["roofline"]["traffic_method"] = t["method"]
+            out["roofline"]["traffic_scale"] = t["scale"]
+        elif t:
+            out["roofline"]["traffic_error"] = t["error"]
+    fr = res.get("fused_run")
+    if fr:
+        fms = fr["hot_ms_per_step"]
+        fab = dict(ab)
+        fab["migrate_classify"] = fab.pop("migrate_wrap") + fab.pop("halo_classify")
+        out["fused"] = {
+            "path": "C++ API extension Decomp::migrate_and_exchange_halo, then reverse_halo "
+                    "(same results as the separate calls; one pass over the positions)",
+            "value": n / (fms * 1e3), "unit": "Mparticles/s", "ms_per_step": fms,
+            "step_hbm_frac_of_peak": sum(fab.values()) / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "kernels": {k: {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"]}
+                        for k, v in fr["kernels"].items() if v["launches"]},
+        }
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(name, n, L, w, sigma, periodic,
                                                                  args.cpu_steps)
     if not args.no_batched:
@@ -425,12 +537,18 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batched", action="store_true", help="skip the cfg5 batched comparison")
-    ap.add_argument("--no-tune", dest="tune", action="store_false",
-                    help="keep the default launch shape instead of autotuning it")
+    ap.add_argument("--no-fused", dest="fused", action="store_false",
+                    help="skip the fused migrate_and_exchange_halo run")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the launch shape of the single box before measuring "
+                         "(default: the library's default shape)")
+    ap.add_argument("--no-batch-tune", dest="batch_tune", action="store_false",
+                    help="keep the default launch shape for the batched replicas")
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=20)
-    ap.add_argument("--traffic", type=float, default=None,
-                    help="HBM bytes per launch of the dominant kernel from rocprofv3 counters")
+    ap.add_argument("--no-traffic", dest="traffic", action="store_false",
+                    help="skip the rocprofv3 counter runs that measure each kernel's HBM bytes "
+                         "(use it when bench.py itself runs under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the loopback path has no "

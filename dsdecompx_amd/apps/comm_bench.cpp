@@ -13,7 +13,9 @@
 // FETCH_SIZE / WRITE_SIZE counters.  Prints one JSON object on stdout.
 //
 // usage: comm_bench particles L width sigma z_periodic steps warmup seed
-//                   [calibration_bytes [kernel_steps]]
+//                   [calibration_bytes [kernel_steps [fused]]]
+// With "fused" the step calls Decomp::migrate_and_exchange_halo instead of
+// migrate + exchange_halo (same results, one pass over the positions).
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -134,7 +136,7 @@ int main(int argc, char** argv)
     if (argc < 9) {
         std::fprintf(stderr,
                      "usage: %s particles L width sigma z_periodic steps warmup seed [calibration_bytes "
-                     "[kernel_steps]]\n",
+                     "[kernel_steps [fused]]]\n",
                      argv[0]);
         return 2;
     }
@@ -148,6 +150,7 @@ int main(int argc, char** argv)
     const uint64_t seed = std::strtoull(argv[8], nullptr, 10);
     const uint64_t calibration = argc > 9 ? std::strtoull(argv[9], nullptr, 10) / 8 : 0;
     const int kernel_steps = argc > 10 ? std::atoi(argv[10]) : steps;
+    const bool fused = argc > 11 && std::string(argv[11]) == "fused";
 
     Config config;
     config.domain.upper = {L, L, L};
@@ -175,8 +178,13 @@ int main(int argc, char** argv)
             hipLaunchKernelGGL(move, dim3(grid(n)), dim3(256), 0, s, p.x_data(), p.y_data(), p.z_data(),
                                n, sigma, seed, step, zper ? 1 : 0, L);
             if (timed) ok(hipEventRecord(ev[0], s), "record");
-            decomp.migrate(p, s);
-            const HaloResult r = decomp.exchange_halo(p, {true, s});
+            HaloResult r;
+            if (fused) {
+                r = decomp.migrate_and_exchange_halo(p, {true, s});
+            } else {
+                decomp.migrate(p, s);
+                r = decomp.exchange_halo(p, {true, s});
+            }
             if (timed) ok(hipEventRecord(ev[1], s), "record");
             const uint64_t total = n + r.ghost_count;
             hipLaunchKernelGGL(fill_forces, dim3(grid(total)), dim3(256), 0, s, p.fx_data(), p.fy_data(),
@@ -237,7 +245,7 @@ int main(int argc, char** argv)
     }
 
     const char* kernels[] = {"migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "halo_copy",
-                             "reverse_halo"};
+                             "reverse_halo", "migrate_classify"};
     std::printf("{\"particles\": %llu, \"steps\": %d, \"warmup\": %d, \"hot_ms_per_step\": %.6f, "
                 "\"wall_ms_per_step\": %.6f, \"ghosts_per_step\": %.1f, \"last_ghosts\": %llu, "
                 "\"owners\": %llu, \"kernel_steps\": %d, \"kernel_pass_hot_ms_per_step\": %.6f, "
@@ -245,7 +253,7 @@ int main(int argc, char** argv)
                 static_cast<unsigned long long>(n), steps, warmup, hot_ms / steps, wall_ms / steps,
                 ghosts_sum / steps, static_cast<unsigned long long>(last_ghosts), owners, kernel_steps,
                 kernel_steps > 0 ? kernel_pass_hot_ms / kernel_steps : 0.0);
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 7; ++k) {
         double ms = 0.0;
         uint64_t launches = 0;
         dsdx_timing_read(kernels[k], &ms, &launches);

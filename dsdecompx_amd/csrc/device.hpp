@@ -72,10 +72,10 @@ struct ReplicaDesc {
 // Launch shape of the classify / scatter / owner-major kernels: items per
 // lane (the chunk is kWave * items particles) and waves per block.
 struct Shape {
-    int items = 8;
-    int waves = 4;
+    int items = 16;
+    int waves = 2;
     int scan_threads = 1024;
-    int xcd_run = 8;    // consecutive blocks per XCD turn (0: hardware order)
+    int xcd_run = 16;   // consecutive blocks per XCD turn (0: hardware order)
 };
 
 // Owner tiles: the owner slots [0, bound_r) of replica r (relative to

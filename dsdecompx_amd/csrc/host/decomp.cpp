@@ -192,6 +192,7 @@ struct Decomp::Impl {
     std::size_t plan_ghosts = 0;
     std::string plan_schema;
     bool shifts_changed = false;   // rescale_domain since the last build
+    bool wrap_in_build = false;    // the next rebuild also applies migrate's wrap
 
     struct Pending {
         bool active = false;
@@ -427,8 +428,11 @@ void Decomp::begin_halo_exchange(ParticleInfo& particles, HaloExchangeOptions op
         m.device_halo = false;
         const dsdx_particles p = to_abi(input);
         const std::size_t capacity = field_capacity(input, particles.capacity());
-        check(dsdx_halo_build(m.plan, &m.box, &p, fields.data(), static_cast<uint32_t>(fields.size()),
-                              owned, capacity, options.stream, nullptr));
+        const bool wrap = m.wrap_in_build;
+        m.wrap_in_build = false;
+        check((wrap ? dsdx_migrate_halo_build : dsdx_halo_build)(
+            m.plan, &m.box, &p, fields.data(), static_cast<uint32_t>(fields.size()), owned, capacity,
+            options.stream, nullptr));
         m.shifts_changed = false;
     }
     pending.active = true;
@@ -484,6 +488,49 @@ HaloResult Decomp::end_halo_exchange(ParticleInfo& particles)
     return HaloResult{ghosts, layout_changed, reallocated, pending.reuse};
 }
 
+namespace {
+
+void check_migration_view(const ConstParticleView& input, std::size_t owned, ExecutionStream stream)
+{
+    if (!stream)
+        throw std::invalid_argument(
+            "dsDecompX CUDA migration requires an explicit communication stream");
+    if (owned && (!input.x || !input.y || !input.z || !input.vx || !input.vy || !input.vz ||
+                  !input.fx || !input.fy || !input.fz || !input.mass || !input.charge ||
+                  !input.type || !input.global_id))
+        throw std::invalid_argument("dsDecompX: incomplete CUDA migration view");
+}
+
+}  // namespace
+
+HaloResult Decomp::migrate_and_exchange_halo(ParticleInfo& particles, HaloExchangeOptions options)
+{
+    Impl& m = *impl_;
+    if (m.operation_active)
+        throw std::logic_error("dsDecompX: migration cannot overlap an active halo operation");
+    const std::size_t owned = particles.owned_count();
+    ConstParticleView input = particles.read_view(owned);
+    input.count = owned;
+    require_device(input.memory_space, "migrate");
+    check_migration_view(input, owned, options.stream);
+    all_fields(input, owned);
+    // migrate()'s bookkeeping, then a rebuild whose classification pass wraps
+    // the positions first (dsdx_migrate_halo_build).
+    m.drop_device_halo();
+    particles.resize_storage(owned, 0);
+    particles.commit_counts(owned, 0);
+    if (m.perf_enabled) ++m.stats.migration_calls;
+    options.rebuild = true;
+    m.wrap_in_build = true;
+    try {
+        begin_halo_exchange(particles, options);
+    } catch (...) {
+        m.wrap_in_build = false;
+        throw;
+    }
+    return end_halo_exchange(particles);
+}
+
 MigrationResult Decomp::migrate(ParticleInfo& particles, ExecutionStream stream)
 {
     Impl& m = *impl_;
@@ -493,13 +540,7 @@ MigrationResult Decomp::migrate(ParticleInfo& particles, ExecutionStream stream)
     ConstParticleView input = particles.read_view(owned);
     input.count = owned;
     require_device(input.memory_space, "migrate");
-    if (!stream)
-        throw std::invalid_argument(
-            "dsDecompX CUDA migration requires an explicit communication stream");
-    if (owned && (!input.x || !input.y || !input.z || !input.vx || !input.vy || !input.vz ||
-                  !input.fx || !input.fy || !input.fz || !input.mass || !input.charge ||
-                  !input.type || !input.global_id))
-        throw std::invalid_argument("dsDecompX: incomplete CUDA migration view");
+    check_migration_view(input, owned, stream);
     all_fields(input, owned);
     // At 1x1x1 every particle stays, in order, with its periodic coordinates
     // wrapped; ghosts are discarded and the device halo dropped

@@ -183,6 +183,12 @@ public:
     HaloResult end_halo_exchange(ParticleInfo& particles);
     bool can_refresh_halo(const ParticleInfo& particles) const;
     MigrationResult migrate(ParticleInfo& particles, ExecutionStream stream = nullptr);
+    // Extension with no counterpart in the reference: migrate(particles,
+    // options.stream) followed by exchange_halo(particles, {rebuild, ...}),
+    // with bit-identical results, but migrate's periodic wrap runs inside the
+    // halo classification pass (one read of the positions instead of two).
+    // For loops in which nothing touches the positions between the two calls.
+    HaloResult migrate_and_exchange_halo(ParticleInfo& particles, HaloExchangeOptions options = {});
     RebalanceResult rebalance(ParticleInfo& particles, RebalanceOptions options = {},
                               ExecutionStream stream = nullptr);
     void reverse_halo(ParticleInfo& particles, ExecutionStream stream = nullptr);

@@ -152,9 +152,11 @@ struct HostState {
 
 // cfg1-style loop through the C++ API, every array compared with the oracle
 // after every step; the halo is rebuilt each step after migrate; every third
-// step also refreshes the reused plan (positions only, then full).
+// step also refreshes the reused plan (positions only, then full).  With
+// `fused` the step calls migrate_and_exchange_halo instead of migrate +
+// exchange_halo; the oracle side is the same two operations.
 static void test_steps_against_oracle(hipStream_t s, std::size_t n, double L, double w,
-                                      bool zperiodic, int steps)
+                                      bool zperiodic, int steps, bool fused = false)
 {
     Decomp decomp(MPI_COMM_WORLD, make_config(L, w, zperiodic));
     ParticleInfo p = ParticleInfo::allocate(MemorySpace::CudaDevice, n);
@@ -224,10 +226,16 @@ static void test_steps_against_oracle(hipStream_t s, std::size_t n, double L, do
             z[i] += 0.1 * (uniform(seed) - 0.5);
         }
         push(n);
-        const MigrationResult mr = decomp.migrate(p, s);
-        EXPECT(mr.new_owned_count == n && !mr.ownership_changed, "migrate keeps every particle");
+        HaloResult hr;
+        if (fused) {
+            hr = decomp.migrate_and_exchange_halo(p, {false, s});
+            EXPECT(p.owned_count() == n, "fused migrate keeps every particle");
+        } else {
+            const MigrationResult mr = decomp.migrate(p, s);
+            EXPECT(mr.new_owned_count == n && !mr.ownership_changed, "migrate keeps every particle");
+            hr = decomp.exchange_halo(p, {false, s});
+        }
         dsdxo_migrate_wrap(&box, static_cast<int64_t>(n), x.data(), y.data(), z.data());
-        const HaloResult hr = decomp.exchange_halo(p, {false, s});
         EXPECT(!hr.plan_reused, "migrate invalidates the plan");
         dsdxo_halo_build(&box, &nb, static_cast<int64_t>(n), x.data(), y.data(), z.data(), &plan);
         dsdxo_halo_write_ghosts(&nb, &plan, static_cast<int64_t>(n), &op, of, 2);
@@ -332,6 +340,8 @@ int main()
         test_steps_against_oracle(s, 32768, 32.0, 2.8, true, 6);
         test_steps_against_oracle(s, 20000, 27.0, 2.8, false, 4);
         test_steps_against_oracle(s, 2048, 11.95, 2.8, true, 4);
+        test_steps_against_oracle(s, 32768, 32.0, 2.8, true, 4, true);
+        test_steps_against_oracle(s, 20000, 27.0, 2.8, false, 3, true);
     } catch (const std::exception& e) {
         std::printf("FAILED: exception %s\n", e.what());
         ++g_failures;
