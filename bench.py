@@ -192,15 +192,13 @@ def tune_shape(args, n, L, w, periodic):
     return report["best"], report
 
 
-def run_comm_bench(args, n, L, w, sigma, periodic, env, fused=False):
+def run_comm_bench(args, n, L, w, sigma, periodic, env, mode="split"):
     import subprocess
     exe = os.path.join(ROOT, "dsdecompx_amd", "bin", "comm_bench")
     if not os.path.exists(exe):
         raise SystemExit(f"{exe} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
     cmd = [exe, str(n), repr(L), repr(w), repr(sigma), "1" if periodic[2] else "0",
-           str(args.steps), str(args.warmup), str(args.seed), "0", str(args.steps)]
-    if fused:
-        cmd.append("fused")
+           str(args.steps), str(args.warmup), str(args.seed), "0", str(args.steps), mode]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env)
     if r.returncode != 0:
         raise SystemExit(f"comm_bench failed ({r.returncode}): {r.stderr[-2000:]}")
@@ -224,7 +222,10 @@ def measure_cpp(args, n, L, w, sigma, periodic):
     res["path"] = "C++ API (dsdecompx::Decomp migrate / exchange_halo / reverse_halo)"
     res["autotune"] = tuned
     if args.fused:
-        res["fused_run"] = run_comm_bench(args, n, L, w, sigma, periodic, env, fused=True)
+        res["fused_run"] = run_comm_bench(args, n, L, w, sigma, periodic, env, mode="fused")
+    if args.refresh:
+        res["refresh_runs"] = {m: run_comm_bench(args, n, L, w, sigma, periodic, env, mode=m)
+                               for m in ("refresh", "refresh_pos")}
     return res
 
 
@@ -508,6 +509,26 @@ def run(args):
             "kernels": {k: {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"]}
                         for k, v in fr["kernels"].items() if v["launches"]},
         }
+    rr = res.get("refresh_runs")
+    if rr:
+        # The reference benchmark's non-rebuild step: exchange_halo(rebuild = false) on the
+        # plan of the last rebuild, then reverse_halo.  Bytes per refresh: full records
+        # 168 B per ghost as in the rebuild's copy; positions only 24 B read (owner x, y, z)
+        # + 48 B written (ghost x, y, z and zeroed forces) per ghost.
+        out["refresh"] = {}
+        for m, per_ghost in (("refresh", 168.0), ("refresh_pos", 72.0)):
+            r = rr[m]
+            k = r["kernels"].get("halo_refresh", {})
+            avg = k["total_ms"] / k["launches"] if k.get("launches") else None
+            g = r["last_ghosts"]
+            out["refresh"]["full" if m == "refresh" else "positions_only"] = {
+                "ms_per_step": r["hot_ms_per_step"],
+                "value": n / (r["hot_ms_per_step"] * 1e3), "unit": "Mparticles/s",
+                "refresh_kernel_avg_ms": avg, "ghosts": g,
+                "refresh_algorithmic_GBps": per_ghost * g / (avg * 1e-3) / 1e9 if avg else None,
+                "reverse_avg_ms": (r["kernels"]["reverse_halo"]["total_ms"] /
+                                   r["kernels"]["reverse_halo"]["launches"]),
+            }
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(name, n, L, w, sigma, periodic,
                                                                  args.cpu_steps)
     if not args.no_batched:
@@ -539,6 +560,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the cfg5 batched comparison")
     ap.add_argument("--no-fused", dest="fused", action="store_false",
                     help="skip the fused migrate_and_exchange_halo run")
+    ap.add_argument("--no-refresh", dest="refresh", action="store_false",
+                    help="skip the reused-plan (refresh) runs")
     ap.add_argument("--tune", action="store_true",
                     help="autotune the launch shape of the single box before measuring "
                          "(default: the library's default shape)")

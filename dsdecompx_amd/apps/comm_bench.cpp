@@ -13,9 +13,14 @@
 // FETCH_SIZE / WRITE_SIZE counters.  Prints one JSON object on stdout.
 //
 // usage: comm_bench particles L width sigma z_periodic steps warmup seed
-//                   [calibration_bytes [kernel_steps [fused]]]
-// With "fused" the step calls Decomp::migrate_and_exchange_halo instead of
-// migrate + exchange_halo (same results, one pass over the positions).
+//                   [calibration_bytes [kernel_steps [mode]]]
+// mode "split" (default): migrate + exchange_halo(rebuild) + reverse_halo.
+// mode "fused": Decomp::migrate_and_exchange_halo instead of migrate +
+// exchange_halo (same results, one pass over the positions).
+// modes "refresh" / "refresh_pos": the reference benchmark's non-rebuild step
+// (examples/communication_test/main.cu:232-236): the plan built in the first
+// warm-up step is reused, exchange_halo(rebuild = false), full records or
+// positions only, then reverse_halo; no migrate.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -136,7 +141,7 @@ int main(int argc, char** argv)
     if (argc < 9) {
         std::fprintf(stderr,
                      "usage: %s particles L width sigma z_periodic steps warmup seed [calibration_bytes "
-                     "[kernel_steps [fused]]]\n",
+                     "[kernel_steps [split|fused|refresh|refresh_pos]]]\n",
                      argv[0]);
         return 2;
     }
@@ -150,7 +155,12 @@ int main(int argc, char** argv)
     const uint64_t seed = std::strtoull(argv[8], nullptr, 10);
     const uint64_t calibration = argc > 9 ? std::strtoull(argv[9], nullptr, 10) / 8 : 0;
     const int kernel_steps = argc > 10 ? std::atoi(argv[10]) : steps;
-    const bool fused = argc > 11 && std::string(argv[11]) == "fused";
+    const std::string mode = argc > 11 ? argv[11] : "split";
+    if (mode != "split" && mode != "fused" && mode != "refresh" && mode != "refresh_pos") {
+        std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
+        return 2;
+    }
+    const bool refresh = mode == "refresh" || mode == "refresh_pos";
 
     Config config;
     config.domain.upper = {L, L, L};
@@ -179,8 +189,10 @@ int main(int argc, char** argv)
                                n, sigma, seed, step, zper ? 1 : 0, L);
             if (timed) ok(hipEventRecord(ev[0], s), "record");
             HaloResult r;
-            if (fused) {
+            if (mode == "fused") {
                 r = decomp.migrate_and_exchange_halo(p, {true, s});
+            } else if (refresh && step > 0) {
+                r = decomp.exchange_halo(p, {false, s, mode == "refresh_pos"});
             } else {
                 decomp.migrate(p, s);
                 r = decomp.exchange_halo(p, {true, s});
@@ -245,7 +257,7 @@ int main(int argc, char** argv)
     }
 
     const char* kernels[] = {"migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "halo_copy",
-                             "reverse_halo", "migrate_classify"};
+                             "reverse_halo", "migrate_classify", "halo_refresh"};
     std::printf("{\"particles\": %llu, \"steps\": %d, \"warmup\": %d, \"hot_ms_per_step\": %.6f, "
                 "\"wall_ms_per_step\": %.6f, \"ghosts_per_step\": %.1f, \"last_ghosts\": %llu, "
                 "\"owners\": %llu, \"kernel_steps\": %d, \"kernel_pass_hot_ms_per_step\": %.6f, "
@@ -253,7 +265,7 @@ int main(int argc, char** argv)
                 static_cast<unsigned long long>(n), steps, warmup, hot_ms / steps, wall_ms / steps,
                 ghosts_sum / steps, static_cast<unsigned long long>(last_ghosts), owners, kernel_steps,
                 kernel_steps > 0 ? kernel_pass_hot_ms / kernel_steps : 0.0);
-    for (int k = 0; k < 7; ++k) {
+    for (int k = 0; k < 8; ++k) {
         double ms = 0.0;
         uint64_t launches = 0;
         dsdx_timing_read(kernels[k], &ms, &launches);
