@@ -226,6 +226,8 @@ def measure_cpp(args, n, L, w, sigma, periodic):
     if args.refresh:
         res["refresh_runs"] = {m: run_comm_bench(args, n, L, w, sigma, periodic, env, mode=m)
                                for m in ("refresh", "refresh_pos")}
+    if args.zwalls and periodic[2]:
+        res["zwalls_run"] = run_comm_bench(args, n, L, w, sigma, (True, True, False), env)
     return res
 
 
@@ -415,6 +417,80 @@ def measure_traffic(n, L, w, sigma, periodic, env, seed):
                       f"{CALIBRATION_BYTES}-byte plain copy x 1024 / {CALIBRATION_BYTES})"}
 
 
+def measure_autograd(args):
+    """cfg4 (65,536 particles, L = 40.3, w = 2.8, one rebuild): the autograd
+    form -- owned positions through HaloPositions (positions-only refresh
+    along the saved plan), a differentiable scalar sum(out * weight) of all
+    positions, backward through the reverse_halo kernel -- against the plain
+    calls that compute the same gradient (positions-only refresh, the weights
+    written into the force arrays, reverse_halo).  Per-iteration times on
+    the same stream, and whether the two gradients agree bit for bit."""
+    import torch
+    from dsdecompx_amd import autograd as ag
+    from dsdecompx_amd.halo import BoxConfig, DeviceParticles, LoopbackHalo
+
+    n, L, w = 65536, 40.3, 2.8
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    torch.cuda.set_stream(torch.cuda.Stream(dev))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed + 9)
+    p = DeviceParticles(2 * n + 4096, dev)
+    for k in ("x", "y", "z"):
+        getattr(p, k)[:n].uniform_(0.0, L, generator=gen)
+    p.global_id[:n] = torch.arange(n, dtype=torch.int32, device=dev)
+    halo = LoopbackHalo(BoxConfig(upper=(L, L, L), width=w))
+    halo.migrate(p, n)
+    g = ag.rebuild(halo, p, n)
+    pos0 = torch.stack([p.x[:n], p.y[:n], p.z[:n]], 1).clone()
+    weight = torch.randn((n + g, 3), dtype=torch.float64, device=dev, generator=gen)
+    wcols = [weight[:, a].contiguous() for a in range(3)]
+
+    def autograd_step():
+        pos = pos0.clone().requires_grad_(True)
+        out = ag.halo_positions(pos, halo, p)
+        (out * weight).sum().backward()
+        return pos.grad
+
+    def plain_step():
+        p.x[:n] = pos0[:, 0]
+        p.y[:n] = pos0[:, 1]
+        p.z[:n] = pos0[:, 2]
+        halo.refresh(p, n, positions_only=True)
+        p.fx[:n + g] = wcols[0]
+        p.fy[:n + g] = wcols[1]
+        p.fz[:n + g] = wcols[2]
+        halo.reverse_halo(p, n)
+        return torch.stack([p.fx[:n], p.fy[:n], p.fz[:n]], 1)
+
+    def timed(fn, iters):
+        for _ in range(5):
+            fn()
+        s0 = torch.cuda.Event(enable_timing=True)
+        s1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        s0.record()
+        for _ in range(iters):
+            fn()
+        s1.record()
+        torch.cuda.synchronize()
+        return s0.elapsed_time(s1) / iters
+
+    iters = 100
+    ms_auto = timed(autograd_step, iters)
+    ms_plain = timed(plain_step, iters)
+    same = torch.equal(autograd_step().view(torch.int64), plain_step().view(torch.int64))
+    return {
+        "workload": f"cfg4: {n} particles uniform in L={L} (periodic), halo width {w}, {g} ghosts",
+        "autograd_ms_per_iter": ms_auto, "plain_ms_per_iter": ms_plain,
+        "plain_vs_autograd": ms_auto / ms_plain,
+        "bit_exact_gradient": bool(same),
+        "form": "autograd: HaloPositions forward (positions-only refresh) + sum(out*weight) + "
+                "backward (reverse_halo kernel); plain: refresh + weights into forces + "
+                "reverse_halo",
+    }
+
+
 def run(args):
     name = args.config
     n, L, w, sigma, periodic = WORKLOADS[name]
@@ -509,6 +585,16 @@ def run(args):
             "kernels": {k: {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"]}
                         for k, v in fr["kernels"].items() if v["launches"]},
         }
+    zr = res.get("zwalls_run")
+    if zr:
+        zms = zr["hot_ms_per_step"]
+        zab = algorithmic_bytes(n, zr["ghosts_per_step"], zr["owners"], 2)
+        out["z_walls"] = {
+            "workload": f"{name} with z non-periodic (reflective walls at 0 and L in the driver)",
+            "value": n / (zms * 1e3), "unit": "Mparticles/s", "ms_per_step": zms,
+            "ghosts_per_step": zr["ghosts_per_step"], "owners_per_step": zr["owners"],
+            "step_hbm_frac_of_peak": sum(zab.values()) / (zms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        }
     rr = res.get("refresh_runs")
     if rr:
         # The reference benchmark's non-rebuild step: exchange_halo(rebuild = false) on the
@@ -533,6 +619,8 @@ def run(args):
                                                                  args.cpu_steps)
     if not args.no_batched:
         out["batched"] = measure_batched(args)
+    if args.autograd:
+        out["autograd"] = measure_autograd(args)
     return out
 
 
@@ -562,6 +650,10 @@ def main():
                     help="skip the fused migrate_and_exchange_halo run")
     ap.add_argument("--no-refresh", dest="refresh", action="store_false",
                     help="skip the reused-plan (refresh) runs")
+    ap.add_argument("--no-autograd", dest="autograd", action="store_false",
+                    help="skip the cfg4 autograd-vs-plain comparison")
+    ap.add_argument("--no-zwalls", dest="zwalls", action="store_false",
+                    help="skip the z-non-periodic repeat of the workload")
     ap.add_argument("--tune", action="store_true",
                     help="autotune the launch shape of the single box before measuring "
                          "(default: the library's default shape)")

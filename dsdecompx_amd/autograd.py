@@ -40,17 +40,19 @@ def _check_same_plan(ctx) -> None:
 
 def _reverse_sum(halo: LoopbackHalo, values: torch.Tensor, owned: int) -> torch.Tensor:
     """[N+G, C] float64 -> [N, C]: owner value plus its ghosts, ascending send slot."""
+    if values.dim() == 2 and values.shape[1] == 3 and values.dtype == torch.float64:
+        # one transposing copy into SoA rows, the reverse_halo kernel on the
+        # rows, and a transposed view of the owned part as the result
+        rows = torch.empty((3, values.shape[0]), dtype=torch.float64, device=values.device)
+        rows.copy_(values.t())   # always a copy: the caller's tensor is not modified
+        _abi.check(_abi.lib().dsdx_reverse_halo(halo.plan, rows[0].data_ptr(), rows[1].data_ptr(),
+                                                rows[2].data_ptr(), owned,
+                                                torch.cuda.current_stream().cuda_stream),
+                   "reverse_halo")
+        return rows[:, :owned].t()
     values = values.contiguous().clone()
     if values.dim() == 1:
         values = values[:, None]
-    comps = values.shape[1]
-    if comps == 3 and values.dtype == torch.float64:
-        cols = [values[:, a].contiguous() for a in range(3)]
-        _abi.check(_abi.lib().dsdx_reverse_halo(halo.plan, cols[0].data_ptr(), cols[1].data_ptr(),
-                                                cols[2].data_ptr(), owned,
-                                                torch.cuda.current_stream().cuda_stream),
-                   "reverse_halo")
-        return torch.stack([c[:owned] for c in cols], 1)
     halo.reverse_exchange([(values, _abi.SUM)], owned)
     return values[:owned]
 
