@@ -34,14 +34,18 @@ class ReplicaBatch:
     """Device storage + batched halo plan for a list of replicas."""
 
     def __init__(self, specs, device="cuda", margin: float = 1.35, extra: int = 256,
-                 stream=None):
+                 stream=None, capacities=None):
+        """`capacities` (optional, one per replica) replaces the shell-based
+        estimate: particle slots for the replica's owned particles and ghosts."""
         self.device = torch.device(device)
         self.specs = list(specs)
         self.offsets, self.capacities = [], []
         off = 0
-        for s in self.specs:
+        for r, s in enumerate(self.specs):
             L = max(s.box.upper[a] - s.box.lower[a] for a in range(3))
             cap = s.owned + int(margin * shell_fraction(L, s.box.width) * s.owned) + extra
+            if capacities is not None:
+                cap = int(capacities[r])
             self.offsets.append(off)
             self.capacities.append(cap)
             off += cap

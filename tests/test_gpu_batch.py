@@ -169,3 +169,66 @@ def test_graph_replays_match_eager(gpu, shape, fused):
         got = _snapshot(b)
         for name in F64 + I32:
             assert_bits_equal(got[name], eager[k][name], f"replay {k} {name}")
+
+
+def test_overflowing_replica_is_skipped(gpu):
+    """A replica whose ghosts do not fit its storage gets its status word and
+    is left untouched (ghost slots, forces); every other replica of the same
+    launches is still bit-exact with the oracle."""
+    torch = gpu
+    from dsdecompx_amd.batch import ReplicaBatch
+    specs = _specs(6, 21)
+    bad = 2
+    probe = ReplicaBatch(specs)
+    caps = list(probe.capacities)
+    probe.close()
+    caps[bad] = specs[bad].owned + 8
+    say("overflow replica", bad, caps[bad])
+    b = ReplicaBatch(specs, capacities=caps)
+    _fill(torch, b, 22)
+    sentinel = float("nan")
+    for r, s_ in enumerate(specs):
+        sl = slice(b.offsets[r] + s_.owned, b.offsets[r] + b.capacities[r])
+        for k in ("x", "y", "z", "vx", "fx"):
+            getattr(b, k)[sl] = sentinel
+    start = _snapshot(b)
+    b.migrate()
+    b.exchange_halo()
+    ghosts, owners, status = b.counts()
+    assert status[bad] != 0 and all(st == 0 for r, st in enumerate(status) if r != bad)
+    rng = np.random.default_rng(23)
+    forces = rng.uniform(-1, 1, (3, b.total))
+    for r, s_ in enumerate(specs):
+        # forces on owned slots everywhere; ghost slots of the good replicas
+        n = s_.owned + (0 if r == bad else ghosts[r])
+        sl = slice(b.offsets[r], b.offsets[r] + n)
+        for a, k in enumerate(("fx", "fy", "fz")):
+            getattr(b, k)[sl] = torch.from_numpy(forces[a][sl]).cuda()
+    before_rev = _snapshot(b)
+    b.reverse_halo()
+    after = _snapshot(b)
+    # the overflowing replica: ghost region and forces untouched
+    lo, hi = b.offsets[bad], b.offsets[bad] + b.capacities[bad]
+    for k in F64 + I32:
+        assert_bits_equal(after[k][lo:hi], before_rev[k][lo:hi], f"overflowed replica {k}")
+    n_bad = specs[bad].owned
+    for k in ("x", "y", "z"):
+        assert np.isnan(after[k][lo + n_bad:hi]).all()
+    # the others: oracle
+    for r, s_ in enumerate(specs):
+        if r == bad:
+            continue
+        n, g, off = s_.owned, ghosts[r], b.offsets[r]
+        o = po.Oracle(po.Box(upper=s_.box.upper, width=s_.box.width))
+        p = po.Particles(b.capacities[r])
+        for k in F64 + I32:
+            getattr(p, k)[:n] = start[k][off:off + n]
+        o.migrate(p, n)
+        assert o.exchange_halo(p, n) == g
+        for k in ("x", "y", "z", "vx", "vy", "vz", "mass", "charge", "type", "global_id"):
+            assert_bits_equal(before_rev[k][off:off + n + g], getattr(p, k)[:n + g], f"r{r} {k}")
+        for a, k in enumerate(("fx", "fy", "fz")):
+            getattr(p, k)[:n + g] = forces[a][off:off + n + g]
+        o.reverse_halo(p, n)
+        for k in ("fx", "fy", "fz"):
+            assert_bits_equal(after[k][off:off + n], getattr(p, k)[:n], f"r{r} reverse {k}")
