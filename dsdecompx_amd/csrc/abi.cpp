@@ -378,6 +378,7 @@ int shape_from_env(Shape* s)
     if (const char* v = std::getenv("DSDX_WAVES")) s->waves = std::atoi(v);
     if (const char* v = std::getenv("DSDX_SCAN_THREADS")) s->scan_threads = std::atoi(v);
     if (const char* v = std::getenv("DSDX_XCD_RUN")) s->xcd_run = std::atoi(v);
+    if (const char* v = std::getenv("DSDX_DESCENDING")) s->descending = std::atoi(v);
     return shape_supported(*s) ? DSDX_OK : fail(DSDX_EINVAL, "unsupported launch shape");
 }
 
@@ -470,7 +471,7 @@ int dsdx_plan_reserve(dsdx_plan* plan, uint64_t owned, uint64_t ghosts)
 int dsdx_plan_set_shape(dsdx_plan* plan, int32_t items, int32_t waves, int32_t scan_threads)
 {
     if (!plan) return fail(DSDX_EINVAL, "null plan");
-    const Shape s{items, waves, scan_threads, plan->shape.xcd_run};
+    const Shape s{items, waves, scan_threads, plan->shape.xcd_run, plan->shape.descending};
     if (!shape_supported(s)) return fail(DSDX_EINVAL, "unsupported launch shape");
     if (s.items != plan->shape.items) {
         plan->classified = plan->built = plan->totals_cached = false;
@@ -761,7 +762,7 @@ int dsdx_batch_set_shape(dsdx_batch* batch, int32_t items, int32_t waves, int32_
 {
     if (!batch) return fail(DSDX_EINVAL, "null batch");
     if (!stream) return fail(DSDX_EINVAL, "batch setup requires an explicit stream");
-    const Shape s{items, waves, scan_threads, batch->shape.xcd_run};
+    const Shape s{items, waves, scan_threads, batch->shape.xcd_run, batch->shape.descending};
     if (!shape_supported(s)) return fail(DSDX_EINVAL, "unsupported launch shape");
     const bool relayout = s.items != batch->shape.items;
     batch->shape = s;

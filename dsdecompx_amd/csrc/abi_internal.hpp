@@ -85,6 +85,7 @@ struct Plan {
         L.chunk_size = chunk_size();
         L.tiles = tiles;
         L.xcd_run = static_cast<uint32_t>(shape.xcd_run);
+        L.descending = 0;
         return L;
     }
 

@@ -76,7 +76,21 @@ struct Shape {
     int waves = 2;
     int scan_threads = 1024;
     int xcd_run = 16;   // consecutive blocks per XCD turn (0: hardware order)
+    // Bit mask of passes that walk their chunks / tiles last to first.  The
+    // first pass over x, y, z after the application wrote them (migrate, or
+    // the fused migrate + classify) walks backwards: the most recently
+    // written lines are the ones still held by the 256 MB memory-side cache.
+    // The classify pass that follows migrate then walks forwards, into the
+    // lines migrate read last.
+    int descending = 1;
 };
+
+// Bits of Shape::descending.
+constexpr int kOrderMigrate = 1;
+constexpr int kOrderClassify = 2;
+constexpr int kOrderRoute = 4;
+constexpr int kOrderCopy = 8;
+constexpr int kOrderOwners = 16;
 
 // Owner tiles: the owner slots [0, bound_r) of replica r (relative to
 // first_chunk_r * K), bound_r = min(owned_r, capacity_r - owned_r,
@@ -94,6 +108,7 @@ struct Layout {
     uint32_t chunk_size;            // kWave * items (a power of two)
     uint32_t tiles;                 // owner tiles over all replicas
     uint32_t xcd_run;               // block renumbering (see xcd_block)
+    uint32_t descending;            // this launch walks its chunks / tiles last to first
 };
 
 struct PlanBuffers {

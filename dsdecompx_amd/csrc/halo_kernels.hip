@@ -65,11 +65,14 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t run)
     return b - l + (l % kXcds) * run + l / kXcds;
 }
 
-// The chunk (or tile) of this wave (wave-uniform).
+// The chunk (or tile) of this wave among `count` (wave-uniform); a
+// descending launch walks them last to first.  Indices >= count (the padding
+// waves of the last block) are returned unchanged.
 template <int WAVES>
-__device__ __forceinline__ uint32_t wave_chunk(uint32_t run)
+__device__ __forceinline__ uint32_t wave_chunk(const Layout& L, uint32_t count)
 {
-    return uniform(xcd_block(run) * WAVES + threadIdx.x / kWave);
+    const uint32_t c = uniform(xcd_block(L.xcd_run) * WAVES + threadIdx.x / kWave);
+    return (L.descending && c < count) ? count - 1u - c : c;
 }
 
 // Replica of a chunk.  The descriptor table and the chunk -> replica map
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
     __shared__ double s_ring_p[WAVES][3][kRing];
     const uint32_t lane = lane_id();
     const uint32_t wave = threadIdx.x / kWave;
-    const uint32_t chunk = wave_chunk<WAVES>(L.xcd_run);
+    const uint32_t chunk = wave_chunk<WAVES>(L, L.chunks);
     if (chunk >= L.chunks) return;
     const ReplicaDesc* d = desc + replica_of(chunk_replica, chunk);
     const uint32_t lut = d->bp.lut[lane];
@@ -348,7 +351,7 @@ __device__ __forceinline__ OwnerLane owner_of_lane(const ReplicaDesc* __restrict
                                                    const Layout& L, const PlanBuffers& B)
 {
     OwnerLane o{nullptr, 0, false};
-    const uint32_t tile = wave_chunk<WAVES>(L.xcd_run);
+    const uint32_t tile = wave_chunk<WAVES>(L, L.tiles);
     if (tile >= L.tiles) return o;
     const uint32_t r = tile_replica ? uniform(tile_replica[tile]) : 0u;
     o.d = desc + r;
@@ -386,7 +389,7 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
 {
     constexpr uint32_t K = ITEMS * kWave;
     const uint32_t lane = lane_id();
-    const uint32_t chunk = wave_chunk<WAVES>(L.xcd_run);
+    const uint32_t chunk = wave_chunk<WAVES>(L, L.chunks);
     if (chunk >= L.chunks) return;
     const uint32_t r = replica_of(chunk_replica, chunk);
     const ReplicaDesc* d = desc + r;
@@ -676,7 +679,7 @@ __global__ __launch_bounds__(WAVES * kWave) void migrate_wrap_kernel(
     const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ chunk_replica, Layout L)
 {
     constexpr uint32_t K = ITEMS * kWave;
-    const uint32_t chunk = wave_chunk<WAVES>(L.xcd_run);
+    const uint32_t chunk = wave_chunk<WAVES>(L, L.chunks);
     if (chunk >= L.chunks) return;
     const ReplicaDesc* d = desc + replica_of(chunk_replica, chunk);
     const uint32_t owned = d->owned;
@@ -753,6 +756,14 @@ hipError_t launch(void (*kernel)(Params...), dim3 grid, dim3 block, hipStream_t 
 }
 
 dim3 chunk_grid(const Layout& L, int waves) { return dim3(wave_blocks(L.chunks, waves)); }
+
+// The layout of one launch of a pass (its walking order from Shape::descending).
+Layout ordered(const Layout& L, const Shape& s, int bit)
+{
+    Layout out = L;
+    out.descending = (s.descending & bit) ? 1u : 0u;
+    return out;
+}
 dim3 tile_grid(const Layout& L, int waves) { return dim3(wave_blocks(L.tiles, waves)); }
 
 template <class F>
@@ -771,14 +782,15 @@ hipError_t dispatch_waves(const Shape& s, F&& f)
 hipError_t launch_classify(double* x, double* y, double* z, bool wrap_first, const Layout& L,
                            const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
+    const Layout O = ordered(L, s, wrap_first ? kOrderMigrate : kOrderClassify);
     if (L.chunks == 0) return hipSuccess;
     return dispatch(s, [&](auto items, auto waves) {
         constexpr int I = decltype(items)::value, W = decltype(waves)::value;
         if (wrap_first)
             return launch(classify_kernel<I, W, true>, chunk_grid(L, W), dim3(W * kWave), stream,
-                          x, y, z, L.desc, L.chunk_replica, L, B);
+                          x, y, z, L.desc, L.chunk_replica, O, B);
         return launch(classify_kernel<I, W, false>, chunk_grid(L, W), dim3(W * kWave), stream,
-                      x, y, z, L.desc, L.chunk_replica, L, B);
+                      x, y, z, L.desc, L.chunk_replica, O, B);
     });
 }
 
@@ -790,88 +802,96 @@ hipError_t launch_scan(const Layout& L, const PlanBuffers& B, const Shape& s, hi
 
 hipError_t launch_route(const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
+    const Layout O = ordered(L, s, kOrderRoute);
     if (L.chunks == 0) return hipSuccess;
     return dispatch(s, [&](auto items, auto waves) {
         constexpr int I = decltype(items)::value, W = decltype(waves)::value;
         return launch(route_kernel<I, W>, chunk_grid(L, W), dim3(W * kWave), stream, L.desc,
-                      L.chunk_replica, L, B);
+                      L.chunk_replica, O, B);
     });
 }
 
 hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Layout& L,
                        const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
+    const Layout O = ordered(L, s, kOrderCopy);
     if (L.tiles == 0) return hipSuccess;
     return dispatch_waves(s, [&](auto waves) {
         constexpr int W = decltype(waves)::value;
         return launch(copy_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, p, fp, L.desc,
-                      L.tile_replica, L, B);
+                      L.tile_replica, O, B);
     });
 }
 
 hipError_t launch_reverse_forces(double* fx, double* fy, double* fz, const Layout& L,
                                  const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
+    const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
     return dispatch_waves(s, [&](auto waves) {
         constexpr int W = decltype(waves)::value;
         return launch(reverse_forces_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, fx, fy, fz,
-                      L.desc, L.tile_replica, L, B);
+                      L.desc, L.tile_replica, O, B);
     });
 }
 
 hipError_t launch_reverse_fields(const TypedFields& tf, const Layout& L, const PlanBuffers& B,
                                  const Shape& s, hipStream_t stream)
 {
+    const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
     return dispatch_waves(s, [&](auto waves) {
         constexpr int W = decltype(waves)::value;
         return launch(reverse_fields_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, tf,
-                      L.desc, L.tile_replica, L, B);
+                      L.desc, L.tile_replica, O, B);
     });
 }
 
 hipError_t launch_forward_fields(const FieldParams& fp, const Layout& L, const PlanBuffers& B,
                                  const Shape& s, hipStream_t stream)
 {
+    const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
     return dispatch_waves(s, [&](auto waves) {
         constexpr int W = decltype(waves)::value;
         return launch(forward_fields_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, fp,
-                      L.desc, L.tile_replica, L, B);
+                      L.desc, L.tile_replica, O, B);
     });
 }
 
 hipError_t launch_refresh(const dsdx_particles& p, const FieldParams& fp, int32_t positions_only,
                           const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
+    const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
     return dispatch_waves(s, [&](auto waves) {
         constexpr int W = decltype(waves)::value;
         return launch(refresh_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, p, fp,
-                      positions_only, L.desc, L.tile_replica, L, B);
+                      positions_only, L.desc, L.tile_replica, O, B);
     });
 }
 
 hipError_t launch_ghost_owners(int32_t* out, const Layout& L, const PlanBuffers& B, const Shape& s,
                                hipStream_t stream)
 {
+    const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
     return dispatch_waves(s, [&](auto waves) {
         constexpr int W = decltype(waves)::value;
         return launch(ghost_owners_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, out, L.desc,
-                      L.tile_replica, L, B);
+                      L.tile_replica, O, B);
     });
 }
 
 hipError_t launch_migrate_wrap(double* x, double* y, double* z, const Layout& L, const Shape& s,
                                hipStream_t stream)
 {
+    const Layout O = ordered(L, s, kOrderMigrate);
     if (L.chunks == 0) return hipSuccess;
     return dispatch(s, [&](auto items, auto waves) {
         constexpr int I = decltype(items)::value, W = decltype(waves)::value;
         return launch(migrate_wrap_kernel<I, W>, chunk_grid(L, W), dim3(W * kWave), stream,
-                      x, y, z, L.desc, L.chunk_replica, L);
+                      x, y, z, L.desc, L.chunk_replica, O);
     });
 }
 
