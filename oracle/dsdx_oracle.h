@@ -126,6 +126,54 @@ int dsdxo_forward_exchange(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
 void dsdxo_migrate_wrap(const dsdxo_box* box, int64_t owned, double* x,
                         double* y, double* z);
 
+/* ---- P-rank process grids (dsdx_oracle_grid.c) ---------------------------
+ * The unmodified Host path on a dims[0] x dims[1] x dims[2] Cartesian grid,
+ * every rank restated in one process.  Parity pinning (tests/test_oracle_grid.py):
+ *   dsdxo_grid_build         closed-form cuts, row-major ranks, neighbour
+ *                            ranks and shifts against an independent
+ *                            enumeration; unsplit-axis skip; the reference
+ *                            examples' neighbour counts
+ *   dsdxo_grid_owner_rank    brute force over the subdomain boxes
+ *   dsdxo_grid_migrate       conservation of the id multiset, every particle
+ *                            inside its owner's subdomain, retained-first
+ *                            order, the reference example particle_migration
+ *   dsdxo_grid_halo          the reference examples' ghost counts and
+ *                            returned forces (getting_started: 1/3/7 ghosts on
+ *                            2/4/8 ranks), exact-rational geometric brute force
+ *                            of every rank's ghost set, the 1x1x1 loopback
+ *                            oracle when the unsplit-axis skip is lifted
+ *   dsdxo_grid_reverse_exchange / dsdxo_grid_forward_exchange
+ *                            exact conservation, adjointness, the loopback
+ *                            oracle at 1x1x1 */
+#define DSDXO_MAX_RANKS 64
+
+typedef struct {
+    dsdxo_box global;
+    int32_t dims[3];
+    int32_t ranks;
+    double cuts[3][DSDXO_MAX_RANKS + 1];
+    int32_t coord[DSDXO_MAX_RANKS][3];
+    dsdxo_box local[DSDXO_MAX_RANKS];          /* subdomain, global width/periodicity */
+    dsdxo_neighbors nb[DSDXO_MAX_RANKS];       /* neighbour list of every rank */
+    int32_t nb_rank[DSDXO_MAX_RANKS][DSDXO_MAX_NEIGHBORS]; /* -1: outside a wall */
+} dsdxo_grid;
+
+int dsdxo_grid_build(const dsdxo_box* box, const int32_t dims[3], int32_t lift_unsplit,
+                     dsdxo_grid* g);
+int32_t dsdxo_grid_owner_rank(const dsdxo_grid* g, double x, double y, double z);
+int dsdxo_grid_migrate(const dsdxo_grid* g, dsdxo_particles* p, const int64_t* capacity,
+                       int64_t* owned, const dsdxo_field* fields, int32_t field_count,
+                       int64_t* sent, int64_t* received);
+int dsdxo_grid_halo(const dsdxo_grid* g, dsdxo_particles* p, const int64_t* capacity,
+                    const int64_t* owned, dsdxo_plan* plans, int32_t* recv_rank,
+                    int64_t* ghosts, const dsdxo_field* fields, int32_t field_count);
+int dsdxo_grid_reverse_exchange(const dsdxo_grid* g, const dsdxo_plan* plans,
+                                const int64_t* owned, const dsdxo_field* fields,
+                                int32_t field_count);
+int dsdxo_grid_forward_exchange(const dsdxo_grid* g, const dsdxo_plan* plans,
+                                const int32_t* recv_rank, const int64_t* owned,
+                                const dsdxo_field* fields, int32_t field_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,6 +53,18 @@ class _Plan(C.Structure):
                 ("ghost_total", C.c_int64)]
 
 
+MAX_RANKS = 64
+
+
+class _Grid(C.Structure):
+    _fields_ = [("global_", _Box), ("dims", C.c_int32 * 3), ("ranks", C.c_int32),
+                ("cuts", (C.c_double * (MAX_RANKS + 1)) * 3),
+                ("coord", (C.c_int32 * 3) * MAX_RANKS),
+                ("local", _Box * MAX_RANKS),
+                ("nb", _Neighbors * MAX_RANKS),
+                ("nb_rank", (C.c_int32 * MAX_NEIGHBORS) * MAX_RANKS)]
+
+
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
 
@@ -102,6 +114,29 @@ def lib():
         L.dsdxo_forward_exchange.argtypes = [C.POINTER(_Neighbors), C.POINTER(_Plan),
                                              C.c_int64, C.POINTER(_Field), C.c_int32]
         L.dsdxo_migrate_wrap.argtypes = [C.POINTER(_Box), C.c_int64, _DP, _DP, _DP]
+        L.dsdxo_grid_build.argtypes = [C.POINTER(_Box), C.POINTER(C.c_int32), C.c_int32,
+                                       C.POINTER(_Grid)]
+        L.dsdxo_grid_build.restype = C.c_int
+        L.dsdxo_grid_owner_rank.argtypes = [C.POINTER(_Grid), C.c_double, C.c_double, C.c_double]
+        L.dsdxo_grid_owner_rank.restype = C.c_int32
+        L.dsdxo_grid_migrate.argtypes = [C.POINTER(_Grid), C.POINTER(_Particles),
+                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                         C.POINTER(_Field), C.c_int32,
+                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.dsdxo_grid_migrate.restype = C.c_int
+        L.dsdxo_grid_halo.argtypes = [C.POINTER(_Grid), C.POINTER(_Particles),
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                      C.POINTER(_Plan), C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int64), C.POINTER(_Field), C.c_int32]
+        L.dsdxo_grid_halo.restype = C.c_int
+        L.dsdxo_grid_reverse_exchange.argtypes = [C.POINTER(_Grid), C.POINTER(_Plan),
+                                                  C.POINTER(C.c_int64), C.POINTER(_Field),
+                                                  C.c_int32]
+        L.dsdxo_grid_reverse_exchange.restype = C.c_int
+        L.dsdxo_grid_forward_exchange.argtypes = [C.POINTER(_Grid), C.POINTER(_Plan),
+                                                  C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                                  C.POINTER(_Field), C.c_int32]
+        L.dsdxo_grid_forward_exchange.restype = C.c_int
         _lib = L
     return _lib
 
@@ -297,3 +332,115 @@ class Oracle:
         arr, n = _fields_c(fields)
         lib().dsdxo_forward_exchange(C.byref(self.nb.raw), C.byref(self.plan.raw),
                                      owned, arr, n)
+
+
+class Grid:
+    """Host path on a dims[0] x dims[1] x dims[2] process grid, every rank
+    held here (src/topology.cpp, src/halo.cpp, src/migration.cpp at P ranks).
+    ``lift_unsplit`` lifts the unsplit-axis skip (src/topology.cpp:118-127),
+    which turns 1x1x1 into the self-loopback of ``Oracle``.
+
+    ``parts[r]`` is rank r's ``Particles`` (owned first, then ghosts) and
+    ``owned[r]`` its owned count; ``fields[r]`` its registered fields (the
+    same schema on every rank)."""
+
+    def __init__(self, box: Box, dims, lift_unsplit: bool = False):
+        self.box = box
+        self.raw = _Grid()
+        d = (C.c_int32 * 3)(*[int(v) for v in dims])
+        if lib().dsdxo_grid_build(C.byref(box.c()), d, 1 if lift_unsplit else 0,
+                                  C.byref(self.raw)) != 0:
+            raise ValueError("bad process grid")
+        self.ranks = int(self.raw.ranks)
+        self.dims = tuple(int(v) for v in dims)
+        self.plans = (_Plan * self.ranks)()
+        self.recv_rank = np.full(self.ranks * MAX_NEIGHBORS, -1, np.int32)
+        self.ghosts = np.zeros(self.ranks, np.int64)
+
+    def __del__(self):
+        try:
+            for r in range(self.ranks):
+                lib().dsdxo_plan_free(C.byref(self.plans[r]))
+        except Exception:
+            pass
+
+    # --- topology -------------------------------------------------------------
+    def local(self, r: int):
+        b = self.raw.local[r]
+        return tuple(b.lower), tuple(b.upper)
+
+    def coord(self, r: int):
+        return tuple(self.raw.coord[r])
+
+    def neighbor_list(self, r: int):
+        """[(direction, slot, rank, shift)] in list order."""
+        nb = self.raw.nb[r]
+        return [(tuple(nb.direction[n]), int(nb.slot[n]), int(self.raw.nb_rank[r][n]),
+                 tuple(nb.shift[n])) for n in range(nb.count)]
+
+    def cuts(self, a: int):
+        return [self.raw.cuts[a][i] for i in range(self.dims[a] + 1)]
+
+    def owner_rank(self, x, y, z) -> int:
+        return int(lib().dsdxo_grid_owner_rank(C.byref(self.raw), x, y, z))
+
+    # --- operations -------------------------------------------------------------
+    def _parts(self, parts):
+        arr = (_Particles * self.ranks)()
+        for r, p in enumerate(parts):
+            arr[r] = p.c()
+        return arr
+
+    def _fields(self, fields):
+        nf = len(fields[0]) if fields else 0
+        arr = (_Field * max(1, self.ranks * nf))()
+        for r in range(self.ranks if nf else 0):
+            for f in range(nf):
+                arr[r * nf + f] = fields[r][f].c()
+        return arr, nf
+
+    def migrate(self, parts, owned, fields=None):
+        cap = (C.c_int64 * self.ranks)(*[p.capacity for p in parts])
+        own = (C.c_int64 * self.ranks)(*[int(v) for v in owned])
+        sent = (C.c_int64 * self.ranks)()
+        recv = (C.c_int64 * self.ranks)()
+        arr, nf = self._fields(fields)
+        if lib().dsdxo_grid_migrate(C.byref(self.raw), self._parts(parts), cap, own, arr, nf,
+                                    sent, recv) != 0:
+            raise ValueError("oracle migration failed (capacity)")
+        for r in range(self.ranks):
+            owned[r] = int(own[r])
+        return list(sent), list(recv)
+
+    def exchange_halo(self, parts, owned, fields=None):
+        cap = (C.c_int64 * self.ranks)(*[p.capacity for p in parts])
+        own = (C.c_int64 * self.ranks)(*[int(v) for v in owned])
+        g = (C.c_int64 * self.ranks)()
+        arr, nf = self._fields(fields)
+        rc = lib().dsdxo_grid_halo(C.byref(self.raw), self._parts(parts), cap, own, self.plans,
+                                   self.recv_rank.ctypes.data_as(C.POINTER(C.c_int32)), g, arr, nf)
+        self.ghosts[:] = list(g)
+        if rc != 0:
+            raise ValueError("oracle halo failed (capacity)")
+        return [int(v) for v in g]
+
+    def send_counts(self, r: int):
+        nb = self.raw.nb[r]
+        return [int(self.plans[r].send_count[n]) for n in range(nb.count)]
+
+    def reverse_exchange(self, owned, fields):
+        own = (C.c_int64 * self.ranks)(*[int(v) for v in owned])
+        arr, nf = self._fields(fields)
+        if lib().dsdxo_grid_reverse_exchange(C.byref(self.raw), self.plans, own, arr, nf) != 0:
+            raise RuntimeError("oracle reverse exchange failed")
+
+    def reverse_halo(self, parts, owned):
+        fields = [[Field(p.fx), Field(p.fy), Field(p.fz)] for p in parts]
+        self.reverse_exchange(owned, fields)
+
+    def forward_exchange(self, owned, fields):
+        own = (C.c_int64 * self.ranks)(*[int(v) for v in owned])
+        arr, nf = self._fields(fields)
+        lib().dsdxo_grid_forward_exchange(C.byref(self.raw), self.plans,
+                                          self.recv_rank.ctypes.data_as(C.POINTER(C.c_int32)),
+                                          own, arr, nf)
