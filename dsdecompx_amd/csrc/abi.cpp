@@ -128,6 +128,13 @@ int make_box_params(const dsdx_box* box, BoxParams* bp)
     return DSDX_OK;
 }
 
+// Self-loopback routing: every send slot of replica r lands in r itself.
+void route_to_self(ReplicaDesc* d, uint32_t replica)
+{
+    for (int s = 0; s < kSlots; ++s) d->dest[s] = replica;
+    d->self_routed = 1;
+}
+
 int make_field_params(const dsdx_field* fields, uint32_t count, FieldParams* fp)
 {
     std::memset(fp, 0, sizeof(*fp));
@@ -291,6 +298,7 @@ int set_single(Plan* plan, const BoxParams& bp, uint64_t owned, uint64_t capacit
     d.bp = bp;
     d.capacity = capacity;
     d.owned = static_cast<uint32_t>(owned);
+    route_to_self(&d, 0);
     ReplicaDesc before;
     std::memset(&before, 0, sizeof(before));
     if (plan->desc.size() == 1) before = plan->desc[0];
@@ -733,6 +741,7 @@ int dsdx_batch_create(const dsdx_replica* replicas, uint32_t count, void* stream
         d.offset = in.offset;
         d.capacity = in.capacity;
         d.owned = static_cast<uint32_t>(in.owned);
+        route_to_self(&d, static_cast<uint32_t>(r));
         d.ghost_list_offset = ghost_list;
         d.ghost_list_capacity = in.capacity - in.owned;
         ghost_list += d.ghost_list_capacity;

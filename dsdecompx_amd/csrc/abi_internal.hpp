@@ -56,6 +56,8 @@ struct Plan {
 
     DeviceBuffer entries, bpos, counts, prefix, totals, owner_index, owner_code, image_start,
         ghost_list, owner_entry;
+    DeviceBuffer recv;   // process grids only: receive-block bases and the grid status
+    bool grid = false;   // replicas send to each other (dsdx_grid), not to themselves
     uint64_t ghost_list_capacity = 0;   // elements
     uint32_t* host_totals = nullptr;    // pinned copy of totals (R x kTotals)
     uint32_t* host_totals_dev = nullptr;  // the same memory, as the device addresses it
@@ -96,11 +98,13 @@ struct Plan {
                 counts.as<uint32_t>(), prefix.as<uint32_t>(),
                 totals.as<uint32_t>(), owner_index.as<uint32_t>(), owner_code.as<uint8_t>(),
                 image_start.as<uint32_t>(), ghost_list.as<uint32_t>(),
-                owner_entry.as<uint32_t>(), desc.size() == 1 ? host_totals_dev : nullptr};
+                owner_entry.as<uint32_t>(), desc.size() == 1 && !grid ? host_totals_dev : nullptr,
+                grid ? recv.as<uint32_t>() : nullptr};
     }
 };
 
 int make_box_params(const dsdx_box* box, BoxParams* bp);
+void route_to_self(ReplicaDesc* d, uint32_t replica);
 int make_field_params(const dsdx_field* fields, uint32_t count, FieldParams* fp);
 int make_typed_fields(const dsdx_field* fields, uint32_t count, TypedFields* tf);
 int check_particles(const dsdx_particles* p, uint64_t owned);

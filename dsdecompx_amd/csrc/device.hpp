@@ -15,7 +15,8 @@
 //   counts[kCols][C]        per chunk: send count of each of the 26 slots,
 //                           boundary owners, images (column-major)
 //   prefix[kCols][C]        exclusive scan of counts over each replica's chunks
-//   totals[R][kTotals]      per replica: column totals, then the status word
+//   totals[R][kTotals]      per replica: column totals, the status word, then
+//                           the ghosts it receives (kColReceived)
 //   owner_index[C * K]      owner slot first_chunk_r * K + u -> owned index
 //   owner_code[C * K] (u8)  ... -> slab code
 //   image_start[C * K]      ... -> first entry of its ghosts in ghost_list
@@ -39,7 +40,9 @@ constexpr int kCols = 28;          // 26 slots + owners + images
 constexpr int kColOwners = 26;
 constexpr int kColImages = 27;
 constexpr int kStatusWord = kCols; // totals[r][kStatusWord]: overflow bits
+constexpr int kColReceived = 29;   // totals[r][kColReceived]: ghosts the replica receives
 constexpr int kTotals = 32;        // words per replica in totals
+constexpr uint32_t kNoPeer = 0xFFFFFFFFu;
 constexpr uint32_t kOverflowStorage = 1u;
 constexpr uint32_t kOverflowGhostList = 2u;
 
@@ -67,6 +70,12 @@ struct ReplicaDesc {
     uint32_t first_chunk;
     uint32_t chunks;
     uint32_t first_tile;           // first owner tile (kOwnerTile owner slots each)
+    // Replica that receives send slot s (into its receive block 25 - s):
+    // the replica itself on a self-loopback box, the neighbouring subdomain
+    // on a process grid, kNoPeer where the slot has no neighbour.
+    uint32_t dest[kSlots];
+    uint32_t self_routed;          // every dest[s] is the replica itself
+    uint32_t pad;
 };
 
 // Launch shape of the classify / scatter / owner-major kernels: items per
@@ -124,6 +133,10 @@ struct PlanBuffers {
     uint32_t* ghost_list;
     uint32_t* owner_entry;   // owner slot -> its entry index (chunk * K + rank)
     uint32_t* host_totals;   // null, or pinned host memory the scan also writes (single box)
+    // Process grids only (null when every replica sends to itself):
+    // recv[r * kTotals + m] = first ghost of receive block m of replica r,
+    // recv[R * kTotals] = overflow bits of the whole grid.
+    uint32_t* recv;
 };
 
 struct FieldParams {

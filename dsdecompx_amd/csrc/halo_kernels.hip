@@ -329,10 +329,28 @@ __global__ __launch_bounds__(kMaxScanThreads) void scan_kernel(const ReplicaDesc
         __syncthreads();
     }
     if (t == 0) {
-        B.totals[static_cast<uint64_t>(r) * kTotals + col] = carry;
-        if (col == 0) B.totals[static_cast<uint64_t>(r) * kTotals + kStatusWord] = 0u;
-        if (B.host_totals) B.host_totals[static_cast<uint64_t>(r) * kTotals + col] = carry;
+        uint32_t* t = B.totals + static_cast<uint64_t>(r) * kTotals;
+        t[col] = carry;
+        if (col == 0) t[kStatusWord] = 0u;
+        // a replica that sends to itself receives what it sends; a process
+        // grid's exchange pass overwrites this with the neighbours' sends
+        if (col == kColImages) t[kColReceived] = carry;
+        if (B.host_totals) {
+            B.host_totals[static_cast<uint64_t>(r) * kTotals + col] = carry;
+            if (col == kColImages) B.host_totals[static_cast<uint64_t>(r) * kTotals + kColReceived] = carry;
+        }
+        if (B.recv && col == 0 && r == 0) B.recv[static_cast<uint64_t>(L.replicas) * kTotals] = 0u;
     }
+}
+
+// Particle slot of ghost `rel` sent by replica d through slot s: ghosts
+// follow the receiving replica's owned particles.
+__device__ __forceinline__ uint64_t ghost_slot(const ReplicaDesc* __restrict__ desc,
+                                               const ReplicaDesc* d, uint32_t s, uint32_t rel)
+{
+    if (d->self_routed) return d->offset + d->owned + rel;
+    const ReplicaDesc* to = desc + d->dest[s];
+    return to->offset + to->owned + rel;
 }
 
 // ---------------------------------------------------------------------------
@@ -397,22 +415,34 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
     const uint32_t lut = d->bp.lut[lane];
 
     const uint32_t owned = d->owned;
-    const uint32_t ghosts = uniform(totals[kColImages]);
+    const uint32_t received = uniform(totals[kColReceived]);
+    const uint32_t images = uniform(totals[kColImages]);
     uint32_t overflow = 0;
-    if (static_cast<uint64_t>(owned) + ghosts > d->capacity) overflow |= kOverflowStorage;
-    if (ghosts > d->ghost_list_capacity) overflow |= kOverflowGhostList;
+    if (static_cast<uint64_t>(owned) + received > d->capacity) overflow |= kOverflowStorage;
+    if (images > d->ghost_list_capacity) overflow |= kOverflowGhostList;
+    // a process grid routes nothing when any of its subdomains overflowed
+    if (B.recv) overflow |= uniform(B.recv[static_cast<uint64_t>(L.replicas) * kTotals]);
     if (overflow) {
         if (chunk == d->first_chunk && lane == 0) totals[kStatusWord] = overflow;
         return;
     }
 
-    // Receive block m holds send slot 25-m: recv_base[m] = sum of totals of
-    // slots 25..26-m.  Lane s ends with slot_base = recv_base[25-s] + prefix.
-    const uint32_t block_size = lane < kSlots ? totals[kSlots - 1 - lane] : 0u;
-    const uint32_t recv_base = wave_inclusive_sum(block_size) - block_size;
-    const uint32_t mirrored = __shfl(recv_base, lane < kSlots ? kSlots - 1 - lane : 0, kWave);
+    // Lane s ends with slot_base = first ghost, in the receiving replica, of
+    // this chunk's sends through slot s.  On a self-loopback box receive
+    // block m holds send slot 25-m: recv_base[m] = sum of totals of slots
+    // 25..26-m.  On a process grid the exchange pass tabulated the receive
+    // blocks of every replica.
+    uint32_t slot_first = 0;
+    if (B.recv) {
+        const uint32_t peer = lane < kSlots ? d->dest[lane] : kNoPeer;
+        slot_first = peer != kNoPeer ? B.recv[static_cast<uint64_t>(peer) * kTotals + kSlots - 1 - lane] : 0u;
+    } else {
+        const uint32_t block_size = lane < kSlots ? totals[kSlots - 1 - lane] : 0u;
+        const uint32_t recv_base = wave_inclusive_sum(block_size) - block_size;
+        slot_first = __shfl(recv_base, lane < kSlots ? kSlots - 1 - lane : 0, kWave);
+    }
     const uint32_t slot_base =
-        lane < kSlots ? mirrored + B.prefix[static_cast<uint64_t>(lane) * L.chunks + chunk] : 0u;
+        lane < kSlots ? slot_first + B.prefix[static_cast<uint64_t>(lane) * L.chunks + chunk] : 0u;
     uint32_t slot_run = 0;   // lane s: ghosts of slot s already routed from this chunk
 
     const uint32_t n_boundary = uniform(B.counts[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
@@ -485,11 +515,10 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
     const double m = p.mass[i], q = p.charge[i];
     const int32_t ty = p.type[i], id = p.global_id[i];
     const uint32_t* glist = B.ghost_list + d->ghost_list_offset;
-    const uint64_t goff = off + d->owned;
     uint32_t k = 0;
     for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k) {
         const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
-        const uint64_t j = goff + glist[first + k];
+        const uint64_t j = ghost_slot(desc, d, s, glist[first + k]);
         p.x[j] = px + d->bp.shift[s][0];
         p.y[j] = py + d->bp.shift[s][1];
         p.z[j] = pz + d->bp.shift[s][2];
@@ -514,14 +543,12 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_forces_kernel(
 {
     const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
     if (!o.live) return;
-    const uint64_t off = o.d->offset;
-    const uint64_t goff = off + o.d->owned;
     const uint32_t* glist = B.ghost_list + o.d->ghost_list_offset + B.image_start[o.u];
-    const uint64_t i = off + B.owner_index[o.u];
-    const uint32_t n_img = static_cast<uint32_t>(__popc(o.d->bp.lut[B.owner_code[o.u]]));
+    const uint64_t i = o.d->offset + B.owner_index[o.u];
+    uint32_t slots = o.d->bp.lut[B.owner_code[o.u]];
     double ax = fx[i], ay = fy[i], az = fz[i];
-    for (uint32_t k = 0; k < n_img; ++k) {
-        const uint64_t j = goff + glist[k];
+    for (uint32_t k = 0; slots; ++k, slots &= slots - 1u) {
+        const uint64_t j = ghost_slot(desc, o.d, static_cast<uint32_t>(__builtin_ctz(slots)), glist[k]);
         ax = ax + fx[j];
         ay = ay + fy[j];
         az = az + fz[j];
@@ -540,13 +567,32 @@ __device__ __forceinline__ T reduce_value(T dst, T v, int32_t op)
     }
 }
 
+// The ghosts of one owner: its slot set and its ghost-list entries (one per
+// set bit, ascending slot).  at(k, s) is the particle slot of the k-th ghost,
+// sent through slot s.
+struct OwnerGhosts {
+    const ReplicaDesc* desc;
+    const ReplicaDesc* d;
+    const uint32_t* list;
+    uint32_t slots;
+    __device__ uint64_t at(uint32_t k, uint32_t s) const { return ghost_slot(desc, d, s, list[k]); }
+};
+
+__device__ __forceinline__ OwnerGhosts ghosts_of(const ReplicaDesc* __restrict__ desc,
+                                                 const OwnerLane& o, const PlanBuffers& B)
+{
+    return {desc, o.d, B.ghost_list + o.d->ghost_list_offset + B.image_start[o.u],
+            o.d->bp.lut[B.owner_code[o.u]]};
+}
+
 template <class T>
-__device__ void reduce_field(T* data, uint32_t c, int32_t op, uint64_t i, uint64_t goff,
-                             const uint32_t* ghosts, uint32_t n_img)
+__device__ void reduce_field(T* data, uint32_t c, int32_t op, uint64_t i, const OwnerGhosts& g)
 {
     for (uint32_t e = 0; e < c; ++e) {
         T acc = data[i * c + e];
-        for (uint32_t k = 0; k < n_img; ++k) acc = reduce_value<T>(acc, data[(goff + ghosts[k]) * c + e], op);
+        uint32_t k = 0;
+        for (uint32_t rest = g.slots; rest; rest &= rest - 1u, ++k)
+            acc = reduce_value<T>(acc, data[g.at(k, static_cast<uint32_t>(__builtin_ctz(rest))) * c + e], op);
         data[i * c + e] = acc;
     }
 }
@@ -560,38 +606,35 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_fields_kernel(
 {
     const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
     if (!o.live) return;
-    const uint64_t goff = o.d->offset + o.d->owned;
-    const uint64_t u = o.u;
-    const uint64_t i = o.d->offset + B.owner_index[u];
-    const uint32_t* ghosts = B.ghost_list + o.d->ghost_list_offset + B.image_start[u];
-    const uint32_t n_img = static_cast<uint32_t>(__popc(o.d->bp.lut[B.owner_code[u]]));
+    const uint64_t i = o.d->offset + B.owner_index[o.u];
+    const OwnerGhosts ghosts = ghosts_of(desc, o, B);
     for (uint32_t f = 0; f < tf.count; ++f) {
         const uint32_t cc = tf.components[f];
         const int32_t op = tf.op[f];
         switch (tf.type[f]) {
         case DSDX_INT32:
             if (op == DSDX_SUM)
-                reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+                reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, ghosts);
             else
-                reduce_field<int32_t>(static_cast<int32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+                reduce_field<int32_t>(static_cast<int32_t*>(tf.data[f]), cc, op, i, ghosts);
             break;
         case DSDX_INT64:
             if (op == DSDX_SUM)
-                reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+                reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, ghosts);
             else
-                reduce_field<int64_t>(static_cast<int64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+                reduce_field<int64_t>(static_cast<int64_t*>(tf.data[f]), cc, op, i, ghosts);
             break;
         case DSDX_UINT32:
-            reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+            reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, ghosts);
             break;
         case DSDX_UINT64:
-            reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+            reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, ghosts);
             break;
         case DSDX_FLOAT32:
-            reduce_field<float>(static_cast<float*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+            reduce_field<float>(static_cast<float*>(tf.data[f]), cc, op, i, ghosts);
             break;
         default:
-            reduce_field<double>(static_cast<double*>(tf.data[f]), cc, op, i, goff, ghosts, n_img);
+            reduce_field<double>(static_cast<double*>(tf.data[f]), cc, op, i, ghosts);
             break;
         }
     }
@@ -605,17 +648,16 @@ __global__ __launch_bounds__(WAVES * kWave) void forward_fields_kernel(
 {
     const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
     if (!o.live) return;
-    const uint64_t goff = o.d->offset + o.d->owned;
-    const uint64_t u = o.u;
-    const uint64_t i = o.d->offset + B.owner_index[u];
-    const uint32_t* ghosts = B.ghost_list + o.d->ghost_list_offset + B.image_start[u];
-    const uint32_t n_img = static_cast<uint32_t>(__popc(o.d->bp.lut[B.owner_code[u]]));
+    const uint64_t i = o.d->offset + B.owner_index[o.u];
+    const OwnerGhosts ghosts = ghosts_of(desc, o, B);
     for (uint32_t f = 0; f < fp.count; ++f) {
         const uint32_t w = fp.words[f];
         uint32_t* dd = static_cast<uint32_t*>(fp.data[f]);
-        for (uint32_t k = 0; k < w; ++k) {
-            const uint32_t value = dd[i * w + k];
-            for (uint32_t g = 0; g < n_img; ++g) dd[(goff + ghosts[g]) * w + k] = value;
+        for (uint32_t c = 0; c < w; ++c) {
+            const uint32_t value = dd[i * w + c];
+            uint32_t k = 0;
+            for (uint32_t rest = ghosts.slots; rest; rest &= rest - 1u, ++k)
+                dd[ghosts.at(k, static_cast<uint32_t>(__builtin_ctz(rest))) * w + c] = value;
         }
     }
 }
@@ -630,16 +672,14 @@ __global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(
     const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
     if (!o.live) return;
     const ReplicaDesc* d = o.d;
-    const uint64_t goff = d->offset + d->owned;
-    const uint64_t u = o.u;
-    const uint64_t i = d->offset + B.owner_index[u];
-    const uint32_t* ghosts = B.ghost_list + d->ghost_list_offset + B.image_start[u];
-    uint32_t slots = d->bp.lut[B.owner_code[u]];
+    const uint64_t i = d->offset + B.owner_index[o.u];
+    const OwnerGhosts ghosts = ghosts_of(desc, o, B);
+    uint32_t slots = ghosts.slots;
     const double px = p.x[i], py = p.y[i], pz = p.z[i];
     for (uint32_t k = 0; slots; ++k) {
         const uint32_t s = static_cast<uint32_t>(__builtin_ctz(slots));
         slots &= slots - 1u;
-        const uint64_t j = goff + ghosts[k];
+        const uint64_t j = ghosts.at(k, s);
         p.x[j] = px + d->bp.shift[s][0];
         p.y[j] = py + d->bp.shift[s][1];
         p.z[j] = pz + d->bp.shift[s][2];
@@ -655,7 +695,8 @@ __global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(
     }
 }
 
-// Owner of every ghost: out[offset_r + rel] = owned index (replica-local).
+// Owner of every ghost: out[offset_q + rel] = owned index within the
+// sending replica, for ghost rel of receiving replica q.
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void ghost_owners_kernel(
     int32_t* __restrict__ out, const ReplicaDesc* __restrict__ desc,
@@ -664,9 +705,14 @@ __global__ __launch_bounds__(WAVES * kWave) void ghost_owners_kernel(
     const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
     if (!o.live) return;
     const uint32_t i = B.owner_index[o.u];
-    const uint32_t* ghosts = B.ghost_list + o.d->ghost_list_offset + B.image_start[o.u];
-    const uint32_t n_img = static_cast<uint32_t>(__popc(o.d->bp.lut[B.owner_code[o.u]]));
-    for (uint32_t k = 0; k < n_img; ++k) out[o.d->offset + ghosts[k]] = static_cast<int32_t>(i);
+    const OwnerGhosts ghosts = ghosts_of(desc, o, B);
+    uint32_t k = 0;
+    for (uint32_t rest = ghosts.slots; rest; rest &= rest - 1u, ++k) {
+        // ghost slot relative to the receiving replica's first owned slot
+        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
+        const ReplicaDesc* to = desc + o.d->dest[s];
+        out[ghosts.at(k, s) - to->owned] = static_cast<int32_t>(i);
+    }
 }
 
 // ---------------------------------------------------------------------------
