@@ -108,6 +108,13 @@ def lib():
         "dsdx_batch_ghost_owners": ([P, P, P], C.c_int),
         "dsdx_batch_counts": ([P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), P], C.c_int),
         "dsdx_batch_device_totals": ([P], C.c_void_p),
+        "dsdx_grid_create": ([C.POINTER(Box), C.POINTER(i32), u64, P, C.POINTER(P)], C.c_int),
+        "dsdx_grid_destroy": ([P], C.c_int),
+        "dsdx_grid_set_owned": ([P, C.POINTER(u64), P], C.c_int),
+        "dsdx_grid_counts": ([P, C.POINTER(u64), C.POINTER(u64)], C.c_int),
+        "dsdx_grid_migrate": ([P, C.POINTER(Particles), C.POINTER(Particles), P], C.c_int),
+        "dsdx_grid_halo_build": ([P, C.POINTER(Particles), P, C.POINTER(u64)], C.c_int),
+        "dsdx_grid_reverse_halo": ([P, P, P, P, P], C.c_int),
         "dsdx_timing_enable": ([i32], C.c_int),
         "dsdx_timing_reset": ([], C.c_int),
         "dsdx_timing_read": ([C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)], C.c_int),
@@ -133,7 +140,9 @@ def exported_symbols():
             "dsdx_batch_reverse_halo", "dsdx_batch_reverse_exchange",
             "dsdx_batch_forward_exchange", "dsdx_batch_ghost_owners", "dsdx_batch_counts",
             "dsdx_batch_device_totals", "dsdx_migrate_halo_build",
-            "dsdx_batch_migrate_halo_build"]
+            "dsdx_batch_migrate_halo_build", "dsdx_grid_create", "dsdx_grid_destroy",
+            "dsdx_grid_set_owned", "dsdx_grid_counts", "dsdx_grid_migrate",
+            "dsdx_grid_halo_build", "dsdx_grid_reverse_halo"]
 
 
 KERNELS = ("migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "halo_copy",

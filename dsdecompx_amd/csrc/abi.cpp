@@ -251,8 +251,12 @@ void number_tiles(Plan* plan)
     plan->tile_replica.clear();
     for (size_t r = 0; r < plan->desc.size(); ++r) {
         ReplicaDesc& d = plan->desc[r];
-        const uint64_t bound = std::min<uint64_t>(
-            {static_cast<uint64_t>(d.owned), d.capacity - d.owned, d.ghost_list_capacity});
+        // a self-routed replica's ghosts are its own sends, so its storage
+        // bounds them too; on a process grid only the ghost list does
+        const uint64_t bound =
+            plan->grid ? std::min<uint64_t>(d.owned, d.ghost_list_capacity)
+                       : std::min<uint64_t>({static_cast<uint64_t>(d.owned), d.capacity - d.owned,
+                                             d.ghost_list_capacity});
         const uint32_t tiles = static_cast<uint32_t>((bound + kOwnerTile - 1) / kOwnerTile);
         d.first_tile = next;
         next += tiles;

@@ -139,6 +139,22 @@ struct PlanBuffers {
     uint32_t* recv;
 };
 
+// Migration between the subdomains of a process grid (grid_kernels.hip).
+constexpr int kMaxRanks = 64;   // subdomains of one grid (one per lane in the rank tables)
+
+struct MigrateParams {
+    double cuts[3][kMaxRanks + 1];  // subdomain boundaries per axis (dims + 1 each)
+    int32_t dims[3];
+    uint32_t ranks;
+    uint8_t* dest;        // per chunk slot: destination rank of the owned particle
+    uint32_t* counts;     // [ranks][C]: per chunk, particles going to each rank
+    uint32_t* prefix;     // [ranks][C]: exclusive scan over each source replica's chunks
+    uint32_t* totals;     // [R][kMaxRanks]: particles replica r sends to each rank
+    uint32_t* base;       // [ranks][kMaxRanks]: first new slot in rank q of those from r
+    uint32_t* new_owned;  // [ranks] device, and the same in pinned host memory
+    uint32_t* new_owned_host;
+};
+
 struct FieldParams {
     uint32_t count;
     uint32_t words[DSDX_MAX_FIELDS];   // 32-bit words per particle

@@ -29,4 +29,15 @@ hipError_t launch_ghost_owners(int32_t* out, const Layout& L, const PlanBuffers&
 hipError_t launch_migrate_wrap(double* x, double* y, double* z, const Layout& L, const Shape& s,
                                hipStream_t stream);
 
+// Process grids (grid_kernels.hip).
+hipError_t launch_exchange_counts(const Layout& L, const PlanBuffers& B, hipStream_t stream);
+hipError_t launch_migrate_count(const dsdx_particles& p, const Layout& L, const MigrateParams& M,
+                                const Shape& s, hipStream_t stream);
+hipError_t launch_migrate_scan(const Layout& L, const MigrateParams& M, const Shape& s,
+                               hipStream_t stream);
+hipError_t launch_migrate_bases(const MigrateParams& M, hipStream_t stream);
+hipError_t launch_migrate_scatter(const dsdx_particles& from, const dsdx_particles& to,
+                                  const Layout& L, const MigrateParams& M, const Shape& s,
+                                  hipStream_t stream);
+
 }  // namespace dsdx

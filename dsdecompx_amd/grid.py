@@ -1,0 +1,93 @@
+"""Process grids on one GPU: the subdomains of a Cartesian decomposition of
+one box, each handled as a replica of the halo kernels, with the semantics
+of the reference's Host path on that many MPI ranks (see ``dsdx_grid_*`` in
+include/dsdx_hip.h).
+
+Storage: two sets of SoA device arrays (``DeviceParticles``) of
+``ranks * capacity_per_rank`` slots each; subdomain r owns slots
+[r * capacity_per_rank, (r + 1) * capacity_per_rank), owned particles first,
+then its ghosts.  ``migrate`` writes the migrated particles into the spare
+set and makes it current.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _abi
+from .halo import BoxConfig, DeviceParticles
+
+
+class SubdomainGrid:
+    def __init__(self, box: BoxConfig, dims, capacity_per_rank: int, device="cuda", stream=None):
+        self.box = box
+        self.dims = tuple(int(d) for d in dims)
+        self.ranks = self.dims[0] * self.dims[1] * self.dims[2]
+        self.capacity = int(capacity_per_rank)
+        self.device = torch.device(device)
+        self._storage = [DeviceParticles(self.ranks * self.capacity, self.device) for _ in range(2)]
+        self._current = 0
+        self._box = box.c()
+        self._grid = C.c_void_p()
+        d = (C.c_int32 * 3)(*self.dims)
+        _abi.check(_abi.lib().dsdx_grid_create(C.byref(self._box), d, self.capacity,
+                                               self._stream(stream), C.byref(self._grid)),
+                   "grid create")
+
+    @staticmethod
+    def _stream(stream):
+        return (stream or torch.cuda.current_stream()).cuda_stream
+
+    def close(self):
+        if self._grid:
+            _abi.lib().dsdx_grid_destroy(self._grid)
+            self._grid = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def particles(self) -> DeviceParticles:
+        """The current storage."""
+        return self._storage[self._current]
+
+    def slots(self, r: int, count: int) -> slice:
+        return slice(r * self.capacity, r * self.capacity + count)
+
+    def set_owned(self, owned, stream=None) -> None:
+        arr = (C.c_uint64 * self.ranks)(*[int(v) for v in owned])
+        _abi.check(_abi.lib().dsdx_grid_set_owned(self._grid, arr, self._stream(stream)),
+                   "grid set owned")
+
+    def counts(self):
+        """(owned, ghosts) of every subdomain (ghosts of the last halo build)."""
+        o, g = (C.c_uint64 * self.ranks)(), (C.c_uint64 * self.ranks)()
+        _abi.check(_abi.lib().dsdx_grid_counts(self._grid, o, g))
+        return list(o), list(g)
+
+    def migrate(self, stream=None) -> list:
+        """Migration on every subdomain; returns the new owned counts."""
+        src = self._storage[self._current].c()
+        dst = self._storage[1 - self._current].c()
+        _abi.check(_abi.lib().dsdx_grid_migrate(self._grid, C.byref(src), C.byref(dst),
+                                                self._stream(stream)), "grid migrate")
+        self._current = 1 - self._current
+        return self.counts()[0]
+
+    def exchange_halo(self, stream=None) -> list:
+        """Halo rebuild on every subdomain; returns the ghost counts."""
+        p = self.particles.c()
+        g = (C.c_uint64 * self.ranks)()
+        _abi.check(_abi.lib().dsdx_grid_halo_build(self._grid, C.byref(p), self._stream(stream), g),
+                   "grid halo")
+        return list(g)
+
+    def reverse_halo(self, stream=None) -> None:
+        p = self.particles
+        _abi.check(_abi.lib().dsdx_grid_reverse_halo(self._grid, p.fx.data_ptr(), p.fy.data_ptr(),
+                                                     p.fz.data_ptr(), self._stream(stream)),
+                   "grid reverse halo")

@@ -268,6 +268,45 @@ int dsdx_batch_counts(const dsdx_batch* batch, uint32_t* ghosts,
  * kernels that must know a replica's ghost count without a host wait. */
 const uint32_t* dsdx_batch_device_totals(const dsdx_batch* batch);
 
+/* ---- Process grids -------------------------------------------------------
+ * The dims[0] x dims[1] x dims[2] Cartesian decomposition of one box, every
+ * subdomain ("rank", row-major as MPI_Cart_rank without reordering) held on
+ * this GPU, with the semantics of the reference's Host path on that many MPI
+ * ranks: cuts lower + extent * i / dims, neighbour lists with the
+ * unsplit-axis skip (src/topology.cpp:57-176), halo slab tests against each
+ * subdomain (src/halo.cpp:164-238), receive blocks filled by the neighbour in
+ * each direction (src/backends/mpi_backend.cpp:344-433), reverse sums in
+ * ascending send slot (src/halo.cpp:1288-1532) and migration to owner_rank,
+ * retained particles first, then those of ranks 0..P-1
+ * (src/migration.cpp:74-128, :391-591).  No reference counterpart as one
+ * object: it stands for P processes of the reference.
+ *
+ * Storage is the caller's: every particle array holds subdomain r in slots
+ * [r * capacity_per_rank, (r + 1) * capacity_per_rank), owned first, then its
+ * ghosts.  All 13 arrays are required.  Registered fields are not carried by
+ * grids.  At most 64 subdomains. */
+typedef struct dsdx_grid dsdx_grid;
+
+int dsdx_grid_create(const dsdx_box* box, const int32_t* dims, uint64_t capacity_per_rank,
+                     void* stream, dsdx_grid** grid);
+int dsdx_grid_destroy(dsdx_grid* grid);
+/* Owned counts of every subdomain (after the caller filled the arrays). */
+int dsdx_grid_set_owned(dsdx_grid* grid, const uint64_t* owned, void* stream);
+/* Host copies: owned counts, and the ghost counts of the last halo build. */
+int dsdx_grid_counts(const dsdx_grid* grid, uint64_t* owned, uint64_t* ghosts);
+/* Migrates from `from` into the separate storage `to` (same layout); waits
+ * for the new counts.  DSDX_ECAPACITY when a subdomain would overflow
+ * (nothing is written then). */
+int dsdx_grid_migrate(dsdx_grid* grid, const dsdx_particles* from, const dsdx_particles* to,
+                      void* stream);
+/* Rebuilds every subdomain's halo; waits for the ghost counts.
+ * DSDX_ECAPACITY when a subdomain cannot hold its ghosts (no ghost is
+ * written then). */
+int dsdx_grid_halo_build(dsdx_grid* grid, const dsdx_particles* particles, void* stream,
+                         uint64_t* ghosts);
+/* Ghost forces of every subdomain added to their owners. */
+int dsdx_grid_reverse_halo(dsdx_grid* grid, double* fx, double* fy, double* fz, void* stream);
+
 /* Per-kernel device timing, process-wide: when enabled, every kernel
  * dispatch carries its own start / stop HIP events (a timed dispatch starts a
  * few microseconds later than an untimed one).  dsdx_timing_read waits for
