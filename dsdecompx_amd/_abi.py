@@ -112,9 +112,14 @@ def lib():
         "dsdx_grid_destroy": ([P], C.c_int),
         "dsdx_grid_set_owned": ([P, C.POINTER(u64), P], C.c_int),
         "dsdx_grid_counts": ([P, C.POINTER(u64), C.POINTER(u64)], C.c_int),
-        "dsdx_grid_migrate": ([P, C.POINTER(Particles), C.POINTER(Particles), P], C.c_int),
-        "dsdx_grid_halo_build": ([P, C.POINTER(Particles), P, C.POINTER(u64)], C.c_int),
+        "dsdx_grid_migrate": ([P, C.POINTER(Particles), C.POINTER(Particles), C.POINTER(Field),
+                               C.POINTER(Field), u32, P], C.c_int),
+        "dsdx_grid_halo_build": ([P, C.POINTER(Particles), C.POINTER(Field), u32, P,
+                                  C.POINTER(u64)], C.c_int),
         "dsdx_grid_reverse_halo": ([P, P, P, P, P], C.c_int),
+        "dsdx_grid_reverse_exchange": ([P, C.POINTER(Field), u32, P], C.c_int),
+        "dsdx_grid_forward_exchange": ([P, C.POINTER(Field), u32, P], C.c_int),
+        "dsdx_grid_refresh": ([P, C.POINTER(Particles), C.POINTER(Field), u32, i32, P], C.c_int),
         "dsdx_timing_enable": ([i32], C.c_int),
         "dsdx_timing_reset": ([], C.c_int),
         "dsdx_timing_read": ([C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)], C.c_int),
@@ -142,7 +147,8 @@ def exported_symbols():
             "dsdx_batch_device_totals", "dsdx_migrate_halo_build",
             "dsdx_batch_migrate_halo_build", "dsdx_grid_create", "dsdx_grid_destroy",
             "dsdx_grid_set_owned", "dsdx_grid_counts", "dsdx_grid_migrate",
-            "dsdx_grid_halo_build", "dsdx_grid_reverse_halo"]
+            "dsdx_grid_halo_build", "dsdx_grid_reverse_halo", "dsdx_grid_reverse_exchange",
+            "dsdx_grid_forward_exchange", "dsdx_grid_refresh"]
 
 
 KERNELS = ("migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "halo_copy",

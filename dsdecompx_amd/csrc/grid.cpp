@@ -221,16 +221,27 @@ int dsdx_grid_counts(const dsdx_grid* grid, uint64_t* owned, uint64_t* ghosts)
 
 // src/migration.cpp:391-591 on every subdomain: `from` holds the current
 // particles (subdomain r in slots [r * capacity, r * capacity + owned_r)),
-// `to` receives the migrated ones in the same layout.  Waits for the new
-// counts; on DSDX_ECAPACITY nothing is written to `to`.
+// `to` receives the migrated ones in the same layout, and each registered
+// field travels from fields_from[f] to fields_to[f] with its particle.  Waits
+// for the new counts; on DSDX_ECAPACITY nothing is written to `to`.
 int dsdx_grid_migrate(dsdx_grid* grid, const dsdx_particles* from, const dsdx_particles* to,
-                      void* stream)
+                      const dsdx_field* fields_from, const dsdx_field* fields_to,
+                      uint32_t field_count, void* stream)
 {
     if (!grid) return fail(DSDX_EINVAL, "null grid");
     if (!stream) return fail(DSDX_EINVAL, "device migration requires an explicit stream");
     if (int rc = check_full(from)) return rc;
     if (int rc = check_full(to)) return rc;
     if (from->x == to->x) return fail(DSDX_EINVAL, "migration needs separate source and target storage");
+    FieldParams ff, ft;
+    if (int rc = make_field_params(fields_from, field_count, &ff)) return rc;
+    if (int rc = make_field_params(fields_to, field_count, &ft)) return rc;
+    for (uint32_t f = 0; f < field_count; ++f) {
+        if (ff.words[f] != ft.words[f] || fields_from[f].type != fields_to[f].type)
+            return fail(DSDX_EINVAL, "source and target fields differ in type or size");
+        if (ff.data[f] == ft.data[f])
+            return fail(DSDX_EINVAL, "migration needs separate source and target field storage");
+    }
     const hipStream_t s = as_stream(stream);
     const Layout L = grid->layout();
     const uint64_t slots = static_cast<uint64_t>(std::max<uint32_t>(1, grid->chunks)) * grid->chunk_size();
@@ -260,7 +271,7 @@ int dsdx_grid_migrate(dsdx_grid* grid, const dsdx_particles* from, const dsdx_pa
     M.new_owned = grid->mig_new_owned.as<uint32_t>();
     M.new_owned_host = grid->new_owned_host_dev;
     {
-        timing::Scope timed(timing::kMigrate, s);
+        timing::Scope timed(timing::kGridMigrate, s);
         DSDX_HIP(launch_migrate_count(*from, L, M, grid->shape, s), "launch migrate count");
     }
     DSDX_HIP(launch_migrate_scan(L, M, grid->shape, s), "launch migrate scan");
@@ -272,7 +283,10 @@ int dsdx_grid_migrate(dsdx_grid* grid, const dsdx_particles* from, const dsdx_pa
         if (owned[q] > grid->rank_capacity)
             return fail(DSDX_ECAPACITY, "a subdomain cannot hold its particles after migration");
     }
-    DSDX_HIP(launch_migrate_scatter(*from, *to, L, M, grid->shape, s), "launch migrate scatter");
+    {
+        timing::Scope timed(timing::kGridScatter, s);
+        DSDX_HIP(launch_migrate_scatter(*from, *to, ff, ft, L, M, grid->shape, s), "launch migrate scatter");
+    }
     for (uint32_t q = 0; q < grid->ranks; ++q) grid->desc[q].owned = owned[q];
     return relayout(grid, s);
 }
@@ -280,19 +294,23 @@ int dsdx_grid_migrate(dsdx_grid* grid, const dsdx_particles* from, const dsdx_pa
 // src/halo.cpp:164-238 on every subdomain and the exchange of
 // src/backends/mpi_backend.cpp:344-433: subdomain q's ghosts follow its owned
 // particles, receive block m holding what its neighbour in direction m sent
-// through the opposite slot.  Waits for the ghost counts (ghosts[r]).
-int dsdx_grid_halo_build(dsdx_grid* grid, const dsdx_particles* particles, void* stream,
-                         uint64_t* ghosts)
+// through the opposite slot; registered fields are copied into the ghosts
+// with their records.  Waits for the ghost counts (ghosts[r]).
+int dsdx_grid_halo_build(dsdx_grid* grid, const dsdx_particles* particles, const dsdx_field* fields,
+                         uint32_t field_count, void* stream, uint64_t* ghosts)
 {
     if (!grid) return fail(DSDX_EINVAL, "null grid");
     if (!stream) return fail(DSDX_EINVAL, "device halo requires an explicit stream");
     if (int rc = check_full(particles)) return rc;
+    FieldParams fp;
+    if (int rc = make_field_params(fields, field_count, &fp)) return rc;
     const hipStream_t s = as_stream(stream);
     grid->classified = grid->built = grid->totals_cached = false;
     if (int rc = run_classify(grid, particles, false, s)) return rc;
-    DSDX_HIP(launch_exchange_counts(grid->layout(), grid->buffers(), s), "launch exchange counts");
-    FieldParams fp;
-    std::memset(&fp, 0, sizeof(fp));
+    {
+        timing::Scope timed(timing::kGridExchange, s);
+        DSDX_HIP(launch_exchange_counts(grid->layout(), grid->buffers(), s), "launch exchange counts");
+    }
     if (int rc = run_scatter(grid, particles, fp, s)) return rc;
     if (int rc = read_totals(grid, s)) return rc;
     for (uint32_t r = 0; r < grid->ranks; ++r)
@@ -316,6 +334,65 @@ int dsdx_grid_reverse_halo(dsdx_grid* grid, double* fx, double* fy, double* fz, 
     timing::Scope timed(timing::kReverse, s);
     DSDX_HIP(launch_reverse_forces(fx, fy, fz, grid->layout(), grid->buffers(), grid->shape, s),
              "launch reverse halo");
+    return DSDX_OK;
+}
+
+// src/halo.cpp:1396-1532 on every subdomain: Sum / Min / Max / Replace of the
+// ghosts' values into their owners, in ascending send slot.
+int dsdx_grid_reverse_exchange(dsdx_grid* grid, const dsdx_field* fields, uint32_t field_count,
+                               void* stream)
+{
+    if (!grid) return fail(DSDX_EINVAL, "null grid");
+    if (!stream) return fail(DSDX_EINVAL, "device reverse exchange requires an explicit stream");
+    TypedFields tf;
+    if (int rc = make_typed_fields(fields, field_count, &tf)) return rc;
+    if (!grid->built) return fail(DSDX_ESTATE, "no halo plan on the grid");
+    if (field_count == 0) return DSDX_OK;
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kReverseFields, s);
+    DSDX_HIP(launch_reverse_fields(tf, grid->layout(), grid->buffers(), grid->shape, s),
+             "launch reverse exchange");
+    return DSDX_OK;
+}
+
+// src/halo.cpp:1084-1207 on every subdomain: owner values into every ghost
+// that copies the owner, in the neighbouring subdomains.
+int dsdx_grid_forward_exchange(dsdx_grid* grid, const dsdx_field* fields, uint32_t field_count,
+                               void* stream)
+{
+    if (!grid) return fail(DSDX_EINVAL, "null grid");
+    if (!stream) return fail(DSDX_EINVAL, "device forward exchange requires an explicit stream");
+    FieldParams fp;
+    if (int rc = make_field_params(fields, field_count, &fp)) return rc;
+    if (!grid->built) return fail(DSDX_ESTATE, "no halo plan on the grid");
+    if (field_count == 0) return DSDX_OK;
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kForward, s);
+    DSDX_HIP(launch_forward_fields(fp, grid->layout(), grid->buffers(), grid->shape, s),
+             "launch forward exchange");
+    return DSDX_OK;
+}
+
+// Reused-plan refresh on every subdomain (launch_halo_gather /
+// launch_halo_gather_positions + launch_unpack_positions,
+// src/cuda/kernels.hpp:103-128): every ghost re-reads its owner along the
+// last build's routes with the build's periodic shifts; positions_only
+// rewrites x, y, z and zeroes the forces, otherwise the whole record and the
+// registered fields.
+int dsdx_grid_refresh(dsdx_grid* grid, const dsdx_particles* particles, const dsdx_field* fields,
+                      uint32_t field_count, int32_t positions_only, void* stream)
+{
+    if (!grid) return fail(DSDX_EINVAL, "null grid");
+    if (!stream) return fail(DSDX_EINVAL, "device halo requires an explicit stream");
+    if (int rc = check_full(particles)) return rc;
+    FieldParams fp;
+    if (int rc = make_field_params(fields, field_count, &fp)) return rc;
+    if (!grid->built) return fail(DSDX_ESTATE, "no halo plan on the grid");
+    const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kRefresh, s);
+    DSDX_HIP(launch_refresh(*particles, fp, positions_only ? 1 : 0, grid->layout(), grid->buffers(),
+                            grid->shape, s),
+             "launch halo refresh");
     return DSDX_OK;
 }
 

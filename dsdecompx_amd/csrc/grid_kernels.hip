@@ -152,11 +152,13 @@ __global__ __launch_bounds__(kWave) void migrate_bases_kernel(MigrateParams M)
 // Pass 4: one wave per chunk of a source replica r.  Each owned particle goes
 // to slot base[q][r] + prefix[q][chunk] + its rank among the chunk's
 // particles bound for q (owned order) of rank q's new arrays, as its wrapped
-// record (wrapped_record: position wrapped, every other member copied).
+// record (wrapped_record: position wrapped, every other member copied) with
+// the words of every registered field (from `ff` into `ft`).
 template <int ITEMS, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void migrate_scatter_kernel(
-    dsdx_particles src, dsdx_particles dst, const ReplicaDesc* __restrict__ desc,
-    const uint32_t* __restrict__ chunk_replica, Layout L, MigrateParams M)
+    dsdx_particles src, dsdx_particles dst, FieldParams ff, FieldParams ft,
+    const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ chunk_replica, Layout L,
+    MigrateParams M)
 {
     constexpr uint32_t K = ITEMS * kWave;
     __shared__ uint32_t s_run[WAVES][kMaxRanks];
@@ -202,6 +204,12 @@ __global__ __launch_bounds__(WAVES * kWave) void migrate_scatter_kernel(
                 dst.fx[j] = src.fx[i]; dst.fy[j] = src.fy[i]; dst.fz[j] = src.fz[i];
                 dst.mass[j] = src.mass[i]; dst.charge[j] = src.charge[i];
                 dst.type[j] = src.type[i]; dst.global_id[j] = src.global_id[i];
+                for (uint32_t f = 0; f < ff.count; ++f) {
+                    const uint32_t w = ff.words[f];
+                    const uint32_t* from = static_cast<const uint32_t*>(ff.data[f]) + i * w;
+                    uint32_t* to = static_cast<uint32_t*>(ft.data[f]) + j * w;
+                    for (uint32_t k = 0; k < w; ++k) to[k] = from[k];
+                }
                 pending = false;
             }
             if (lane == 0) s_run[wave][q0] = run + static_cast<uint32_t>(__popcll(group));
@@ -246,6 +254,7 @@ hipError_t launch_migrate_bases(const MigrateParams& M, hipStream_t stream)
 }
 
 hipError_t launch_migrate_scatter(const dsdx_particles& from, const dsdx_particles& to,
+                                  const FieldParams& fields_from, const FieldParams& fields_to,
                                   const Layout& L, const MigrateParams& M, const Shape& s,
                                   hipStream_t stream)
 {
@@ -254,7 +263,7 @@ hipError_t launch_migrate_scatter(const dsdx_particles& from, const dsdx_particl
     return dispatch(s, [&](auto items, auto waves) {
         constexpr int I = decltype(items)::value, W = decltype(waves)::value;
         return launch(migrate_scatter_kernel<I, W>, chunk_grid(L, W), dim3(W * kWave), stream, from,
-                      to, L.desc, L.chunk_replica, O, M);
+                      to, fields_from, fields_to, L.desc, L.chunk_replica, O, M);
     });
 }
 

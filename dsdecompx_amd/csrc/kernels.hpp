@@ -37,6 +37,7 @@ hipError_t launch_migrate_scan(const Layout& L, const MigrateParams& M, const Sh
                                hipStream_t stream);
 hipError_t launch_migrate_bases(const MigrateParams& M, hipStream_t stream);
 hipError_t launch_migrate_scatter(const dsdx_particles& from, const dsdx_particles& to,
+                                  const FieldParams& fields_from, const FieldParams& fields_to,
                                   const Layout& L, const MigrateParams& M, const Shape& s,
                                   hipStream_t stream);
 

@@ -14,7 +14,9 @@ namespace {
 const char* const kNames[kKernelCount] = {"migrate_wrap", "halo_classify", "halo_scan",
                                           "halo_scatter", "reverse_halo", "reverse_exchange",
                                           "forward_exchange", "halo_refresh", "ghost_owners",
-                                          "migrate_classify", "halo_copy"};
+                                          "migrate_classify", "halo_copy",
+                                          "grid_migrate_count", "grid_migrate_scatter",
+                                          "grid_exchange_counts"};
 
 struct Pending {
     Kernel kernel;

@@ -23,6 +23,9 @@ enum Kernel {
     kGhostOwners,
     kMigrateClassify,
     kCopy,
+    kGridMigrate,
+    kGridScatter,
+    kGridExchange,
     kKernelCount
 };
 

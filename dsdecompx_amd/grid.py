@@ -7,7 +7,11 @@ Storage: two sets of SoA device arrays (``DeviceParticles``) of
 ``ranks * capacity_per_rank`` slots each; subdomain r owns slots
 [r * capacity_per_rank, (r + 1) * capacity_per_rank), owned particles first,
 then its ghosts.  ``migrate`` writes the migrated particles into the spare
-set and makes it current.
+set and makes it current.  Fields registered with ``add_field`` (the
+reference's ParticleInfo::add_field, include/dsdecompx.hpp:105) are held the same
+way, travel with their particles in ``migrate`` and into the ghosts in
+``exchange_halo``, and are exchanged by ``forward_exchange`` /
+``reverse_exchange``.
 """
 from __future__ import annotations
 
@@ -16,7 +20,7 @@ import ctypes as C
 import torch
 
 from . import _abi
-from .halo import BoxConfig, DeviceParticles
+from .halo import BoxConfig, DeviceParticles, _field_array, field_struct
 
 
 class SubdomainGrid:
@@ -28,6 +32,7 @@ class SubdomainGrid:
         self.device = torch.device(device)
         self._storage = [DeviceParticles(self.ranks * self.capacity, self.device) for _ in range(2)]
         self._current = 0
+        self._fields = {}   # name -> ([tensor, tensor], op)
         self._box = box.c()
         self._grid = C.c_void_p()
         d = (C.c_int32 * 3)(*self.dims)
@@ -55,6 +60,24 @@ class SubdomainGrid:
         """The current storage."""
         return self._storage[self._current]
 
+    def add_field(self, name: str, dtype=torch.float64, components: int = 1, op: int = _abi.SUM):
+        """Registers a per-particle field of `components` values (zeros)."""
+        if name in self._fields:
+            raise ValueError(f"field {name!r} is already registered")
+        shape = (self.ranks * self.capacity,) if components == 1 else (self.ranks * self.capacity,
+                                                                        components)
+        self._fields[name] = ([torch.zeros(shape, dtype=dtype, device=self.device) for _ in range(2)],
+                              int(op))
+
+    def field(self, name: str) -> torch.Tensor:
+        """The current storage of a registered field."""
+        return self._fields[name][0][self._current]
+
+    def _table(self, names=None, side=None):
+        side = self._current if side is None else side
+        names = list(self._fields) if names is None else list(names)
+        return _field_array([field_struct(self._fields[n][0][side], self._fields[n][1]) for n in names])
+
     def slots(self, r: int, count: int) -> slice:
         return slice(r * self.capacity, r * self.capacity + count)
 
@@ -73,7 +96,9 @@ class SubdomainGrid:
         """Migration on every subdomain; returns the new owned counts."""
         src = self._storage[self._current].c()
         dst = self._storage[1 - self._current].c()
-        _abi.check(_abi.lib().dsdx_grid_migrate(self._grid, C.byref(src), C.byref(dst),
+        ff, n = self._table(side=self._current)
+        ft, _ = self._table(side=1 - self._current)
+        _abi.check(_abi.lib().dsdx_grid_migrate(self._grid, C.byref(src), C.byref(dst), ff, ft, n,
                                                 self._stream(stream)), "grid migrate")
         self._current = 1 - self._current
         return self.counts()[0]
@@ -82,9 +107,30 @@ class SubdomainGrid:
         """Halo rebuild on every subdomain; returns the ghost counts."""
         p = self.particles.c()
         g = (C.c_uint64 * self.ranks)()
-        _abi.check(_abi.lib().dsdx_grid_halo_build(self._grid, C.byref(p), self._stream(stream), g),
-                   "grid halo")
+        arr, n = self._table()
+        _abi.check(_abi.lib().dsdx_grid_halo_build(self._grid, C.byref(p), arr, n,
+                                                   self._stream(stream), g), "grid halo")
         return list(g)
+
+    def refresh(self, positions_only: bool = False, stream=None) -> None:
+        """Ghosts re-read their owners along the last build's routes."""
+        p = self.particles.c()
+        arr, n = self._table()
+        _abi.check(_abi.lib().dsdx_grid_refresh(self._grid, C.byref(p), arr, n,
+                                                1 if positions_only else 0, self._stream(stream)),
+                   "grid refresh")
+
+    def reverse_exchange(self, names=None, stream=None) -> None:
+        """Ghost values of the registered fields reduced into their owners."""
+        arr, n = self._table(names)
+        _abi.check(_abi.lib().dsdx_grid_reverse_exchange(self._grid, arr, n, self._stream(stream)),
+                   "grid reverse exchange")
+
+    def forward_exchange(self, names=None, stream=None) -> None:
+        """Owner values of the registered fields copied into their ghosts."""
+        arr, n = self._table(names)
+        _abi.check(_abi.lib().dsdx_grid_forward_exchange(self._grid, arr, n, self._stream(stream)),
+                   "grid forward exchange")
 
     def reverse_halo(self, stream=None) -> None:
         p = self.particles

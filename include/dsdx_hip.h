@@ -283,8 +283,9 @@ const uint32_t* dsdx_batch_device_totals(const dsdx_batch* batch);
  *
  * Storage is the caller's: every particle array holds subdomain r in slots
  * [r * capacity_per_rank, (r + 1) * capacity_per_rank), owned first, then its
- * ghosts.  All 13 arrays are required.  Registered fields are not carried by
- * grids.  At most 64 subdomains. */
+ * ghosts; a registered field's array (dsdx_field, AoS of `components`
+ * values per slot) has the same slots.  All 13 particle arrays are required.
+ * At most 64 subdomains. */
 typedef struct dsdx_grid dsdx_grid;
 
 int dsdx_grid_create(const dsdx_box* box, const int32_t* dims, uint64_t capacity_per_rank,
@@ -294,18 +295,33 @@ int dsdx_grid_destroy(dsdx_grid* grid);
 int dsdx_grid_set_owned(dsdx_grid* grid, const uint64_t* owned, void* stream);
 /* Host copies: owned counts, and the ghost counts of the last halo build. */
 int dsdx_grid_counts(const dsdx_grid* grid, uint64_t* owned, uint64_t* ghosts);
-/* Migrates from `from` into the separate storage `to` (same layout); waits
- * for the new counts.  DSDX_ECAPACITY when a subdomain would overflow
+/* Migrates from `from` into the separate storage `to` (same layout), each
+ * registered field from fields_from[f] into fields_to[f] with its particle
+ * (Decomp::migrate with the fields of add_field, src/migration.cpp:391-591);
+ * waits for the new counts.  DSDX_ECAPACITY when a subdomain would overflow
  * (nothing is written then). */
 int dsdx_grid_migrate(dsdx_grid* grid, const dsdx_particles* from, const dsdx_particles* to,
-                      void* stream);
-/* Rebuilds every subdomain's halo; waits for the ghost counts.
- * DSDX_ECAPACITY when a subdomain cannot hold its ghosts (no ghost is
- * written then). */
-int dsdx_grid_halo_build(dsdx_grid* grid, const dsdx_particles* particles, void* stream,
+                      const dsdx_field* fields_from, const dsdx_field* fields_to,
+                      uint32_t field_count, void* stream);
+/* Rebuilds every subdomain's halo, the registered fields copied into the
+ * ghosts with their records; waits for the ghost counts.  DSDX_ECAPACITY
+ * when a subdomain cannot hold its ghosts (no ghost is written then). */
+int dsdx_grid_halo_build(dsdx_grid* grid, const dsdx_particles* particles,
+                         const dsdx_field* fields, uint32_t field_count, void* stream,
                          uint64_t* ghosts);
 /* Ghost forces of every subdomain added to their owners. */
 int dsdx_grid_reverse_halo(dsdx_grid* grid, double* fx, double* fy, double* fz, void* stream);
+/* reverse_exchange / forward_exchange of registered fields on every
+ * subdomain (src/halo.cpp:1396-1532, :1084-1207), as dsdx_reverse_exchange
+ * and dsdx_forward_exchange. */
+int dsdx_grid_reverse_exchange(dsdx_grid* grid, const dsdx_field* fields, uint32_t field_count,
+                               void* stream);
+int dsdx_grid_forward_exchange(dsdx_grid* grid, const dsdx_field* fields, uint32_t field_count,
+                               void* stream);
+/* Reused-plan refresh of every subdomain's ghosts, as dsdx_halo_refresh with
+ * the shifts of the build. */
+int dsdx_grid_refresh(dsdx_grid* grid, const dsdx_particles* particles, const dsdx_field* fields,
+                      uint32_t field_count, int32_t positions_only, void* stream);
 
 /* Per-kernel device timing, process-wide: when enabled, every kernel
  * dispatch carries its own start / stop HIP events (a timed dispatch starts a

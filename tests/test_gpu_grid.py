@@ -136,3 +136,147 @@ def test_steps_bit_exact_with_oracle(gpu, case):
         og.reverse_halo(parts, owned)
         grid.reverse_halo()
         _compare(grid, parts, owned, [0] * og.ranks, f"step {step} reverse", ("fx", "fy", "fz"))
+
+
+def _random_setup(torch, dims, upper, periodic, w, corners, n, seed, cap=None):
+    from dsdecompx_amd.grid import SubdomainGrid
+    from dsdecompx_amd.halo import BoxConfig
+    og = po.Grid(po.Box(upper=upper, periodic=periodic, width=w, edges_and_corners=corners), dims)
+    cap = cap or 4 * n + 512
+    grid = SubdomainGrid(BoxConfig(upper=upper, periodic=periodic, width=w,
+                                   edges_and_corners=corners), dims, cap)
+    rng = np.random.default_rng(seed)
+    parts, owned = [], []
+    L = np.array(upper)
+    for r in range(og.ranks):
+        p = po.Particles(cap)
+        lo, hi = np.array(og.local(r)[0]), np.array(og.local(r)[1])
+        pos = rng.uniform(lo - 0.05 * L, hi + 0.05 * L, (n, 3))
+        for a in range(3):
+            if not periodic[a]:
+                pos[:, a] = np.clip(pos[:, a], 0.0, upper[a] * (1 - 1e-12))
+        p.x[:n], p.y[:n], p.z[:n] = pos.T
+        for k in ("vx", "vy", "vz", "fx", "fy", "fz", "charge"):
+            getattr(p, k)[:n] = rng.standard_normal(n)
+        p.mass[:n] = rng.uniform(0.5, 2.0, n)
+        p.type[:n] = rng.integers(0, 4, n)
+        p.global_id[:n] = np.arange(r * n, (r + 1) * n)
+        parts.append(p)
+        owned.append(n)
+    _load(torch, grid, parts, owned)
+    return og, grid, parts, owned, rng
+
+
+# name, dtype, components, op: one field of each type and reduction
+FIELDS = [("q64", np.float64, 1, po.SUM), ("v32", np.float32, 3, po.MIN),
+          ("i32", np.int32, 2, po.MAX), ("u64", np.uint64, 1, po.REPLACE),
+          ("i64", np.int64, 1, po.SUM), ("u32", np.uint32, 1, po.SUM)]
+
+
+def _field_values(rng, dtype, shape):
+    if np.issubdtype(dtype, np.floating):
+        return rng.standard_normal(shape).astype(dtype)
+    info = np.iinfo(dtype)
+    return rng.integers(max(info.min, -2**40), min(info.max, 2**40), shape, dtype=dtype, endpoint=True)
+
+
+def test_fields_migrate_halo_reverse_forward(gpu):
+    """Registered fields of every type on a 2x2x2 grid: carried by migration
+    and into the ghosts by the halo build, reduced by reverse_exchange with
+    each ReduceOp, copied by forward_exchange -- bit-exact with the oracle."""
+    torch = gpu
+    say("grid fields")
+    og, grid, parts, owned, rng = _random_setup(torch, (2, 2, 2), (12.0, 12.0, 12.0),
+                                                (True, True, True), 1.2, True, 2000, 5)
+    cap = grid.capacity
+    of = [[] for _ in range(og.ranks)]
+    for name, dt, comps, op in FIELDS:
+        tdt = {np.float64: torch.float64, np.float32: torch.float32, np.int32: torch.int32,
+               np.uint64: torch.uint64, np.int64: torch.int64, np.uint32: torch.uint32}[dt]
+        grid.add_field(name, tdt, comps, op)
+        shape = (cap,) if comps == 1 else (cap, comps)
+        for r in range(og.ranks):
+            a = np.zeros(shape, dt)
+            a[:owned[r]] = _field_values(rng, dt, (owned[r],) + shape[1:])
+            of[r].append(po.Field(a, op))
+            grid.field(name)[grid.slots(r, owned[r])] = torch.from_numpy(a[:owned[r]]).cuda()
+
+    def compare(what, counts):
+        for f, (name, *_rest) in enumerate(FIELDS):
+            dev = host(grid.field(name))
+            for r in range(og.ranks):
+                sl = grid.slots(r, counts[r])
+                assert_bits_equal(dev[sl], of[r][f].data[:counts[r]], f"{what} rank {r} {name}")
+
+    og.migrate(parts, owned, of)
+    assert grid.migrate() == owned
+    compare("migrate", owned)
+    ghosts = og.exchange_halo(parts, owned, of)
+    assert grid.exchange_halo() == ghosts
+    compare("halo", [o + g for o, g in zip(owned, ghosts)])
+    _compare(grid, parts, owned, ghosts, "halo records")
+    # reverse: fresh values in every ghost slot
+    for f, (name, dt, comps, op) in enumerate(FIELDS):
+        for r in range(og.ranks):
+            m = owned[r] + ghosts[r]
+            a = of[r][f].data
+            a[owned[r]:m] = _field_values(rng, dt, (ghosts[r],) + a.shape[1:])
+            grid.field(name)[grid.slots(r, m)] = torch.from_numpy(a[:m]).cuda()
+    og.reverse_exchange(owned, of)
+    grid.reverse_exchange()
+    compare("reverse", owned)
+    # forward: fresh owner values
+    for f, (name, dt, comps, op) in enumerate(FIELDS):
+        for r in range(og.ranks):
+            a = of[r][f].data
+            a[:owned[r]] = _field_values(rng, dt, (owned[r],) + a.shape[1:])
+            grid.field(name)[grid.slots(r, owned[r])] = torch.from_numpy(a[:owned[r]]).cuda()
+    og.forward_exchange(owned, of)
+    grid.forward_exchange()
+    compare("forward", [o + g for o, g in zip(owned, ghosts)])
+
+
+@pytest.mark.parametrize("positions_only", [False, True])
+def test_refresh_rewrites_ghosts_from_owners(gpu, positions_only):
+    """A reused plan on a 3x2x1 grid (y non-periodic): after the owners move
+    and change, every ghost equals its owner (found by global id, in whichever
+    subdomain) with the build's periodic shift added to the new position;
+    positions_only leaves the other members and zeroes the forces."""
+    torch = gpu
+    say("grid refresh", positions_only)
+    upper, periodic = (18.0, 12.0, 9.0), (True, False, True)
+    og, grid, parts, owned, rng = _random_setup(torch, (3, 2, 1), upper, periodic, 1.0, True,
+                                                1500, 9)
+    og.migrate(parts, owned)
+    grid.migrate()
+    ghosts = grid.exchange_halo()
+    p = grid.particles
+    before = {k: host(getattr(p, k)).copy() for k in F64 + I32}
+    # owners move (well inside the box) and their other members change
+    n_all = grid.ranks * grid.capacity
+    owned_mask = np.zeros(n_all, bool)
+    for r in range(grid.ranks):
+        owned_mask[grid.slots(r, owned[r])] = True
+    after = {k: v.copy() for k, v in before.items()}
+    for k in ("x", "y", "z"):
+        after[k][owned_mask] += rng.normal(0, 0.01, owned_mask.sum())
+    for k in ("vx", "mass"):
+        after[k][owned_mask] += 1.0
+    for k in ("x", "y", "z", "vx", "mass"):
+        getattr(p, k).copy_(torch.from_numpy(after[k]))
+    p.fx.fill_(3.0)
+    grid.refresh(positions_only=positions_only)
+    dev = {k: host(getattr(p, k)) for k in F64 + I32}
+    where = {int(before["global_id"][j]): j for j in np.flatnonzero(owned_mask)}
+    L = np.array(upper)
+    for r in range(grid.ranks):
+        for j in range(r * grid.capacity + owned[r], r * grid.capacity + owned[r] + ghosts[r]):
+            o = where[int(before["global_id"][j])]
+            for a, k in enumerate(("x", "y", "z")):
+                shift = np.round((before[k][j] - before[k][o]) / L[a]) * L[a]
+                assert after[k][o] + shift == dev[k][j], (r, j, k)
+            for k in ("vx", "vy", "vz", "mass", "charge", "type", "global_id"):
+                expect = before[k][j] if positions_only else after[k][o]
+                assert expect == dev[k][j], (r, j, k)
+            for k in ("fx", "fy", "fz"):
+                assert dev[k][j] == 0.0, (r, j, k)
