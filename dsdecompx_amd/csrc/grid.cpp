@@ -27,10 +27,20 @@ struct dsdx_grid : Plan {
     DeviceBuffer mig_dest, mig_counts, mig_prefix, mig_totals, mig_base, mig_new_owned;
     uint32_t* new_owned_host = nullptr;
     uint32_t* new_owned_host_dev = nullptr;
+    hipEvent_t migrated = nullptr;   // recorded after the migration counts
+    // pinned staging of the replica, chunk and tile tables: the upload does
+    // not wait; the next one waits for `staged` before it rewrites the stage
+    unsigned char* stage = nullptr;
+    size_t stage_bytes = 0;
+    hipEvent_t staged = nullptr;
 
     ~dsdx_grid()
     {
+        if (staged) (void)hipEventSynchronize(staged);
         if (new_owned_host) (void)hipHostFree(new_owned_host);
+        if (stage) (void)hipHostFree(stage);
+        if (migrated) (void)hipEventDestroy(migrated);
+        if (staged) (void)hipEventDestroy(staged);
     }
 };
 
@@ -126,6 +136,47 @@ void rank_desc(dsdx_grid* g, uint32_t r, ReplicaDesc* d)
     d->ghost_list_capacity = g->rank_capacity;
 }
 
+// The replica, chunk and tile tables to the device through the pinned stage,
+// in stream order, without waiting for the copies.
+int upload_tables(dsdx_grid* g, hipStream_t s)
+{
+    const size_t desc_bytes = g->desc.size() * sizeof(ReplicaDesc);
+    const size_t chunk_bytes = g->chunk_replica.size() * 4;
+    const size_t tile_bytes = g->tile_replica.size() * 4;
+    const size_t need = desc_bytes + chunk_bytes + tile_bytes;
+    if (!g->staged) DSDX_HIP(hipEventCreateWithFlags(&g->staged, hipEventDisableTiming), "hipEventCreate");
+    else DSDX_HIP(hipEventSynchronize(g->staged), "wait for the previous table upload");
+    if (need > g->stage_bytes) {
+        if (g->stage) (void)hipHostFree(g->stage);
+        g->stage = nullptr;
+        g->stage_bytes = 0;
+        void* host = nullptr;
+        const size_t grown = std::max(need, g->stage_bytes + g->stage_bytes / 2);
+        DSDX_HIP(hipHostMalloc(&host, grown, hipHostMallocDefault), "hipHostMalloc");
+        g->stage = static_cast<unsigned char*>(host);
+        g->stage_bytes = grown;
+    }
+    DSDX_HIP(g->d_desc.reserve(std::max<size_t>(1, desc_bytes)), "hipMalloc");
+    DSDX_HIP(g->d_chunk_replica.reserve(std::max<size_t>(4, chunk_bytes)), "hipMalloc");
+    DSDX_HIP(g->d_tile_replica.reserve(std::max<size_t>(4, tile_bytes)), "hipMalloc");
+    std::memcpy(g->stage, g->desc.data(), desc_bytes);
+    std::memcpy(g->stage + desc_bytes, g->chunk_replica.data(), chunk_bytes);
+    std::memcpy(g->stage + desc_bytes + chunk_bytes, g->tile_replica.data(), tile_bytes);
+    DSDX_HIP(hipMemcpyAsync(g->d_desc.ptr, g->stage, desc_bytes, hipMemcpyHostToDevice, s),
+             "upload replica table");
+    if (chunk_bytes)
+        DSDX_HIP(hipMemcpyAsync(g->d_chunk_replica.ptr, g->stage + desc_bytes, chunk_bytes,
+                                hipMemcpyHostToDevice, s),
+                 "upload chunk table");
+    if (tile_bytes)
+        DSDX_HIP(hipMemcpyAsync(g->d_tile_replica.ptr, g->stage + desc_bytes + chunk_bytes, tile_bytes,
+                                hipMemcpyHostToDevice, s),
+                 "upload tile table");
+    DSDX_HIP(hipEventRecord(g->staged, s), "record table upload");
+    g->desc_on_device = true;
+    return DSDX_OK;
+}
+
 // Owned counts changed: renumber chunks and tiles, size the buffers, upload.
 int relayout(dsdx_grid* g, hipStream_t s)
 {
@@ -134,7 +185,7 @@ int relayout(dsdx_grid* g, hipStream_t s)
     number_tiles(g);
     if (int rc = reserve_buffers(g, static_cast<uint64_t>(g->ranks) * g->rank_capacity)) return rc;
     DSDX_HIP(g->recv.reserve((static_cast<size_t>(g->ranks) + 1) * kTotals * 4), "hipMalloc");
-    return upload_table(g, s);
+    return upload_tables(g, s);
 }
 
 int check_full(const dsdx_particles* p)
@@ -276,16 +327,21 @@ int dsdx_grid_migrate(dsdx_grid* grid, const dsdx_particles* from, const dsdx_pa
     }
     DSDX_HIP(launch_migrate_scan(L, M, grid->shape, s), "launch migrate scan");
     DSDX_HIP(launch_migrate_bases(M, s), "launch migrate bases");
-    DSDX_HIP(hipStreamSynchronize(s), "wait for the migration counts");
+    if (!grid->migrated)
+        DSDX_HIP(hipEventCreateWithFlags(&grid->migrated, hipEventDisableTiming), "hipEventCreate");
+    DSDX_HIP(hipEventRecord(grid->migrated, s), "record migration counts");
+    // the scatter skips itself when a subdomain would overflow; the host
+    // reads the counts while it runs
+    {
+        timing::Scope timed(timing::kGridScatter, s);
+        DSDX_HIP(launch_migrate_scatter(*from, *to, ff, ft, L, M, grid->shape, s), "launch migrate scatter");
+    }
+    DSDX_HIP(hipEventSynchronize(grid->migrated), "wait for the migration counts");
     std::vector<uint32_t> owned(grid->ranks);
     for (uint32_t q = 0; q < grid->ranks; ++q) {
         owned[q] = grid->new_owned_host[q];
         if (owned[q] > grid->rank_capacity)
             return fail(DSDX_ECAPACITY, "a subdomain cannot hold its particles after migration");
-    }
-    {
-        timing::Scope timed(timing::kGridScatter, s);
-        DSDX_HIP(launch_migrate_scatter(*from, *to, ff, ft, L, M, grid->shape, s), "launch migrate scatter");
     }
     for (uint32_t q = 0; q < grid->ranks; ++q) grid->desc[q].owned = owned[q];
     return relayout(grid, s);
@@ -309,10 +365,16 @@ int dsdx_grid_halo_build(dsdx_grid* grid, const dsdx_particles* particles, const
     if (int rc = run_classify(grid, particles, false, s)) return rc;
     {
         timing::Scope timed(timing::kGridExchange, s);
-        DSDX_HIP(launch_exchange_counts(grid->layout(), grid->buffers(), s), "launch exchange counts");
+        DSDX_HIP(launch_exchange_counts(grid->layout(), grid->buffers(), grid->host_totals_dev, s),
+                 "launch exchange counts");
     }
+    if (!grid->counted) DSDX_HIP(hipEventCreateWithFlags(&grid->counted, hipEventDisableTiming), "hipEventCreate");
+    DSDX_HIP(hipEventRecord(grid->counted, s), "record ghost counts");
+    // route and copy skip themselves when a subdomain overflows; the host
+    // reads the counts while they run
     if (int rc = run_scatter(grid, particles, fp, s)) return rc;
-    if (int rc = read_totals(grid, s)) return rc;
+    DSDX_HIP(hipEventSynchronize(grid->counted), "wait for the ghost counts");
+    grid->totals_cached = true;
     for (uint32_t r = 0; r < grid->ranks; ++r)
         if (grid->host_totals[r * kTotals + kStatusWord])
             return fail(DSDX_ECAPACITY, "a subdomain cannot hold its ghosts");

@@ -17,9 +17,12 @@ namespace {
 // direction m sends through the opposite slot 25 - m
 // (src/backends/mpi_backend.cpp:391-433), blocks in slot order
 // (src/runtime.cpp:946-957).  Writes the block bases, the replica's ghost
-// count, and ORs the overflow bits of every replica into the grid status.
+// count, and ORs the overflow bits of every replica into the grid status;
+// the ghost count and the replica's overflow bits also go to `host` (mapped
+// host memory, kTotals words per replica).
 __global__ __launch_bounds__(kWave) void exchange_counts_kernel(const ReplicaDesc* __restrict__ desc,
-                                                                Layout L, PlanBuffers B)
+                                                                Layout L, PlanBuffers B,
+                                                                uint32_t* __restrict__ host)
 {
     const uint32_t q = blockIdx.x;
     if (q >= L.replicas) return;
@@ -38,6 +41,8 @@ __global__ __launch_bounds__(kWave) void exchange_counts_kernel(const ReplicaDes
         if (static_cast<uint64_t>(d->owned) + received > d->capacity) overflow |= kOverflowStorage;
         if (tot[kColImages] > d->ghost_list_capacity) overflow |= kOverflowGhostList;
         if (overflow) atomicOr(B.recv + static_cast<uint64_t>(L.replicas) * kTotals, overflow);
+        host[static_cast<uint64_t>(q) * kTotals + kColReceived] = received;
+        host[static_cast<uint64_t>(q) * kTotals + kStatusWord] = overflow;
     }
 }
 
@@ -149,11 +154,50 @@ __global__ __launch_bounds__(kWave) void migrate_bases_kernel(MigrateParams M)
     }
 }
 
+// One array of a chunk moved to its new slots: the chunk's ITEMS loads are
+// issued together (coalesced, owned order), then the ITEMS stores.  `op`
+// transforms each value on the way (the periodic wrap of a coordinate).
+template <int ITEMS, typename T, typename Op>
+__device__ __forceinline__ void move_column(const T* __restrict__ src, T* __restrict__ dst,
+                                            uint32_t first, uint32_t owned, bool full,
+                                            const uint32_t (&to)[ITEMS], Op op)
+{
+    const uint32_t lane = lane_id();
+    T v[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t i = first + k * kWave + lane;
+        v[k] = (full || i < owned) ? src[i] : T();
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t i = first + k * kWave + lane;
+        if (full || i < owned) dst[to[k]] = op(v[k]);
+    }
+}
+
+struct Same {
+    template <typename T>
+    __device__ __forceinline__ T operator()(T v) const { return v; }
+};
+
+struct WrapAxis {
+    int periodic;
+    double lower, extent;
+    __device__ __forceinline__ double operator()(double v) const
+    {
+        return periodic ? wrap(v, lower, extent) : v;
+    }
+};
+
 // Pass 4: one wave per chunk of a source replica r.  Each owned particle goes
 // to slot base[q][r] + prefix[q][chunk] + its rank among the chunk's
 // particles bound for q (owned order) of rank q's new arrays, as its wrapped
 // record (wrapped_record: position wrapped, every other member copied) with
-// the words of every registered field (from `ff` into `ft`).
+// the words of every registered field (from `ff` into `ft`).  The new slots
+// of the chunk's particles are found first (one round per distinct
+// destination of an item row, usually one); then every array moves as a
+// column.
 template <int ITEMS, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void migrate_scatter_kernel(
     dsdx_particles src, dsdx_particles dst, FieldParams ff, FieldParams ft,
@@ -166,6 +210,8 @@ __global__ __launch_bounds__(WAVES * kWave) void migrate_scatter_kernel(
     const uint32_t wave = threadIdx.x / kWave;
     const uint32_t chunk = wave_chunk<WAVES>(L, L.chunks);
     if (chunk >= L.chunks) return;
+    // nothing moves when any subdomain would overflow
+    if (__ballot(lane < M.ranks && M.new_owned[lane] > desc[lane].capacity)) return;
     const uint32_t r = replica_of(chunk_replica, chunk);
     const ReplicaDesc* d = desc + r;
     s_run[wave][lane] = 0u;
@@ -173,19 +219,13 @@ __global__ __launch_bounds__(WAVES * kWave) void migrate_scatter_kernel(
     const uint64_t off = d->offset;
     const uint32_t first = (chunk - d->first_chunk) * K;
     const bool full = first + K <= owned;
-    double px[ITEMS], py[ITEMS], pz[ITEMS];
-    load_items<ITEMS>(src.x + off, first, owned, full, px);
-    load_items<ITEMS>(src.y + off, first, owned, full, py);
-    load_items<ITEMS>(src.z + off, first, owned, full, pz);
     const uint64_t slot0 = static_cast<uint64_t>(chunk) * K;
+    uint32_t to[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t il = first + k * kWave + lane;
-        bool pending = il < owned;
+        bool pending = first + k * kWave + lane < owned;
         const uint32_t q = pending ? M.dest[slot0 + k * kWave + lane] : 0u;
-        double v[3];
-        wrapped<ITEMS>(d, px, py, pz, k, v);
-        // one round per distinct destination among the pending lanes
+        to[k] = 0u;
         for (uint64_t left = __ballot(pending); left; left = __ballot(pending)) {
             const uint32_t q0 = __builtin_amdgcn_readlane(q, static_cast<int>(__builtin_ctzll(left)));
             const bool mine = pending && q == q0;
@@ -194,22 +234,10 @@ __global__ __launch_bounds__(WAVES * kWave) void migrate_scatter_kernel(
             __builtin_amdgcn_wave_barrier();
             const uint32_t run = uniform(s_run[wave][q0]);
             if (mine) {
-                const uint64_t pos = static_cast<uint64_t>(M.base[static_cast<uint64_t>(q0) * kMaxRanks + r]) +
-                                     M.prefix[static_cast<uint64_t>(q0) * L.chunks + chunk] + run +
-                                     lanes_below(group);
-                const uint64_t j = desc[q0].offset + pos;
-                const uint64_t i = off + il;
-                dst.x[j] = v[0]; dst.y[j] = v[1]; dst.z[j] = v[2];
-                dst.vx[j] = src.vx[i]; dst.vy[j] = src.vy[i]; dst.vz[j] = src.vz[i];
-                dst.fx[j] = src.fx[i]; dst.fy[j] = src.fy[i]; dst.fz[j] = src.fz[i];
-                dst.mass[j] = src.mass[i]; dst.charge[j] = src.charge[i];
-                dst.type[j] = src.type[i]; dst.global_id[j] = src.global_id[i];
-                for (uint32_t f = 0; f < ff.count; ++f) {
-                    const uint32_t w = ff.words[f];
-                    const uint32_t* from = static_cast<const uint32_t*>(ff.data[f]) + i * w;
-                    uint32_t* to = static_cast<uint32_t*>(ft.data[f]) + j * w;
-                    for (uint32_t k = 0; k < w; ++k) to[k] = from[k];
-                }
+                // ranks * capacity <= 2^32: a slot fits 32 bits
+                to[k] = static_cast<uint32_t>(desc[q0].offset) +
+                        M.base[static_cast<uint64_t>(q0) * kMaxRanks + r] +
+                        M.prefix[static_cast<uint64_t>(q0) * L.chunks + chunk] + run + lanes_below(group);
                 pending = false;
             }
             if (lane == 0) s_run[wave][q0] = run + static_cast<uint32_t>(__popcll(group));
@@ -217,14 +245,45 @@ __global__ __launch_bounds__(WAVES * kWave) void migrate_scatter_kernel(
             __builtin_amdgcn_wave_barrier();
         }
     }
+    const BoxParams& bp = d->bp;
+    move_column<ITEMS>(src.x + off, dst.x, first, owned, full, to,
+                       WrapAxis{bp.periodic[0], bp.lower[0], bp.extent[0]});
+    move_column<ITEMS>(src.y + off, dst.y, first, owned, full, to,
+                       WrapAxis{bp.periodic[1], bp.lower[1], bp.extent[1]});
+    move_column<ITEMS>(src.z + off, dst.z, first, owned, full, to,
+                       WrapAxis{bp.periodic[2], bp.lower[2], bp.extent[2]});
+    move_column<ITEMS>(src.vx + off, dst.vx, first, owned, full, to, Same{});
+    move_column<ITEMS>(src.vy + off, dst.vy, first, owned, full, to, Same{});
+    move_column<ITEMS>(src.vz + off, dst.vz, first, owned, full, to, Same{});
+    move_column<ITEMS>(src.fx + off, dst.fx, first, owned, full, to, Same{});
+    move_column<ITEMS>(src.fy + off, dst.fy, first, owned, full, to, Same{});
+    move_column<ITEMS>(src.fz + off, dst.fz, first, owned, full, to, Same{});
+    move_column<ITEMS>(src.mass + off, dst.mass, first, owned, full, to, Same{});
+    move_column<ITEMS>(src.charge + off, dst.charge, first, owned, full, to, Same{});
+    move_column<ITEMS>(src.type + off, dst.type, first, owned, full, to, Same{});
+    move_column<ITEMS>(src.global_id + off, dst.global_id, first, owned, full, to, Same{});
+    for (uint32_t f = 0; f < ff.count; ++f) {
+        const uint32_t w = ff.words[f];
+        const uint32_t* from = static_cast<const uint32_t*>(ff.data[f]) + off * w;
+        uint32_t* into = static_cast<uint32_t*>(ft.data[f]);
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t i = first + k * kWave + lane;
+            if (i >= owned) continue;
+            const uint64_t a = static_cast<uint64_t>(i) * w;
+            const uint64_t b = static_cast<uint64_t>(to[k]) * w;
+            for (uint32_t c = 0; c < w; ++c) into[b + c] = from[a + c];
+        }
+    }
 }
 
 }  // namespace
 
-hipError_t launch_exchange_counts(const Layout& L, const PlanBuffers& B, hipStream_t stream)
+hipError_t launch_exchange_counts(const Layout& L, const PlanBuffers& B, uint32_t* host,
+                                  hipStream_t stream)
 {
     if (L.replicas == 0) return hipSuccess;
-    return launch(exchange_counts_kernel, dim3(L.replicas), dim3(kWave), stream, L.desc, L, B);
+    return launch(exchange_counts_kernel, dim3(L.replicas), dim3(kWave), stream, L.desc, L, B, host);
 }
 
 hipError_t launch_migrate_count(const dsdx_particles& p, const Layout& L, const MigrateParams& M,

@@ -30,7 +30,8 @@ hipError_t launch_migrate_wrap(double* x, double* y, double* z, const Layout& L,
                                hipStream_t stream);
 
 // Process grids (grid_kernels.hip).
-hipError_t launch_exchange_counts(const Layout& L, const PlanBuffers& B, hipStream_t stream);
+hipError_t launch_exchange_counts(const Layout& L, const PlanBuffers& B, uint32_t* host,
+                                  hipStream_t stream);
 hipError_t launch_migrate_count(const dsdx_particles& p, const Layout& L, const MigrateParams& M,
                                 const Shape& s, hipStream_t stream);
 hipError_t launch_migrate_scan(const Layout& L, const MigrateParams& M, const Shape& s,

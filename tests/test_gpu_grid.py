@@ -280,3 +280,56 @@ def test_refresh_rewrites_ghosts_from_owners(gpu, positions_only):
                 assert expect == dev[k][j], (r, j, k)
             for k in ("fx", "fy", "fz"):
                 assert dev[k][j] == 0.0, (r, j, k)
+
+
+def test_overflow_is_reported_and_writes_nothing(gpu):
+    """A migration that would overfill a subdomain, and a halo that would,
+    raise CapacityError; the target storage and the ghost slots keep their
+    contents, and the grid still builds the oracle's halo afterwards."""
+    torch = gpu
+    from dsdecompx_amd._abi import CapacityError
+    from dsdecompx_amd.grid import SubdomainGrid
+    from dsdecompx_amd.halo import BoxConfig
+    say("grid overflow")
+    upper, cap = (8.0, 8.0, 8.0), 160
+    og = po.Grid(po.Box(upper=upper, width=1.5), (2, 1, 1))
+    grid = SubdomainGrid(BoxConfig(upper=upper, width=1.5), (2, 1, 1), cap)
+    rng = np.random.default_rng(3)
+    parts = []
+    owned = [120, 60]
+    for r, n in enumerate(owned):
+        p = po.Particles(cap)
+        # every particle of rank 0 belongs to rank 1: 180 > 160 after migration
+        p.x[:n] = rng.uniform(4.0, 8.0, n)
+        p.y[:n], p.z[:n] = rng.uniform(0, 8, n), rng.uniform(0, 8, n)
+        p.global_id[:n] = np.arange(n) + 1000 * r
+        parts.append(p)
+    _load(torch, grid, parts, owned)
+    spare = grid._storage[1]
+    spare.x.fill_(-7.0)
+    with pytest.raises(CapacityError):
+        grid.migrate()
+    assert grid._current == 0 and grid.counts()[0] == owned
+    assert bool((host(spare.x) == -7.0).all())
+    # halo: 120 + 60 owned near the x faces of 2 subdomains cannot hold the ghosts
+    grid.particles.x[grid.slots(0, 120)] = torch.from_numpy(rng.uniform(0.0, 0.5, 120)).cuda()
+    grid.particles.x[grid.slots(1, 60)] = torch.from_numpy(rng.uniform(7.5, 8.0, 60)).cuda()
+    grid.particles.y.fill_(4.0)
+    grid.particles.z.fill_(4.0)
+    grid.particles.x[grid.slots(0, cap)][120:] = -9.0
+    grid.particles.x[grid.slots(1, cap)][60:] = -9.0
+    with pytest.raises(CapacityError):
+        grid.exchange_halo()
+    xs = host(grid.particles.x)
+    assert (xs[120:cap] == -9.0).all() and (xs[cap + 60:] == -9.0).all()
+    # back to a state that fits (rank 1 interior, about half of rank 0 near
+    # its +x face): the oracle's halo, bit-exact
+    parts[0].x[:120] = rng.uniform(2.0, 3.0, 120)
+    parts[1].x[:60] = rng.uniform(5.6, 6.4, 60)
+    for r, p in enumerate(parts):
+        p.y[:owned[r]] = 4.0
+        p.z[:owned[r]] = 4.0
+        grid.particles.x[grid.slots(r, owned[r])] = torch.from_numpy(p.x[:owned[r]].copy()).cuda()
+    ghosts = og.exchange_halo(parts, owned)
+    assert grid.exchange_halo() == ghosts
+    _compare(grid, parts, owned, ghosts, "after overflow")
