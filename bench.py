@@ -231,6 +231,41 @@ def measure_cpp(args, n, L, w, sigma, periodic):
     return res
 
 
+def measure_grid(args, n, L, w, sigma, dims=(2, 2, 2)):
+    """The same particles split over a dims process grid, every subdomain on
+    this GPU (dsdx_grid_*), driven by dsdecompx_amd/bin/grid_bench: migrate
+    between subdomains, halo build, reverse_halo per step.  Algorithmic bytes
+    per step: migrate 24 B (positions) + 192 B (every record is rewritten:
+    retained particles are compacted, then arrivals appended) per particle;
+    halo 24 B per particle + 168 B per ghost; reverse 24 B per ghost + 48 B
+    per distinct owner."""
+    import subprocess
+    exe = os.path.join(ROOT, "dsdecompx_amd", "bin", "grid_bench")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    steps = min(args.steps, 50)
+    cmd = [exe, str(n), repr(L), repr(w), repr(sigma), *map(str, dims), str(steps),
+           str(min(args.warmup, 5)), str(args.seed), "10"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        raise SystemExit(f"grid_bench failed ({r.returncode}): {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    ms, G, U = res["hot_ms_per_step"], res["ghosts_per_step"], res["owners_per_step"]
+    step_bytes = 216.0 * n + 24.0 * n + 168.0 * G + 24.0 * G + 48.0 * U
+    return {
+        "workload": f"{n} particles uniform in a periodic box L={L}, halo width {w}, displacement "
+                    f"N(0,{sigma}); {'x'.join(map(str, dims))} process grid, every subdomain on this GPU",
+        "dims": list(dims), "steps": steps,
+        "value": n / (ms * 1e3), "unit": "Mparticles/s", "ms_per_step": ms,
+        "ghosts_per_step": G, "owners_per_step": U,
+        "owned_min": res["owned_min"], "owned_max": res["owned_max"],
+        "step_algorithmic_bytes": step_bytes,
+        "step_hbm_frac_of_peak": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "kernels": {k: {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"]}
+                    for k, v in res["kernels"].items() if v["launches"]},
+    }
+
+
 def measure_batched(args):
     """cfg5: 512 replica boxes, particle counts log-uniform in [2048, 262144],
     density U[0.6, 1.2], w = 2.8, displacement N(0, 0.1).  The same steps
@@ -615,6 +650,8 @@ def run(args):
                 "reverse_avg_ms": (r["kernels"]["reverse_halo"]["total_ms"] /
                                    r["kernels"]["reverse_halo"]["launches"]),
             }
+    if args.grid and all(periodic):
+        out["grid"] = measure_grid(args, n, L, w, sigma)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(name, n, L, w, sigma, periodic,
                                                                  args.cpu_steps)
     if not args.no_batched:
@@ -652,6 +689,8 @@ def main():
                     help="skip the reused-plan (refresh) runs")
     ap.add_argument("--no-autograd", dest="autograd", action="store_false",
                     help="skip the cfg4 autograd-vs-plain comparison")
+    ap.add_argument("--no-grid", dest="grid", action="store_false",
+                    help="skip the 2x2x2 process-grid run of the same particles")
     ap.add_argument("--no-zwalls", dest="zwalls", action="store_false",
                     help="skip the z-non-periodic repeat of the workload")
     ap.add_argument("--tune", action="store_true",

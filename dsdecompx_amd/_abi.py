@@ -111,7 +111,7 @@ def lib():
         "dsdx_grid_create": ([C.POINTER(Box), C.POINTER(i32), u64, P, C.POINTER(P)], C.c_int),
         "dsdx_grid_destroy": ([P], C.c_int),
         "dsdx_grid_set_owned": ([P, C.POINTER(u64), P], C.c_int),
-        "dsdx_grid_counts": ([P, C.POINTER(u64), C.POINTER(u64)], C.c_int),
+        "dsdx_grid_counts": ([P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)], C.c_int),
         "dsdx_grid_migrate": ([P, C.POINTER(Particles), C.POINTER(Particles), C.POINTER(Field),
                                C.POINTER(Field), u32, P], C.c_int),
         "dsdx_grid_halo_build": ([P, C.POINTER(Particles), C.POINTER(Field), u32, P,

@@ -167,7 +167,7 @@ int main(int argc, char** argv)
     Storage* cur = &a;
     Storage* spare = &b;
     hipLaunchKernelGGL(place, dim3(grid(n)), dim3(256), 0, s, cur->p, n, per, cap, L, seed);
-    std::vector<uint64_t> owned(ranks), ghosts(ranks), count(ranks);
+    std::vector<uint64_t> owned(ranks), ghosts(ranks), owners(ranks), count(ranks);
     for (uint32_t r = 0; r < ranks; ++r) owned[r] = std::min<uint64_t>(per, n - std::min<uint64_t>(n, r * per));
     check(dsdx_grid_set_owned(g, owned.data(), s), "set owned");
     uint64_t* d_counts = nullptr;
@@ -177,11 +177,11 @@ int main(int argc, char** argv)
     hipEvent_t ev[4];
     for (auto& e : ev) ok(hipEventCreate(&e), "event");
     uint64_t step = 0;
-    double ghosts_sum = 0.0;
+    double ghosts_sum = 0.0, owners_sum = 0.0;
     auto pass = [&](int k_steps, bool timed, bool count_ghosts) {
         double hot = 0.0;
         for (int k = 0; k < k_steps; ++k, ++step) {
-            check(dsdx_grid_counts(g, owned.data(), nullptr), "counts");
+            check(dsdx_grid_counts(g, owned.data(), nullptr, nullptr), "counts");
             ok(hipMemcpyAsync(d_counts, owned.data(), ranks * 8, hipMemcpyHostToDevice, s), "copy");
             hipLaunchKernelGGL(move, dim3(grid(ranks * cap)), dim3(256), 0, s, cur->p, d_counts, cap, ranks,
                                sigma, seed, step);
@@ -190,7 +190,7 @@ int main(int argc, char** argv)
             std::swap(cur, spare);
             check(dsdx_grid_halo_build(g, &cur->p, nullptr, 0, s, ghosts.data()), "halo build");
             if (timed) ok(hipEventRecord(ev[1], s), "record");
-            check(dsdx_grid_counts(g, owned.data(), nullptr), "counts");
+            check(dsdx_grid_counts(g, owned.data(), nullptr, owners.data()), "counts");
             for (uint32_t r = 0; r < ranks; ++r) count[r] = owned[r] + ghosts[r];
             ok(hipMemcpyAsync(d_counts + ranks, count.data(), ranks * 8, hipMemcpyHostToDevice, s), "copy");
             hipLaunchKernelGGL(fill_forces, dim3(grid(ranks * cap)), dim3(256), 0, s, cur->p, d_counts + ranks,
@@ -206,7 +206,10 @@ int main(int argc, char** argv)
                 hot += t0 + t1;
             }
             if (count_ghosts)
-                for (uint32_t r = 0; r < ranks; ++r) ghosts_sum += static_cast<double>(ghosts[r]);
+                for (uint32_t r = 0; r < ranks; ++r) {
+                    ghosts_sum += static_cast<double>(ghosts[r]);
+                    owners_sum += static_cast<double>(owners[r]);
+                }
         }
         return hot;
     };
@@ -230,11 +233,12 @@ int main(int argc, char** argv)
     const char* kernels[] = {"grid_migrate_count", "grid_migrate_scatter", "halo_classify", "halo_scan",
                              "grid_exchange_counts", "halo_scatter", "halo_copy", "reverse_halo"};
     std::printf("{\"particles\": %llu, \"dims\": [%d, %d, %d], \"capacity_per_rank\": %llu, \"steps\": %d, "
-                "\"warmup\": %d, \"hot_ms_per_step\": %.6f, \"ghosts_per_step\": %.1f, "
+                "\"warmup\": %d, \"hot_ms_per_step\": %.6f, \"ghosts_per_step\": %.1f, \"owners_per_step\": %.1f, "
                 "\"owned_min\": %llu, \"owned_max\": %llu, \"kernel_steps\": %d, "
                 "\"kernel_pass_hot_ms_per_step\": %.6f, \"kernels\": {",
                 static_cast<unsigned long long>(n), dims[0], dims[1], dims[2],
                 static_cast<unsigned long long>(cap), steps, warmup, hot_ms / steps, ghosts_sum / steps,
+                owners_sum / steps,
                 static_cast<unsigned long long>(owned_min), static_cast<unsigned long long>(owned_max),
                 kernel_steps, kernel_steps > 0 ? kernel_hot_ms / kernel_steps : 0.0);
     for (int k = 0; k < 8; ++k) {

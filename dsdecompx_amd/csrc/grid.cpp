@@ -260,12 +260,13 @@ int dsdx_grid_set_owned(dsdx_grid* grid, const uint64_t* owned, void* stream)
     return relayout(grid, as_stream(stream));
 }
 
-int dsdx_grid_counts(const dsdx_grid* grid, uint64_t* owned, uint64_t* ghosts)
+int dsdx_grid_counts(const dsdx_grid* grid, uint64_t* owned, uint64_t* ghosts, uint64_t* owners)
 {
     if (!grid) return fail(DSDX_EINVAL, "null grid");
     for (uint32_t r = 0; r < grid->ranks; ++r) {
         if (owned) owned[r] = grid->desc[r].owned;
         if (ghosts) ghosts[r] = grid->built ? grid->host_totals[r * kTotals + kColReceived] : 0;
+        if (owners) owners[r] = grid->built ? grid->host_totals[r * kTotals + kColOwners] : 0;
     }
     return DSDX_OK;
 }

@@ -18,8 +18,8 @@ namespace {
 // (src/backends/mpi_backend.cpp:391-433), blocks in slot order
 // (src/runtime.cpp:946-957).  Writes the block bases, the replica's ghost
 // count, and ORs the overflow bits of every replica into the grid status;
-// the ghost count and the replica's overflow bits also go to `host` (mapped
-// host memory, kTotals words per replica).
+// the ghost count, the distinct boundary owners and the replica's overflow
+// bits also go to `host` (mapped host memory, kTotals words per replica).
 __global__ __launch_bounds__(kWave) void exchange_counts_kernel(const ReplicaDesc* __restrict__ desc,
                                                                 Layout L, PlanBuffers B,
                                                                 uint32_t* __restrict__ host)
@@ -42,6 +42,7 @@ __global__ __launch_bounds__(kWave) void exchange_counts_kernel(const ReplicaDes
         if (tot[kColImages] > d->ghost_list_capacity) overflow |= kOverflowGhostList;
         if (overflow) atomicOr(B.recv + static_cast<uint64_t>(L.replicas) * kTotals, overflow);
         host[static_cast<uint64_t>(q) * kTotals + kColReceived] = received;
+        host[static_cast<uint64_t>(q) * kTotals + kColOwners] = tot[kColOwners];
         host[static_cast<uint64_t>(q) * kTotals + kStatusWord] = overflow;
     }
 }

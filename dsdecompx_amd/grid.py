@@ -87,10 +87,11 @@ class SubdomainGrid:
                    "grid set owned")
 
     def counts(self):
-        """(owned, ghosts) of every subdomain (ghosts of the last halo build)."""
-        o, g = (C.c_uint64 * self.ranks)(), (C.c_uint64 * self.ranks)()
-        _abi.check(_abi.lib().dsdx_grid_counts(self._grid, o, g))
-        return list(o), list(g)
+        """(owned, ghosts, owners) of every subdomain: ghosts and distinct
+        boundary owners of the last halo build."""
+        o, g, u = (C.c_uint64 * self.ranks)(), (C.c_uint64 * self.ranks)(), (C.c_uint64 * self.ranks)()
+        _abi.check(_abi.lib().dsdx_grid_counts(self._grid, o, g, u))
+        return list(o), list(g), list(u)
 
     def migrate(self, stream=None) -> list:
         """Migration on every subdomain; returns the new owned counts."""

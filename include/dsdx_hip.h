@@ -293,8 +293,10 @@ int dsdx_grid_create(const dsdx_box* box, const int32_t* dims, uint64_t capacity
 int dsdx_grid_destroy(dsdx_grid* grid);
 /* Owned counts of every subdomain (after the caller filled the arrays). */
 int dsdx_grid_set_owned(dsdx_grid* grid, const uint64_t* owned, void* stream);
-/* Host copies: owned counts, and the ghost counts of the last halo build. */
-int dsdx_grid_counts(const dsdx_grid* grid, uint64_t* owned, uint64_t* ghosts);
+/* Host copies: owned counts, and the ghost counts and distinct boundary
+ * owners (particles with at least one image) of the last halo build.  Any
+ * output may be null. */
+int dsdx_grid_counts(const dsdx_grid* grid, uint64_t* owned, uint64_t* ghosts, uint64_t* owners);
 /* Migrates from `from` into the separate storage `to` (same layout), each
  * registered field from fields_from[f] into fields_to[f] with its particle
  * (Decomp::migrate with the fields of add_field, src/migration.cpp:391-591);

@@ -127,6 +127,12 @@ def test_steps_bit_exact_with_oracle(gpu, case):
         ghosts = og.exchange_halo(parts, owned)
         assert grid.exchange_halo() == ghosts, f"step {step} ghost counts"
         _compare(grid, parts, owned, ghosts, f"step {step} halo")
+        # distinct boundary owners: owned particles with an image anywhere
+        imaged = set()
+        for r, p in enumerate(parts):
+            imaged.update(p.global_id[owned[r]:owned[r] + ghosts[r]].tolist())
+        owners = [len(imaged.intersection(p.global_id[:owned[r]].tolist())) for r, p in enumerate(parts)]
+        assert grid.counts()[2] == owners, f"step {step} owners"
         f = rng.uniform(-1, 1, (og.ranks, 3, cap))
         for r, p in enumerate(parts):
             m = owned[r] + ghosts[r]
