@@ -120,6 +120,17 @@ def lib():
         "dsdx_grid_reverse_exchange": ([P, C.POINTER(Field), u32, P], C.c_int),
         "dsdx_grid_forward_exchange": ([P, C.POINTER(Field), u32, P], C.c_int),
         "dsdx_grid_refresh": ([P, C.POINTER(Particles), C.POINTER(Field), u32, i32, P], C.c_int),
+        "dsdx_wire_layout": ([C.POINTER(Field), u32, i32, C.POINTER(u32), C.POINTER(u32)], C.c_int),
+        "dsdx_halo_pack": ([P, C.POINTER(Particles), C.POINTER(Field), u32, u64, i32, P, P], C.c_int),
+        "dsdx_unpack_records": ([P, u64, C.POINTER(Particles), u64, C.POINTER(Field), u32, P], C.c_int),
+        "dsdx_unpack_positions": ([P, u64, C.POINTER(Particles), u64, P], C.c_int),
+        "dsdx_pack_reverse_records": ([P, P, P, u64, u64, P, P], C.c_int),
+        "dsdx_accumulate_reverse_records": ([P, P, P, P, P, u64, P], C.c_int),
+        "dsdx_pack_field_records": ([C.POINTER(Field), u32, u64, u64, P, P], C.c_int),
+        "dsdx_scatter_field_records": ([C.POINTER(Field), u32, P, u64, u64, P], C.c_int),
+        "dsdx_gather_field_records": ([P, C.POINTER(Field), u32, u64, P, P], C.c_int),
+        "dsdx_reduce_field_records": ([P, C.POINTER(Field), u32, P, u64, P], C.c_int),
+        "dsdx_loopback_exchange": ([P, P, P, u32, i32, P], C.c_int),
         "dsdx_timing_enable": ([i32], C.c_int),
         "dsdx_timing_reset": ([], C.c_int),
         "dsdx_timing_read": ([C.c_char_p, C.POINTER(C.c_double), C.POINTER(u64)], C.c_int),
@@ -148,7 +159,11 @@ def exported_symbols():
             "dsdx_batch_migrate_halo_build", "dsdx_grid_create", "dsdx_grid_destroy",
             "dsdx_grid_set_owned", "dsdx_grid_counts", "dsdx_grid_migrate",
             "dsdx_grid_halo_build", "dsdx_grid_reverse_halo", "dsdx_grid_reverse_exchange",
-            "dsdx_grid_forward_exchange", "dsdx_grid_refresh"]
+            "dsdx_grid_forward_exchange", "dsdx_grid_refresh", "dsdx_wire_layout", "dsdx_halo_pack",
+            "dsdx_unpack_records", "dsdx_unpack_positions", "dsdx_pack_reverse_records",
+            "dsdx_accumulate_reverse_records", "dsdx_pack_field_records",
+            "dsdx_scatter_field_records", "dsdx_gather_field_records", "dsdx_reduce_field_records",
+            "dsdx_loopback_exchange"]
 
 
 KERNELS = ("migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "halo_copy",

@@ -399,8 +399,6 @@ int shape_from_env(Shape* s)
 using namespace dsdx;
 using namespace dsdx::abi;
 
-struct dsdx_plan : Plan {};
-struct dsdx_batch : Plan {};
 
 namespace {
 

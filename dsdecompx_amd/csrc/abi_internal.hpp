@@ -121,3 +121,8 @@ int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipS
 int shape_from_env(Shape* s);
 
 }  // namespace dsdx::abi
+
+// The opaque handles of include/dsdx_hip.h: a single-box plan and a batch are
+// plans over one and over R replicas.
+struct dsdx_plan : dsdx::abi::Plan {};
+struct dsdx_batch : dsdx::abi::Plan {};

@@ -161,6 +161,30 @@ struct FieldParams {
     void* data[DSDX_MAX_FIELDS];
 };
 
+// Registered fields inside communication records: the particle record
+// layout (from byte kRecordBytes, back to back) or the exchange record layout
+// (from byte 0, each field aligned to its element size); `stride` bytes per
+// record, a multiple of 8.
+struct RecordFields {
+    uint32_t count;
+    uint32_t stride;
+    int32_t type[DSDX_MAX_FIELDS];
+    int32_t op[DSDX_MAX_FIELDS];
+    uint32_t components[DSDX_MAX_FIELDS];
+    uint32_t words[DSDX_MAX_FIELDS];    // 32-bit words per particle
+    uint32_t offset[DSDX_MAX_FIELDS];   // byte offset inside the record
+    void* data[DSDX_MAX_FIELDS];
+};
+
+constexpr uint32_t kRecordBytes = 96;   // ParticleRecord: 11 doubles, 2 int32
+
+// Runs of one loopback exchange: block b copies words[b] 8-byte words.
+struct BlockMap {
+    uint64_t src[kSlots];
+    uint64_t dst[kSlots];
+    uint64_t words[kSlots];
+};
+
 struct TypedFields {
     uint32_t count;
     int32_t type[DSDX_MAX_FIELDS];

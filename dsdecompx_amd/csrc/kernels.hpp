@@ -42,4 +42,28 @@ hipError_t launch_migrate_scatter(const dsdx_particles& from, const dsdx_particl
                                   const Layout& L, const MigrateParams& M, const Shape& s,
                                   hipStream_t stream);
 
+// Communication records (wire_kernels.hip).  The plan-aware launchers need a
+// single-box plan; their buffers hold one block per send slot, slot order.
+hipError_t launch_pack_halo(const dsdx_particles& p, const RecordFields& rf, int32_t positions_only,
+                            unsigned char* out, const Layout& L, const PlanBuffers& B, const Shape& s,
+                            hipStream_t stream);
+hipError_t launch_unpack_records(const unsigned char* in, uint64_t count, const dsdx_particles& p,
+                                 uint64_t dest, const RecordFields& rf, hipStream_t stream);
+hipError_t launch_unpack_positions(const double* in, uint64_t count, const dsdx_particles& p, uint64_t dest,
+                                   hipStream_t stream);
+hipError_t launch_pack_reverse(const double* fx, const double* fy, const double* fz, uint64_t first,
+                               uint64_t count, unsigned char* out, hipStream_t stream);
+hipError_t launch_accumulate_reverse(const unsigned char* in, double* fx, double* fy, double* fz,
+                                     const Layout& L, const PlanBuffers& B, const Shape& s,
+                                     hipStream_t stream);
+hipError_t launch_pack_field_records(const RecordFields& rf, uint64_t first, uint64_t count,
+                                     unsigned char* out, hipStream_t stream);
+hipError_t launch_scatter_field_records(const RecordFields& rf, const unsigned char* in, uint64_t count,
+                                        uint64_t first, hipStream_t stream);
+hipError_t launch_gather_field_records(const RecordFields& rf, unsigned char* out, const Layout& L,
+                                       const PlanBuffers& B, const Shape& s, hipStream_t stream);
+hipError_t launch_reduce_field_records(const RecordFields& rf, const unsigned char* in, const Layout& L,
+                                       const PlanBuffers& B, const Shape& s, hipStream_t stream);
+hipError_t launch_block_copy(const void* src, void* dst, const BlockMap& map, hipStream_t stream);
+
 }  // namespace dsdx

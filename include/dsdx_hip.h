@@ -268,6 +268,84 @@ int dsdx_batch_counts(const dsdx_batch* batch, uint32_t* ghosts,
  * kernels that must know a replica's ghost count without a host wait. */
 const uint32_t* dsdx_batch_device_totals(const dsdx_batch* batch);
 
+/* ---- Communication records ------------------------------------------------
+ * The reference's wire formats, for callers that carry records over a
+ * transport of their own (MPI, RCCL) between the pack and unpack steps:
+ *   particle record  ParticleRecord (include/field.hpp:150-157): x, y, z, vx,
+ *                    vy, vz, fx, fy, fz, mass, charge (double), type,
+ *                    global_id (int32) -- DSDX_RECORD_BYTES -- then the
+ *                    registered fields back to back; stride rounded up to 8
+ *                    (make_device_wire_layout, src/device_fields.cpp:16-53)
+ *   exchange record  the fields of a forward / reverse exchange from byte 0,
+ *                    each aligned to its element size; stride rounded up to 8
+ *                    (src/halo.cpp:1237-1271, :1422-1457)
+ *   position record  x, y, z (24 bytes)
+ * A send buffer holds one block per send slot in slot order (the reference's
+ * neighbour order), each in owned order, block s starting at the sum of the
+ * send counts of the slots before it (dsdx_plan_query); a receive buffer
+ * holds the receive blocks in slot order, i.e. the ghosts in ghost order.
+ * The plan-aware entry points need a single-box plan with a built halo. */
+#define DSDX_RECORD_BYTES 96
+enum { DSDX_WIRE_PARTICLE = 0, DSDX_WIRE_EXCHANGE = 1 };
+
+/* Byte offset of every field inside a record of `kind`, and the stride. */
+int dsdx_wire_layout(const dsdx_field* fields, uint32_t count, int32_t kind,
+                     uint32_t* offsets, uint32_t* stride);
+/* Particle records of every ghost of the plan into its send block: owner +
+ * periodic shift, zero forces, the fields (positions_only: position records,
+ * no fields).  After a build: launch_halo_scatter's records; on a reused
+ * plan: launch_halo_gather / launch_halo_gather_positions
+ * (src/cuda/kernels.hpp:89-121). */
+int dsdx_halo_pack(const dsdx_plan* plan, const dsdx_particles* particles,
+                   const dsdx_field* fields, uint32_t field_count, uint64_t owned,
+                   int32_t positions_only, void* send, void* stream);
+/* launch_unpack_records / launch_unpack_positions (src/cuda/kernels.hpp:
+ * 124-137): record k into particle dest_offset + k (null arrays skipped);
+ * positions also zero the forces. */
+int dsdx_unpack_records(const void* records, uint64_t count,
+                        const dsdx_particles* particles, uint64_t dest_offset,
+                        const dsdx_field* fields, uint32_t field_count, void* stream);
+int dsdx_unpack_positions(const double* positions, uint64_t count,
+                          const dsdx_particles* particles, uint64_t dest_offset,
+                          void* stream);
+/* launch_pack_reverse_records (src/cuda/kernels.hpp:139-143): particle
+ * records of ghosts [ghost_offset, ghost_offset + count) carrying only their
+ * forces. */
+int dsdx_pack_reverse_records(const double* fx, const double* fy, const double* fz,
+                              uint64_t ghost_offset, uint64_t count, void* out,
+                              void* stream);
+/* The returned force records (send-buffer layout) added to their owners in
+ * ascending send slot: the reference's launch_accumulate_reverse_records once
+ * (src/cuda/kernels.hpp:146-150) once per neighbour block
+ * (src/halo.cpp:1069-1079), as one ordered pass. */
+int dsdx_accumulate_reverse_records(const dsdx_plan* plan, const void* records,
+                                    double* fx, double* fy, double* fz, uint64_t owned,
+                                    void* stream);
+/* launch_pack_reverse_fields / launch_scatter_fields (src/cuda/kernels.hpp:
+ * 56-60, :71-72): exchange records of particles
+ * [ghost_offset, ghost_offset + count). */
+int dsdx_pack_field_records(const dsdx_field* fields, uint32_t field_count,
+                            uint64_t ghost_offset, uint64_t count, void* out, void* stream);
+int dsdx_scatter_field_records(const dsdx_field* fields, uint32_t field_count,
+                               const void* records, uint64_t count, uint64_t ghost_offset,
+                               void* stream);
+/* launch_gather_fields over every send block (exchange records of the
+ * owners, send-buffer layout), and launch_reduce_reverse_fields over the
+ * returned blocks with each field's op, in ascending send slot
+ * (src/cuda/kernels.hpp:62-70). */
+int dsdx_gather_field_records(const dsdx_plan* plan, const dsdx_field* fields,
+                              uint32_t field_count, uint64_t owned, void* out, void* stream);
+int dsdx_reduce_field_records(const dsdx_plan* plan, const dsdx_field* fields,
+                              uint32_t field_count, const void* records, uint64_t owned,
+                              void* stream);
+/* The self messages of the loopback box (src/backends/mpi_backend.cpp:
+ * 366-375): forward, send blocks -> receive blocks (block m takes slot
+ * 25 - m); reverse (non-zero `reverse`), records in ghost order -> back to
+ * the send slots that produced them.  record_bytes: a multiple of 8.  Waits
+ * for the plan's counts if they are not on the host yet. */
+int dsdx_loopback_exchange(dsdx_plan* plan, const void* send, void* recv,
+                           uint32_t record_bytes, int32_t reverse, void* stream);
+
 /* ---- Process grids -------------------------------------------------------
  * The dims[0] x dims[1] x dims[2] Cartesian decomposition of one box, every
  * subdomain ("rank", row-major as MPI_Cart_rank without reordering) held on

@@ -255,6 +255,23 @@ static void reduce_double(double* dst, double v, int op)
     }
 }
 
+/* `components` elements of `type` at src reduced into dst with `op`, element
+ * by element (reduce_field_values, src/halo.cpp:1351-1372). */
+void dsdxo_reduce_elements(int32_t type, int32_t op, void* dst, const void* src,
+                           int32_t components)
+{
+    for (int32_t e = 0; e < components; ++e) {
+        switch (type) {
+        case DSDXO_INT32: reduce_int32_t((int32_t*)dst + e, ((const int32_t*)src)[e], op); break;
+        case DSDXO_INT64: reduce_int64_t((int64_t*)dst + e, ((const int64_t*)src)[e], op); break;
+        case DSDXO_UINT32: reduce_uint32_t((uint32_t*)dst + e, ((const uint32_t*)src)[e], op); break;
+        case DSDXO_UINT64: reduce_uint64_t((uint64_t*)dst + e, ((const uint64_t*)src)[e], op); break;
+        case DSDXO_FLOAT32: reduce_float((float*)dst + e, ((const float*)src)[e], op); break;
+        default: reduce_double((double*)dst + e, ((const double*)src)[e], op); break;
+        }
+    }
+}
+
 /* src/halo.cpp:1396-1532 (Host reverse_exchange).  Per field, send block n
  * holds the ghosts of receive block n; the exchange goes through self
  * messages matched by opposite-slot tags (direct_self_copy=false,

@@ -126,6 +126,58 @@ int dsdxo_forward_exchange(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
 void dsdxo_migrate_wrap(const dsdxo_box* box, int64_t owned, double* x,
                         double* y, double* z);
 
+/* ---- Wire format (dsdx_oracle_wire.c) ------------------------------------
+ * ParticleRecord (include/field.hpp:150-157), the particle and exchange
+ * record layouts and the device path's pack / unpack / reduce steps on the
+ * loopback plan.  Parity pinning (tests/test_oracle_wire.py):
+ *   dsdxo_record / dsdxo_wire_layout  byte offsets and sizes of the record
+ *                            members, stride known answers for every field
+ *                            type mix (both layouts)
+ *   dsdxo_halo_pack + dsdxo_loopback_exchange + dsdxo_unpack_records
+ *                            round trip equal to dsdxo_halo_write_ghosts
+ *                            (records and fields), every record decoded by
+ *                            numpy against its owner + shift
+ *   dsdxo_pack_reverse_records + dsdxo_accumulate_reverse
+ *                            equal to dsdxo_reverse_halo bit for bit
+ *   field records            pack / scatter / gather / reduce round trips
+ *                            equal to dsdxo_forward_exchange and
+ *                            dsdxo_reverse_exchange for every type and op */
+typedef struct {
+    double x, y, z, vx, vy, vz, fx, fy, fz, mass, charge;
+    int32_t type, global_id;
+} dsdxo_record;
+
+enum { DSDXO_WIRE_PARTICLE = 0, DSDXO_WIRE_EXCHANGE = 1 };
+
+void dsdxo_reduce_elements(int32_t type, int32_t op, void* dst, const void* src,
+                           int32_t components);
+int dsdxo_wire_layout(const dsdxo_field* fields, int32_t count, int32_t kind, int64_t* offsets,
+                      int64_t* stride);
+int64_t dsdxo_halo_pack(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
+                        const dsdxo_particles* p, const dsdxo_field* fields, int32_t field_count,
+                        int32_t positions_only, unsigned char* out);
+void dsdxo_unpack_records(const unsigned char* records, int64_t count, const dsdxo_particles* p,
+                          int64_t dest, const dsdxo_field* fields, int32_t field_count);
+void dsdxo_unpack_positions(const double* positions, int64_t count, const dsdxo_particles* p,
+                            int64_t dest);
+void dsdxo_pack_reverse_records(const double* fx, const double* fy, const double* fz,
+                                int64_t ghost_offset, int64_t count, unsigned char* out);
+void dsdxo_accumulate_reverse(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
+                              const unsigned char* records, double* fx, double* fy, double* fz);
+void dsdxo_pack_field_records(const dsdxo_field* fields, int32_t field_count, int64_t first,
+                              int64_t count, unsigned char* out);
+void dsdxo_scatter_field_records(const dsdxo_field* fields, int32_t field_count,
+                                 const unsigned char* records, int64_t count, int64_t first);
+int64_t dsdxo_gather_field_records(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
+                                   const dsdxo_field* fields, int32_t field_count,
+                                   unsigned char* out);
+void dsdxo_reduce_field_records(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
+                                const dsdxo_field* fields, int32_t field_count,
+                                const unsigned char* records);
+void dsdxo_loopback_exchange(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
+                             const unsigned char* send, unsigned char* recv, int64_t record_bytes,
+                             int32_t reverse);
+
 /* ---- P-rank process grids (dsdx_oracle_grid.c) ---------------------------
  * The unmodified Host path on a dims[0] x dims[1] x dims[2] Cartesian grid,
  * every rank restated in one process.  Parity pinning (tests/test_oracle_grid.py):

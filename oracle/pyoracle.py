@@ -137,6 +137,26 @@ def lib():
                                                   C.POINTER(C.c_int32), C.POINTER(C.c_int64),
                                                   C.POINTER(_Field), C.c_int32]
         L.dsdxo_grid_forward_exchange.restype = C.c_int
+        _U8 = C.POINTER(C.c_ubyte)
+        N, PL = C.POINTER(_Neighbors), C.POINTER(_Plan)
+        L.dsdxo_wire_layout.argtypes = [C.POINTER(_Field), C.c_int32, C.c_int32,
+                                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.dsdxo_halo_pack.argtypes = [N, PL, C.POINTER(_Particles), C.POINTER(_Field), C.c_int32,
+                                      C.c_int32, _U8]
+        L.dsdxo_halo_pack.restype = C.c_int64
+        L.dsdxo_unpack_records.argtypes = [_U8, C.c_int64, C.POINTER(_Particles), C.c_int64,
+                                           C.POINTER(_Field), C.c_int32]
+        L.dsdxo_unpack_positions.argtypes = [_DP, C.c_int64, C.POINTER(_Particles), C.c_int64]
+        L.dsdxo_pack_reverse_records.argtypes = [_DP, _DP, _DP, C.c_int64, C.c_int64, _U8]
+        L.dsdxo_accumulate_reverse.argtypes = [N, PL, _U8, _DP, _DP, _DP]
+        L.dsdxo_pack_field_records.argtypes = [C.POINTER(_Field), C.c_int32, C.c_int64,
+                                               C.c_int64, _U8]
+        L.dsdxo_scatter_field_records.argtypes = [C.POINTER(_Field), C.c_int32, _U8, C.c_int64,
+                                                  C.c_int64]
+        L.dsdxo_gather_field_records.argtypes = [N, PL, C.POINTER(_Field), C.c_int32, _U8]
+        L.dsdxo_gather_field_records.restype = C.c_int64
+        L.dsdxo_reduce_field_records.argtypes = [N, PL, C.POINTER(_Field), C.c_int32, _U8]
+        L.dsdxo_loopback_exchange.argtypes = [N, PL, _U8, _U8, C.c_int64, C.c_int32]
         _lib = L
     return _lib
 
@@ -332,6 +352,103 @@ class Oracle:
         arr, n = _fields_c(fields)
         lib().dsdxo_forward_exchange(C.byref(self.nb.raw), C.byref(self.plan.raw),
                                      owned, arr, n)
+
+
+# --- wire format (dsdx_oracle_wire.c) -------------------------------------------
+WIRE_PARTICLE, WIRE_EXCHANGE = 0, 1
+RECORD_BYTES = 96
+
+
+def _u8(a: np.ndarray):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_ubyte))
+
+
+def wire_layout(fields, kind: int):
+    """(field byte offsets, record stride) of a particle record (kind 0) or an
+    exchange record (kind 1)."""
+    arr, n = _fields_c(fields)
+    offs = (C.c_int64 * max(1, n))()
+    stride = C.c_int64()
+    lib().dsdxo_wire_layout(arr, n, kind, offs, C.byref(stride))
+    return [int(offs[f]) for f in range(n)], int(stride.value)
+
+
+class Wire:
+    """The device path's wire steps on an ``Oracle``'s last plan: buffers are
+    numpy uint8 arrays; record counts are the plan's send / receive totals."""
+
+    def __init__(self, ora: "Oracle"):
+        self.ora = ora
+
+    @property
+    def sent(self) -> int:
+        return int(sum(self.ora.plan.raw.send_count[n] for n in range(self.ora.nb.raw.count)))
+
+    def pack(self, p: Particles, fields=(), positions_only=False) -> np.ndarray:
+        stride = 24 if positions_only else wire_layout(fields, WIRE_PARTICLE)[1]
+        out = np.zeros(max(1, self.sent) * stride, np.uint8)
+        arr, n = _fields_c(fields)
+        lib().dsdxo_halo_pack(C.byref(self.ora.nb.raw), C.byref(self.ora.plan.raw), C.byref(p.c()),
+                              arr, n, 1 if positions_only else 0, _u8(out))
+        return out[:self.sent * stride]
+
+    def exchange(self, send: np.ndarray, record_bytes: int, reverse=False) -> np.ndarray:
+        recv = np.zeros(max(1, len(send)), np.uint8)
+        src = np.ascontiguousarray(send) if len(send) else np.zeros(1, np.uint8)
+        lib().dsdxo_loopback_exchange(C.byref(self.ora.nb.raw), C.byref(self.ora.plan.raw),
+                                      _u8(src), _u8(recv), record_bytes, 1 if reverse else 0)
+        return recv[:len(send)]
+
+    @staticmethod
+    def unpack(records: np.ndarray, count: int, p: Particles, dest: int, fields=()) -> None:
+        arr, n = _fields_c(fields)
+        rec = records if len(records) else np.zeros(1, np.uint8)
+        lib().dsdxo_unpack_records(_u8(rec), count, C.byref(p.c()), dest, arr, n)
+
+    @staticmethod
+    def unpack_positions(records: np.ndarray, count: int, p: Particles, dest: int) -> None:
+        pos = np.frombuffer(records.tobytes() or bytes(8), np.float64).copy()
+        lib().dsdxo_unpack_positions(_dp(pos), count, C.byref(p.c()), dest)
+
+    @staticmethod
+    def pack_reverse(p: Particles, first: int, count: int) -> np.ndarray:
+        out = np.zeros(max(1, count) * RECORD_BYTES, np.uint8)
+        lib().dsdxo_pack_reverse_records(_dp(p.fx), _dp(p.fy), _dp(p.fz), first, count, _u8(out))
+        return out[:count * RECORD_BYTES]
+
+    def accumulate_reverse(self, records: np.ndarray, p: Particles) -> None:
+        rec = records if len(records) else np.zeros(1, np.uint8)
+        lib().dsdxo_accumulate_reverse(C.byref(self.ora.nb.raw), C.byref(self.ora.plan.raw),
+                                       _u8(rec), _dp(p.fx), _dp(p.fy), _dp(p.fz))
+
+    @staticmethod
+    def pack_fields(fields, first: int, count: int) -> np.ndarray:
+        stride = wire_layout(fields, WIRE_EXCHANGE)[1]
+        out = np.zeros(max(1, count) * stride, np.uint8)
+        arr, n = _fields_c(fields)
+        lib().dsdxo_pack_field_records(arr, n, first, count, _u8(out))
+        return out[:count * stride]
+
+    @staticmethod
+    def scatter_fields(fields, records: np.ndarray, count: int, first: int) -> None:
+        arr, n = _fields_c(fields)
+        rec = records if len(records) else np.zeros(1, np.uint8)
+        lib().dsdxo_scatter_field_records(arr, n, _u8(rec), count, first)
+
+    def gather_fields(self, fields) -> np.ndarray:
+        stride = wire_layout(fields, WIRE_EXCHANGE)[1]
+        out = np.zeros(max(1, self.sent) * stride, np.uint8)
+        arr, n = _fields_c(fields)
+        lib().dsdxo_gather_field_records(C.byref(self.ora.nb.raw), C.byref(self.ora.plan.raw),
+                                         arr, n, _u8(out))
+        return out[:self.sent * stride]
+
+    def reduce_fields(self, fields, records: np.ndarray) -> None:
+        arr, n = _fields_c(fields)
+        rec = records if len(records) else np.zeros(1, np.uint8)
+        lib().dsdxo_reduce_field_records(C.byref(self.ora.nb.raw), C.byref(self.ora.plan.raw),
+                                         arr, n, _u8(rec))
 
 
 class Grid:

@@ -63,3 +63,36 @@ def test_validation_errors_without_device(lib):
     rc = lib.dsdx_halo_refresh(plan, None, C.byref(p), None, 0, 0, 0, C.c_void_p(1))
     assert rc == _abi.ESTATE
     assert lib.dsdx_plan_destroy(plan) == 0
+
+
+def test_record_layouts_match_the_oracle(lib):
+    """dsdx_wire_layout (host only) against the oracle's restatement of
+    make_device_wire_layout and the exchange-record rule, for every field type
+    mix below; and the plan-aware record calls refuse a plan without a halo."""
+    import numpy as np
+
+    from dsdecompx_amd import _abi
+    from oracle import pyoracle as po
+    mixes = [(), ((np.float64, 1),), ((np.float32, 3),), ((np.float32, 1), (np.float64, 1)),
+             ((np.int32, 1), (np.int64, 2), (np.uint32, 1)), ((np.uint64, 5), (np.float32, 1), (np.int64, 1))]
+    code = {np.int32: _abi.INT32, np.int64: _abi.INT64, np.uint32: _abi.UINT32, np.uint64: _abi.UINT64,
+            np.float32: _abi.FLOAT32, np.float64: _abi.FLOAT64}
+    for mix in mixes:
+        ofields = [po.Field(np.zeros((4, c) if c > 1 else 4, dt)) for dt, c in mix]
+        arr = (_abi.Field * max(1, len(mix)))()
+        for k, (dt, c) in enumerate(mix):
+            arr[k] = _abi.Field(1, code[dt], c, 0)
+        for kind in (0, 1):
+            offs = (C.c_uint32 * max(1, len(mix)))()
+            stride = C.c_uint32()
+            assert lib.dsdx_wire_layout(arr, len(mix), kind, offs, C.byref(stride)) == 0
+            assert ([offs[k] for k in range(len(mix))], stride.value) == po.wire_layout(ofields, kind)
+    assert lib.dsdx_wire_layout(None, 0, 2, None, None) == _abi.EINVAL
+    plan = C.c_void_p()
+    assert lib.dsdx_plan_create(C.byref(plan)) == 0
+    p = _abi.Particles()
+    rc = lib.dsdx_halo_pack(plan, C.byref(p), None, 0, 0, 0, C.c_void_p(8), C.c_void_p(1))
+    assert rc == _abi.ESTATE
+    rc = lib.dsdx_loopback_exchange(plan, None, None, 96, 0, C.c_void_p(1))
+    assert rc == _abi.ESTATE
+    assert lib.dsdx_plan_destroy(plan) == 0
