@@ -452,6 +452,92 @@ def measure_traffic(n, L, w, sigma, periodic, env, seed):
                       f"{CALIBRATION_BYTES}-byte plain copy x 1024 / {CALIBRATION_BYTES})"}
 
 
+def measure_wire(args, n, L, w):
+    """The halo and reverse of the cfg3 workload through the communication
+    records (dsdx_halo_pack ... dsdx_accumulate_reverse_records), as each rank
+    of a transport-based multi-GPU run would execute them: pack the send
+    buffer (96-byte ParticleRecords), move it (dsdx_loopback_exchange stands
+    in for the transport), unpack the ghosts; pack the returned force records,
+    move them back, accumulate them in ascending send slot.  Buffers are
+    allocated once; per-kernel averages come from the library timers.
+    Algorithmic bytes: pack 72 B per owner (x, v, mass, charge, type, id) +
+    96 B per record; loopback 2 x 96 B per record; unpack 96 + 96 B per
+    record; pack_reverse 24 + 96 B per record; accumulate 24 B per record +
+    48 B per owner."""
+    import ctypes as C
+
+    import torch
+    from dsdecompx_amd import _abi
+    from dsdecompx_amd.halo import BoxConfig, DeviceParticles, LoopbackHalo
+    from dsdecompx_amd.wire import RECORD_BYTES, _buffer, _ptr
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.Stream(dev)
+    torch.cuda.set_stream(stream)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(args.seed)
+    p = DeviceParticles(int(n * 1.15) + 4096, dev)
+    for k in ("x", "y", "z"):
+        getattr(p, k)[:n] = torch.rand(n, device=dev, dtype=torch.float64, generator=gen) * L
+    for k in ("vx", "vy", "vz", "fx", "fy", "fz"):
+        getattr(p, k)[:n] = torch.rand(n, device=dev, dtype=torch.float64, generator=gen) - 0.5
+    p.mass[:n] = 1.0
+    p.global_id[:n] = torch.arange(n, device=dev, dtype=torch.int32)
+    halo = LoopbackHalo(BoxConfig(upper=(L, L, L), width=w))
+    G = halo.exchange_halo(p, n)
+    U = _owners(halo)
+    lib = _abi.lib()
+    s = stream.cuda_stream
+    pc = p.c()
+    send, recv = _buffer(G * RECORD_BYTES, dev), _buffer(G * RECORD_BYTES, dev)
+    back, home = _buffer(G * RECORD_BYTES, dev), _buffer(G * RECORD_BYTES, dev)
+    ptrs = [_ptr(b) for b in (send, recv, back, home)]
+    fx, fy, fz = p.fx.data_ptr(), p.fy.data_ptr(), p.fz.data_ptr()
+
+    def step():
+        _abi.check(lib.dsdx_halo_pack(halo.plan, C.byref(pc), None, 0, n, 0, ptrs[0], s))
+        _abi.check(lib.dsdx_loopback_exchange(halo.plan, ptrs[0], ptrs[1], RECORD_BYTES, 0, s))
+        _abi.check(lib.dsdx_unpack_records(ptrs[1], G, C.byref(pc), n, None, 0, s))
+        _abi.check(lib.dsdx_pack_reverse_records(fx, fy, fz, n, G, ptrs[2], s))
+        _abi.check(lib.dsdx_loopback_exchange(halo.plan, ptrs[2], ptrs[3], RECORD_BYTES, 1, s))
+        _abi.check(lib.dsdx_accumulate_reverse_records(halo.plan, ptrs[3], fx, fy, fz, n, s))
+
+    steps = max(10, min(args.steps, 100))
+    for _ in range(5):
+        step()
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(steps):
+        step()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / steps
+    lib.dsdx_timing_reset()
+    lib.dsdx_timing_enable(1)
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    lib.dsdx_timing_enable(0)
+    algo = {"halo_pack": 72.0 * U + 96.0 * G, "loopback_exchange": 2 * 96.0 * G,
+            "unpack_records": 192.0 * G, "pack_reverse": 120.0 * G,
+            "accumulate_reverse": 24.0 * G + 48.0 * U}
+    kernels = {}
+    for name, nbytes in algo.items():
+        tot, cnt = C.c_double(), C.c_uint64()
+        lib.dsdx_timing_read(name.encode(), C.byref(tot), C.byref(cnt))
+        if cnt.value:
+            avg = tot.value / cnt.value
+            kernels[name] = {"avg_ms": avg, "launches": cnt.value, "algorithmic_bytes": nbytes,
+                             "algorithmic_GBps": nbytes / (avg * 1e-3) / 1e9}
+    return {
+        "workload": f"{n} particles uniform in a periodic box L={L}, halo width {w}; one rebuild, "
+                    f"then per step: pack, loopback, unpack, pack_reverse, loopback, accumulate",
+        "records": G, "owners": U, "steps": steps, "ms_per_step": ms, "kernels": kernels,
+    }
+
+
 def measure_autograd(args):
     """cfg4 (65,536 particles, L = 40.3, w = 2.8, one rebuild): the autograd
     form -- owned positions through HaloPositions (positions-only refresh
@@ -652,6 +738,8 @@ def run(args):
             }
     if args.grid and all(periodic):
         out["grid"] = measure_grid(args, n, L, w, sigma)
+    if args.wire and all(periodic):
+        out["wire"] = measure_wire(args, n, L, w)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(name, n, L, w, sigma, periodic,
                                                                  args.cpu_steps)
     if not args.no_batched:
@@ -689,6 +777,8 @@ def main():
                     help="skip the reused-plan (refresh) runs")
     ap.add_argument("--no-autograd", dest="autograd", action="store_false",
                     help="skip the cfg4 autograd-vs-plain comparison")
+    ap.add_argument("--no-wire", dest="wire", action="store_false",
+                    help="skip the communication-record (pack / unpack) run")
     ap.add_argument("--no-grid", dest="grid", action="store_false",
                     help="skip the 2x2x2 process-grid run of the same particles")
     ap.add_argument("--no-zwalls", dest="zwalls", action="store_false",

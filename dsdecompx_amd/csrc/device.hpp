@@ -100,6 +100,7 @@ constexpr int kOrderClassify = 2;
 constexpr int kOrderRoute = 4;
 constexpr int kOrderCopy = 8;
 constexpr int kOrderOwners = 16;
+constexpr int kOrderMigrateScatter = 32;   // process grids: the migration's record moves
 
 // Owner tiles: the owner slots [0, bound_r) of replica r (relative to
 // first_chunk_r * K), bound_r = min(owned_r, capacity_r - owned_r,

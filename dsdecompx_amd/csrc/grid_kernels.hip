@@ -319,7 +319,7 @@ hipError_t launch_migrate_scatter(const dsdx_particles& from, const dsdx_particl
                                   hipStream_t stream)
 {
     if (L.chunks == 0) return hipSuccess;
-    const Layout O = ordered(L, s, kOrderClassify);
+    const Layout O = ordered(L, s, kOrderMigrateScatter);
     return dispatch(s, [&](auto items, auto waves) {
         constexpr int I = decltype(items)::value, W = decltype(waves)::value;
         return launch(migrate_scatter_kernel<I, W>, chunk_grid(L, W), dim3(W * kWave), stream, from,

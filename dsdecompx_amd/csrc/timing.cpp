@@ -16,7 +16,8 @@ const char* const kNames[kKernelCount] = {"migrate_wrap", "halo_classify", "halo
                                           "forward_exchange", "halo_refresh", "ghost_owners",
                                           "migrate_classify", "halo_copy",
                                           "grid_migrate_count", "grid_migrate_scatter",
-                                          "grid_exchange_counts"};
+                                          "grid_exchange_counts", "halo_pack", "unpack_records",
+                                          "pack_reverse", "accumulate_reverse", "loopback_exchange"};
 
 struct Pending {
     Kernel kernel;

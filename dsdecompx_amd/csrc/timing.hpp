@@ -26,6 +26,11 @@ enum Kernel {
     kGridMigrate,
     kGridScatter,
     kGridExchange,
+    kPack,
+    kUnpack,
+    kPackReverse,
+    kAccumulate,
+    kLoopback,
     kKernelCount
 };
 

@@ -9,6 +9,7 @@
 #include "abi_internal.hpp"
 #include "dsdx_hip.h"
 #include "kernels.hpp"
+#include "timing.hpp"
 
 using namespace dsdx;
 using namespace dsdx::abi;
@@ -100,6 +101,7 @@ int dsdx_halo_pack(const dsdx_plan* plan, const dsdx_particles* particles, const
                                     DSDX_WIRE_PARTICLE, false, &rf))
         return rc;
     const hipStream_t s = as_stream(stream);
+    timing::Scope timed(timing::kPack, s);
     DSDX_HIP(launch_pack_halo(*particles, rf, positions_only ? 1 : 0, static_cast<unsigned char*>(send),
                               plan->layout(), plan->buffers(), plan->shape, s),
              "launch halo pack");
@@ -112,6 +114,7 @@ int dsdx_unpack_records(const void* records, uint64_t count, const dsdx_particle
     if (!stream || !particles || (count && !records)) return fail(DSDX_EINVAL, "invalid record unpack");
     RecordFields rf;
     if (int rc = make_record_fields(fields, field_count, DSDX_WIRE_PARTICLE, false, &rf)) return rc;
+    timing::Scope timed(timing::kUnpack, as_stream(stream));
     DSDX_HIP(launch_unpack_records(static_cast<const unsigned char*>(records), count, *particles, dest_offset,
                                    rf, as_stream(stream)),
              "launch record unpack");
@@ -131,6 +134,7 @@ int dsdx_pack_reverse_records(const double* fx, const double* fy, const double* 
                               uint64_t count, void* out, void* stream)
 {
     if (!stream || (count && (!fx || !fy || !fz || !out))) return fail(DSDX_EINVAL, "invalid reverse pack");
+    timing::Scope timed(timing::kPackReverse, as_stream(stream));
     DSDX_HIP(launch_pack_reverse(fx, fy, fz, ghost_offset, count, static_cast<unsigned char*>(out),
                                  as_stream(stream)),
              "launch reverse pack");
@@ -142,6 +146,7 @@ int dsdx_accumulate_reverse_records(const dsdx_plan* plan, const void* records, 
 {
     if (int rc = check_plan(plan, owned)) return rc;
     if (!stream || !records || !fx || !fy || !fz) return fail(DSDX_EINVAL, "invalid reverse accumulate");
+    timing::Scope timed(timing::kAccumulate, as_stream(stream));
     DSDX_HIP(launch_accumulate_reverse(static_cast<const unsigned char*>(records), fx, fy, fz, plan->layout(),
                                        plan->buffers(), plan->shape, as_stream(stream)),
              "launch reverse accumulate");
@@ -234,6 +239,7 @@ int dsdx_loopback_exchange(dsdx_plan* plan, const void* send, void* recv, uint32
         map.src[m] = (reverse ? recv_start[m] : send_start[from]) * words;
         map.dst[m] = (reverse ? send_start[from] : recv_start[m]) * words;
     }
+    timing::Scope timed(timing::kLoopback, s);
     DSDX_HIP(launch_block_copy(send, recv, map, s), "launch loopback exchange");
     return DSDX_OK;
 }

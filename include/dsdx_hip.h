@@ -178,7 +178,9 @@ int dsdx_plan_ghost_owners(const dsdx_plan* plan, int32_t* owners,
  * ascending send-slot order, deterministically (Host order of
  * src/halo.cpp:1351-1372 via :1336-1346).  Replaces
  * launch_pack_reverse_records + launch_accumulate_reverse_records
- * (src/cuda/kernels.hpp:139-150), whose FP64 atomics have no fixed order. */
+ * (src/cuda/kernels.hpp:139-150), which reach the same order through one
+ * atomic-add launch per neighbour block (src/halo.cpp:1069-1079); here one
+ * pass, one lane per owner, no atomics and no records. */
 int dsdx_reverse_halo(const dsdx_plan* plan, double* fx, double* fy,
                       double* fz, uint64_t owned, void* stream);
 
@@ -410,8 +412,10 @@ int dsdx_grid_refresh(dsdx_grid* grid, const dsdx_particles* particles, const ds
  * count of one kernel: "migrate_wrap", "halo_classify", "halo_scan",
  * "halo_scatter" (the route pass: owner map and ghost routes), "halo_copy"
  * (the ghost records), "reverse_halo", "reverse_exchange",
- * "forward_exchange", "halo_refresh", "ghost_owners" or "migrate_classify"
- * (the fused pass).  The GPU-event counterpart of the reference's host-clock
+ * "forward_exchange", "halo_refresh", "ghost_owners", "migrate_classify"
+ * (the fused pass), "grid_migrate_count", "grid_migrate_scatter",
+ * "grid_exchange_counts", "halo_pack", "unpack_records", "pack_reverse",
+ * "accumulate_reverse" or "loopback_exchange".  The GPU-event counterpart of the reference's host-clock
  * timings (include/decomposition.hpp:30-38, src/halo.cpp:316-323). */
 int dsdx_timing_enable(int32_t enable);
 int dsdx_timing_reset(void);
