@@ -53,12 +53,22 @@ __device__ __forceinline__ void copy_words(const uint32_t* __restrict__ from, ui
     for (uint32_t w = 0; w < words; ++w) to[w] = from[w];
 }
 
+// Records are staged in LDS 13 words apart (12 used): 64-bit reads of lane l
+// at word 13 l spread over the banks.
+constexpr uint32_t kRecordWords = kRecordBytes / 8;
+constexpr uint32_t kStageWords = kRecordWords + 1;
+
+__device__ __forceinline__ uint64_t bits_of(double v) { return static_cast<uint64_t>(__double_as_longlong(v)); }
+__device__ __forceinline__ double double_of(uint64_t v) { return __longlong_as_double(static_cast<long long>(v)); }
+
 // store_halo_record (src/cuda/pack.cu:129-142) of every image of every
 // boundary owner: owner + periodic shift of the send slot, velocities, zero
 // forces, mass, charge, type, id, then the field words; or, positions_only,
 // the three shifted coordinates (halo_gather_positions_kernel, :204-216).
 // Reads the owners' current values: after a build it is launch_halo_scatter's
-// output, on a reused plan launch_halo_gather's.
+// output, on a reused plan launch_halo_gather's.  The owner gathers bound it
+// (one lane per owner, sparse in owned order); staging the records through
+// LDS for full-wave stores measured no faster.
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void pack_halo_kernel(
     dsdx_particles p, RecordFields rf, int32_t positions_only, unsigned char* __restrict__ out,
@@ -107,30 +117,46 @@ __global__ __launch_bounds__(WAVES * kWave) void pack_halo_kernel(
 }
 
 // unpack_kernel (src/cuda/unpack.cu:16-31): record k into particle dest + k,
-// every member as received (null arrays skipped), then the field words.
+// every member as received (null arrays skipped), then the field words.  A
+// wave takes 64 records: their header words come in through consecutive
+// lanes into LDS, then each lane writes one particle.
 __global__ __launch_bounds__(256) void unpack_records_kernel(const unsigned char* __restrict__ in,
                                                              uint64_t count, dsdx_particles p,
                                                              uint64_t dest, RecordFields rf)
 {
-    const uint64_t k = blockIdx.x * 256ull + threadIdx.x;
-    if (k >= count) return;
-    const unsigned char* rec = in + k * rf.stride;
-    const double* r = reinterpret_cast<const double*>(rec);
-    const uint64_t ids = reinterpret_cast<const uint64_t*>(rec)[11];
+    __shared__ uint64_t s_rec[4][kWave * kStageWords];
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint64_t base = (blockIdx.x * 4ull + wave) * kWave;
+    if (base >= count) return;
+    const uint32_t n = static_cast<uint32_t>(min<uint64_t>(kWave, count - base));
+    const uint32_t stride_w = rf.stride / 8;
+    const uint64_t* in64 = reinterpret_cast<const uint64_t*>(in);
+    uint64_t* stage = s_rec[wave];
+    for (uint32_t w = lane; w < n * kRecordWords; w += kWave) {
+        const uint32_t j = w / kRecordWords, c = w - j * kRecordWords;
+        stage[j * kStageWords + c] = in64[(base + j) * stride_w + c];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane >= n) return;
+    const uint64_t* r = stage + lane * kStageWords;
+    const uint64_t k = base + lane;
     const uint64_t i = dest + k;
-    if (p.x) p.x[i] = r[0];
-    if (p.y) p.y[i] = r[1];
-    if (p.z) p.z[i] = r[2];
-    if (p.vx) p.vx[i] = r[3];
-    if (p.vy) p.vy[i] = r[4];
-    if (p.vz) p.vz[i] = r[5];
-    if (p.fx) p.fx[i] = r[6];
-    if (p.fy) p.fy[i] = r[7];
-    if (p.fz) p.fz[i] = r[8];
-    if (p.mass) p.mass[i] = r[9];
-    if (p.charge) p.charge[i] = r[10];
-    if (p.type) p.type[i] = static_cast<int32_t>(static_cast<uint32_t>(ids));
-    if (p.global_id) p.global_id[i] = static_cast<int32_t>(static_cast<uint32_t>(ids >> 32));
+    if (p.x) p.x[i] = double_of(r[0]);
+    if (p.y) p.y[i] = double_of(r[1]);
+    if (p.z) p.z[i] = double_of(r[2]);
+    if (p.vx) p.vx[i] = double_of(r[3]);
+    if (p.vy) p.vy[i] = double_of(r[4]);
+    if (p.vz) p.vz[i] = double_of(r[5]);
+    if (p.fx) p.fx[i] = double_of(r[6]);
+    if (p.fy) p.fy[i] = double_of(r[7]);
+    if (p.fz) p.fz[i] = double_of(r[8]);
+    if (p.mass) p.mass[i] = double_of(r[9]);
+    if (p.charge) p.charge[i] = double_of(r[10]);
+    if (p.type) p.type[i] = static_cast<int32_t>(static_cast<uint32_t>(r[11]));
+    if (p.global_id) p.global_id[i] = static_cast<int32_t>(static_cast<uint32_t>(r[11] >> 32));
+    const unsigned char* rec = in + k * rf.stride;
     for (uint32_t f = 0; f < rf.count; ++f)
         copy_words(reinterpret_cast<const uint32_t*>(rec + rf.offset[f]),
                    static_cast<uint32_t*>(rf.data[f]) + i * rf.words[f], rf.words[f]);
@@ -153,22 +179,32 @@ __global__ __launch_bounds__(256) void unpack_positions_kernel(const double* __r
 }
 
 // pack_reverse_kernel (src/cuda/pack.cu:219-227): a default record carrying
-// ghost first + k's force.
+// ghost first + k's force.  A wave takes 64 records: the forces go through
+// LDS and the 768 words leave through consecutive lanes.
 __global__ __launch_bounds__(256) void pack_reverse_kernel(const double* __restrict__ fx,
                                                            const double* __restrict__ fy,
                                                            const double* __restrict__ fz,
                                                            uint64_t first, uint64_t count,
                                                            unsigned char* __restrict__ out)
 {
-    const uint64_t k = blockIdx.x * 256ull + threadIdx.x;
-    if (k >= count) return;
-    double* r = reinterpret_cast<double*>(out + k * kRecordBytes);
-    r[0] = 0.0; r[1] = 0.0; r[2] = 0.0; r[3] = 0.0; r[4] = 0.0; r[5] = 0.0;
-    r[6] = fx[first + k];
-    r[7] = fy[first + k];
-    r[8] = fz[first + k];
-    r[9] = 0.0; r[10] = 0.0;
-    reinterpret_cast<uint64_t*>(r)[11] = 0u;
+    __shared__ double s_f[4][3][kWave];
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = lane_id();
+    const uint64_t base = (blockIdx.x * 4ull + wave) * kWave;
+    if (base >= count) return;
+    const uint32_t n = static_cast<uint32_t>(min<uint64_t>(kWave, count - base));
+    if (lane < n) {
+        s_f[wave][0][lane] = fx[first + base + lane];
+        s_f[wave][1][lane] = fy[first + base + lane];
+        s_f[wave][2][lane] = fz[first + base + lane];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint64_t* o = reinterpret_cast<uint64_t*>(out) + base * kRecordWords;
+    for (uint32_t w = lane; w < n * kRecordWords; w += kWave) {
+        const uint32_t j = w / kRecordWords, c = w - j * kRecordWords;
+        o[w] = (c >= 6 && c <= 8) ? bits_of(s_f[wave][c - 6][j]) : 0u;
+    }
 }
 
 // The returned force records (send-buffer layout) added to their owners, one
