@@ -453,17 +453,20 @@ def measure_traffic(n, L, w, sigma, periodic, env, seed):
 
 
 def measure_wire(args, n, L, w):
-    """The halo and reverse of the cfg3 workload through the communication
-    records (dsdx_halo_pack ... dsdx_accumulate_reverse_records), as each rank
-    of a transport-based multi-GPU run would execute them: pack the send
-    buffer (96-byte ParticleRecords), move it (dsdx_loopback_exchange stands
-    in for the transport), unpack the ghosts; pack the returned force records,
-    move them back, accumulate them in ascending send slot.  Buffers are
+    """The cfg3 halo rebuild and reverse through the communication records,
+    as each rank of a transport-based multi-GPU run would execute them:
+    dsdx_halo_build_records (classify, route, 96-byte ParticleRecords into the
+    send buffer; no ghost written), the transfer (dsdx_loopback_exchange
+    stands in for it), dsdx_unpack_records; then the returned force records:
+    dsdx_pack_reverse_records, the transfer back, dsdx_accumulate_reverse_records.
+    Against the direct path on the same particles: dsdx_halo_build (ghosts
+    written in place, counts waited for) and dsdx_reverse_halo.  Buffers are
     allocated once; per-kernel averages come from the library timers.
-    Algorithmic bytes: pack 72 B per owner (x, v, mass, charge, type, id) +
-    96 B per record; loopback 2 x 96 B per record; unpack 96 + 96 B per
-    record; pack_reverse 24 + 96 B per record; accumulate 24 B per record +
-    48 B per owner."""
+    Algorithmic bytes: pack 48 B per owner (velocities, mass, charge, type,
+    id; positions come densely from the classification pass) + 24 B per owner
+    entry position + 96 B per record; loopback 2 x 96 B per record; unpack
+    96 + 96 B per record; pack_reverse 24 + 96 B per record; accumulate 24 B
+    per record + 48 B per owner."""
     import ctypes as C
 
     import torch
@@ -477,7 +480,8 @@ def measure_wire(args, n, L, w):
     torch.cuda.set_stream(stream)
     gen = torch.Generator(device=dev)
     gen.manual_seed(args.seed)
-    p = DeviceParticles(int(n * 1.15) + 4096, dev)
+    cap = int(n * 1.15) + 4096
+    p = DeviceParticles(cap, dev)
     for k in ("x", "y", "z"):
         getattr(p, k)[:n] = torch.rand(n, device=dev, dtype=torch.float64, generator=gen) * L
     for k in ("vx", "vy", "vz", "fx", "fy", "fz"):
@@ -490,51 +494,70 @@ def measure_wire(args, n, L, w):
     lib = _abi.lib()
     s = stream.cuda_stream
     pc = p.c()
+    box = halo._box
     send, recv = _buffer(G * RECORD_BYTES, dev), _buffer(G * RECORD_BYTES, dev)
     back, home = _buffer(G * RECORD_BYTES, dev), _buffer(G * RECORD_BYTES, dev)
     ptrs = [_ptr(b) for b in (send, recv, back, home)]
     fx, fy, fz = p.fx.data_ptr(), p.fy.data_ptr(), p.fz.data_ptr()
+    count = C.c_uint64()
 
-    def step():
-        _abi.check(lib.dsdx_halo_pack(halo.plan, C.byref(pc), None, 0, n, 0, ptrs[0], s))
+    def record_step():
+        _abi.check(lib.dsdx_halo_build_records(halo.plan, C.byref(box), C.byref(pc), None, 0, n, ptrs[0], G,
+                                               s, C.byref(count)))
         _abi.check(lib.dsdx_loopback_exchange(halo.plan, ptrs[0], ptrs[1], RECORD_BYTES, 0, s))
         _abi.check(lib.dsdx_unpack_records(ptrs[1], G, C.byref(pc), n, None, 0, s))
         _abi.check(lib.dsdx_pack_reverse_records(fx, fy, fz, n, G, ptrs[2], s))
         _abi.check(lib.dsdx_loopback_exchange(halo.plan, ptrs[2], ptrs[3], RECORD_BYTES, 1, s))
         _abi.check(lib.dsdx_accumulate_reverse_records(halo.plan, ptrs[3], fx, fy, fz, n, s))
 
+    def direct_step():
+        _abi.check(lib.dsdx_halo_build(halo.plan, C.byref(box), C.byref(pc), None, 0, n, cap, s,
+                                       C.byref(count)))
+        _abi.check(lib.dsdx_reverse_halo(halo.plan, fx, fy, fz, n, s))
+
     steps = max(10, min(args.steps, 100))
-    for _ in range(5):
-        step()
-    torch.cuda.synchronize()
     t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0.record(stream)
-    for _ in range(steps):
-        step()
-    t1.record(stream)
-    torch.cuda.synchronize()
-    ms = t0.elapsed_time(t1) / steps
+
+    def timed(fn):
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        t0.record(stream)
+        for _ in range(steps):
+            fn()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / steps
+
+    record_ms = timed(record_step)
+    direct_ms = timed(direct_step)
+    assert count.value == G
     lib.dsdx_timing_reset()
     lib.dsdx_timing_enable(1)
     for _ in range(steps):
-        step()
+        record_step()
     torch.cuda.synchronize()
     lib.dsdx_timing_enable(0)
     algo = {"halo_pack": 72.0 * U + 96.0 * G, "loopback_exchange": 2 * 96.0 * G,
             "unpack_records": 192.0 * G, "pack_reverse": 120.0 * G,
-            "accumulate_reverse": 24.0 * G + 48.0 * U}
+            "accumulate_reverse": 24.0 * G + 48.0 * U, "halo_classify": 24.0 * n}
     kernels = {}
-    for name, nbytes in algo.items():
+    for name in ("halo_classify", "halo_scan", "halo_scatter", "halo_pack", "loopback_exchange",
+                 "unpack_records", "pack_reverse", "accumulate_reverse"):
         tot, cnt = C.c_double(), C.c_uint64()
         lib.dsdx_timing_read(name.encode(), C.byref(tot), C.byref(cnt))
         if cnt.value:
             avg = tot.value / cnt.value
-            kernels[name] = {"avg_ms": avg, "launches": cnt.value, "algorithmic_bytes": nbytes,
-                             "algorithmic_GBps": nbytes / (avg * 1e-3) / 1e9}
+            kernels[name] = {"avg_ms": avg, "launches": cnt.value}
+            if name in algo:
+                kernels[name]["algorithmic_bytes"] = algo[name]
+                kernels[name]["algorithmic_GBps"] = algo[name] / (avg * 1e-3) / 1e9
     return {
-        "workload": f"{n} particles uniform in a periodic box L={L}, halo width {w}; one rebuild, "
-                    f"then per step: pack, loopback, unpack, pack_reverse, loopback, accumulate",
-        "records": G, "owners": U, "steps": steps, "ms_per_step": ms, "kernels": kernels,
+        "workload": f"{n} particles uniform in a periodic box L={L}, halo width {w}; per step a halo "
+                    f"rebuild and reverse_halo, through records (build_records, loopback, unpack, "
+                    f"pack_reverse, loopback, accumulate) and directly (halo_build, reverse_halo)",
+        "records": G, "owners": U, "steps": steps,
+        "record_ms_per_step": record_ms, "direct_ms_per_step": direct_ms, "kernels": kernels,
     }
 
 

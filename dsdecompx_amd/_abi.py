@@ -120,6 +120,8 @@ def lib():
         "dsdx_grid_reverse_exchange": ([P, C.POINTER(Field), u32, P], C.c_int),
         "dsdx_grid_forward_exchange": ([P, C.POINTER(Field), u32, P], C.c_int),
         "dsdx_grid_refresh": ([P, C.POINTER(Particles), C.POINTER(Field), u32, i32, P], C.c_int),
+        "dsdx_halo_build_records": ([P, C.POINTER(Box), C.POINTER(Particles), C.POINTER(Field), u32, u64,
+                                     P, u64, P, C.POINTER(u64)], C.c_int),
         "dsdx_wire_layout": ([C.POINTER(Field), u32, i32, C.POINTER(u32), C.POINTER(u32)], C.c_int),
         "dsdx_halo_pack": ([P, C.POINTER(Particles), C.POINTER(Field), u32, u64, i32, P, P], C.c_int),
         "dsdx_unpack_records": ([P, u64, C.POINTER(Particles), u64, C.POINTER(Field), u32, P], C.c_int),
@@ -159,7 +161,7 @@ def exported_symbols():
             "dsdx_batch_migrate_halo_build", "dsdx_grid_create", "dsdx_grid_destroy",
             "dsdx_grid_set_owned", "dsdx_grid_counts", "dsdx_grid_migrate",
             "dsdx_grid_halo_build", "dsdx_grid_reverse_halo", "dsdx_grid_reverse_exchange",
-            "dsdx_grid_forward_exchange", "dsdx_grid_refresh", "dsdx_wire_layout", "dsdx_halo_pack",
+            "dsdx_grid_forward_exchange", "dsdx_grid_refresh", "dsdx_halo_build_records", "dsdx_wire_layout", "dsdx_halo_pack",
             "dsdx_unpack_records", "dsdx_unpack_positions", "dsdx_pack_reverse_records",
             "dsdx_accumulate_reverse_records", "dsdx_pack_field_records",
             "dsdx_scatter_field_records", "dsdx_gather_field_records", "dsdx_reduce_field_records",

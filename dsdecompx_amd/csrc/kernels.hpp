@@ -45,8 +45,8 @@ hipError_t launch_migrate_scatter(const dsdx_particles& from, const dsdx_particl
 // Communication records (wire_kernels.hip).  The plan-aware launchers need a
 // single-box plan; their buffers hold one block per send slot, slot order.
 hipError_t launch_pack_halo(const dsdx_particles& p, const RecordFields& rf, int32_t positions_only,
-                            unsigned char* out, const Layout& L, const PlanBuffers& B, const Shape& s,
-                            hipStream_t stream);
+                            int32_t fresh, unsigned char* out, const Layout& L, const PlanBuffers& B,
+                            const Shape& s, hipStream_t stream);
 hipError_t launch_unpack_records(const unsigned char* in, uint64_t count, const dsdx_particles& p,
                                  uint64_t dest, const RecordFields& rf, hipStream_t stream);
 hipError_t launch_unpack_positions(const double* in, uint64_t count, const dsdx_particles& p, uint64_t dest,

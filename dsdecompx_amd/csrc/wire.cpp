@@ -4,6 +4,7 @@
 // of a loopback box.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 
 #include "abi_internal.hpp"
@@ -17,6 +18,8 @@ using namespace dsdx::abi;
 namespace {
 
 hipStream_t as_stream(void* stream) { return static_cast<hipStream_t>(stream); }
+
+constexpr uint64_t kMaxRecordOwned = (1ull << 31) - 1;
 
 uint32_t element_bytes(int32_t type)
 {
@@ -79,6 +82,52 @@ int check_plan(const dsdx_plan* plan, uint64_t owned)
 
 extern "C" {
 
+// launch_halo_classify + count readback + launch_halo_scatter
+// (src/halo.cpp:464-531): the halo of `owned` particles classified, routed
+// and packed as particle records into `send` (send-buffer layout), with no
+// ghost written to the particle arrays.  The records' positions come from
+// the classification pass.  Waits for the counts (*records = ghosts); a send
+// buffer of fewer than that many records gives DSDX_ECAPACITY with the plan
+// built, after which dsdx_halo_pack fills a larger buffer.
+int dsdx_halo_build_records(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles* particles,
+                            const dsdx_field* fields, uint32_t field_count, uint64_t owned, void* send,
+                            uint64_t send_capacity, void* stream, uint64_t* records)
+{
+    if (!plan) return fail(DSDX_EINVAL, "null plan");
+    if (!stream || !records) return fail(DSDX_EINVAL, "record build needs a stream and a count output");
+    if (owned > kMaxRecordOwned) return fail(DSDX_EINVAL, "too many particles for one device plan");
+    BoxParams bp;
+    if (int rc = make_box_params(box, &bp)) return rc;
+    if (int rc = check_particles(particles, owned)) return rc;
+    RecordFields rf;
+    if (int rc = make_record_fields(fields, field_count, DSDX_WIRE_PARTICLE, false, &rf)) return rc;
+    const hipStream_t s = as_stream(stream);
+    plan->classified = plan->built = plan->totals_cached = false;
+    // no ghost goes to particle storage: only the ghost list bounds the route
+    const uint64_t unbounded = owned + (1ull << 40);
+    const uint64_t guess = std::max<uint64_t>(plan->ghost_list_capacity, owned + 1024);
+    if (int rc = set_single(plan, bp, owned, unbounded, guess, s)) return rc;
+    if (int rc = run_classify(plan, particles, false, s)) return rc;
+    if (int rc = wait_counts(plan)) return rc;
+    const uint64_t g = plan->host_totals[kColImages];
+    if (plan->host_totals[kStatusWord] & kOverflowGhostList)
+        if (int rc = set_single(plan, bp, owned, unbounded, g + g / 4 + 1024, s)) return rc;
+    {
+        timing::Scope timed(timing::kScatter, s);
+        DSDX_HIP(launch_route(plan->layout(), plan->buffers(), plan->shape, s), "launch halo route");
+    }
+    plan->host_totals[kStatusWord] = 0;
+    plan->built = true;
+    *records = g;
+    if (g > send_capacity) return fail(DSDX_ECAPACITY, "send buffer too small for the halo records");
+    if (g && !send) return fail(DSDX_EINVAL, "null send buffer");
+    timing::Scope timed(timing::kPack, s);
+    DSDX_HIP(launch_pack_halo(*particles, rf, 0, 1, static_cast<unsigned char*>(send), plan->layout(),
+                              plan->buffers(), plan->shape, s),
+             "launch halo pack");
+    return DSDX_OK;
+}
+
 int dsdx_wire_layout(const dsdx_field* fields, uint32_t count, int32_t kind, uint32_t* offsets,
                      uint32_t* stride)
 {
@@ -102,7 +151,7 @@ int dsdx_halo_pack(const dsdx_plan* plan, const dsdx_particles* particles, const
         return rc;
     const hipStream_t s = as_stream(stream);
     timing::Scope timed(timing::kPack, s);
-    DSDX_HIP(launch_pack_halo(*particles, rf, positions_only ? 1 : 0, static_cast<unsigned char*>(send),
+    DSDX_HIP(launch_pack_halo(*particles, rf, positions_only ? 1 : 0, 0, static_cast<unsigned char*>(send),
                               plan->layout(), plan->buffers(), plan->shape, s),
              "launch halo pack");
     return DSDX_OK;

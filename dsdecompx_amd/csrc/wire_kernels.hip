@@ -66,12 +66,14 @@ __device__ __forceinline__ double double_of(uint64_t v) { return __longlong_as_d
 // forces, mass, charge, type, id, then the field words; or, positions_only,
 // the three shifted coordinates (halo_gather_positions_kernel, :204-216).
 // Reads the owners' current values: after a build it is launch_halo_scatter's
-// output, on a reused plan launch_halo_gather's.  The owner gathers bound it
-// (one lane per owner, sparse in owned order); staging the records through
-// LDS for full-wave stores measured no faster.
+// output, on a reused plan launch_halo_gather's.  `fresh`: the positions have
+// not changed since the classification pass, which stored them densely per
+// boundary entry (bpos); read them there instead of gathering x, y, z.  The
+// owner gathers bound it (one lane per owner, sparse in owned order);
+// staging the records through LDS for full-wave stores measured no faster.
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void pack_halo_kernel(
-    dsdx_particles p, RecordFields rf, int32_t positions_only, unsigned char* __restrict__ out,
+    dsdx_particles p, RecordFields rf, int32_t positions_only, int32_t fresh, unsigned char* __restrict__ out,
     const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ tile_replica, Layout L,
     PlanBuffers B)
 {
@@ -82,7 +84,17 @@ __global__ __launch_bounds__(WAVES * kWave) void pack_halo_kernel(
     const BoxParams& bp = o.d->bp;
     const uint64_t i = o.d->offset + B.owner_index[o.u];
     const OwnerGhosts g = ghosts_of(desc, o, B);
-    const double x = p.x[i], y = p.y[i], z = p.z[i];
+    double x, y, z;
+    if (fresh) {
+        const uint32_t E = B.owner_entry[o.u];
+        x = B.bpos[E];
+        y = B.bpos[B.bpos_stride + E];
+        z = B.bpos[2 * B.bpos_stride + E];
+    } else {
+        x = p.x[i];
+        y = p.y[i];
+        z = p.z[i];
+    }
     if (positions_only) {
         uint32_t k = 0;
         for (uint32_t rest = g.slots; rest; rest &= rest - 1u, ++k) {
@@ -367,15 +379,15 @@ unsigned blocks_for(uint64_t n) { return static_cast<unsigned>((n + 255) / 256);
 }  // namespace
 
 hipError_t launch_pack_halo(const dsdx_particles& p, const RecordFields& rf, int32_t positions_only,
-                            unsigned char* out, const Layout& L, const PlanBuffers& B, const Shape& s,
-                            hipStream_t stream)
+                            int32_t fresh, unsigned char* out, const Layout& L, const PlanBuffers& B,
+                            const Shape& s, hipStream_t stream)
 {
     if (L.tiles == 0) return hipSuccess;
     const Layout O = ordered(L, s, kOrderOwners);
     return dispatch_waves(s, [&](auto waves) {
         constexpr int W = decltype(waves)::value;
         return launch(pack_halo_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, p, rf, positions_only,
-                      out, L.desc, L.tile_replica, O, B);
+                      fresh, out, L.desc, L.tile_replica, O, B);
     });
 }
 

@@ -55,6 +55,29 @@ class Records:
     def __init__(self, halo: LoopbackHalo):
         self.halo = halo
 
+    def build(self, p: DeviceParticles, owned: int, fields=(), capacity=None, stream=None):
+        """dsdx_halo_build_records: classify, route and pack the particle
+        records of a new halo (no ghost written to `p`).  Returns
+        (send buffer, records); grows the buffer once if `capacity` (records)
+        is too small."""
+        stride = layout(fields, PARTICLE)[1]
+        cap = int(capacity if capacity is not None else max(1024, owned // 8))
+        out = _buffer(cap * stride, p.x.device)
+        arr, n = _field_array(fields)
+        pc = p.c()
+        box = self.halo._box
+        g = C.c_uint64()
+        rc = _abi.lib().dsdx_halo_build_records(self.halo.plan, C.byref(box), C.byref(pc), arr, n, owned,
+                                                _ptr(out), cap, _stream_ptr(stream), C.byref(g))
+        if rc == _abi.ECAPACITY:
+            out = self.pack(p, owned, fields, stream=stream)
+        else:
+            _abi.check(rc, "halo build records")
+            out = out[:g.value * stride]
+        self.halo.owned = owned
+        self.halo.ghosts = int(g.value)
+        return out, int(g.value)
+
     def counts(self, stream=None):
         """(records sent, records received): the plan's send and ghost totals."""
         send, recv, g, _, _ = self.halo.query(stream)

@@ -6,8 +6,10 @@
  * self-loopback grid: each of the 26 neighbour slots is this GPU, and the
  * ghost block of receive slot m is filled directly from the owners in the
  * send list of the opposite slot 25-m (the reference's self-neighbour rule,
- * src/backends/mpi_backend.cpp:366-375), so no wire records and no transport
- * are involved.
+ * src/backends/mpi_backend.cpp:366-375), so the direct entry points involve
+ * no wire records and no transport.  The record entry points ("Communication
+ * records") produce and consume the reference's wire format for callers that
+ * keep a transport of their own, and dsdx_grid_* holds several subdomains.
  *
  * Conventions
  *   - Plain pointers and sizes only.  Particle arrays are device (or managed)
@@ -290,6 +292,18 @@ const uint32_t* dsdx_batch_device_totals(const dsdx_batch* batch);
 #define DSDX_RECORD_BYTES 96
 enum { DSDX_WIRE_PARTICLE = 0, DSDX_WIRE_EXCHANGE = 1 };
 
+/* launch_halo_classify + count readback + launch_halo_scatter
+ * (src/halo.cpp:464-531, src/cuda/kernels.hpp:78-101): the halo of `owned`
+ * particles classified, routed and packed as particle records into `send`
+ * (send-buffer layout); no ghost is written to the particle arrays, and the
+ * records' positions come from the classification pass.  Waits for the
+ * counts (*records = number of records).  With send_capacity (in records)
+ * below that: DSDX_ECAPACITY, the plan built, and dsdx_halo_pack fills a
+ * larger buffer. */
+int dsdx_halo_build_records(dsdx_plan* plan, const dsdx_box* box,
+                            const dsdx_particles* particles, const dsdx_field* fields,
+                            uint32_t field_count, uint64_t owned, void* send,
+                            uint64_t send_capacity, void* stream, uint64_t* records);
 /* Byte offset of every field inside a record of `kind`, and the stride. */
 int dsdx_wire_layout(const dsdx_field* fields, uint32_t count, int32_t kind,
                      uint32_t* offsets, uint32_t* stride);

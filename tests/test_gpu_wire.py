@@ -172,3 +172,47 @@ def test_empty_halo_records(gpu):
     rec.unpack(rec.loopback(send, 96), 0, dev, 4)
     rec.accumulate_reverse(rec.loopback(rec.pack_reverse(dev, 4, 0), 96, reverse=True), dev, 4)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
+def test_build_records_equals_oracle_pack(gpu, case):
+    """dsdx_halo_build_records (classify, route, pack from the classification
+    pass's positions, no ghost written): the oracle's send buffer byte for
+    byte; the particle arrays past `owned` untouched; a send buffer one record
+    short gives DSDX_ECAPACITY, after which dsdx_halo_pack fills a larger one."""
+    torch = gpu
+    import ctypes as C
+    from dsdecompx_amd import _abi
+    from dsdecompx_amd.wire import PARTICLE, Records, _buffer, _ptr, layout
+    say("build records", case[0])
+    ora, op, of, halo, dev, df, n = _setup(torch, case, seed=13)
+    g = ora.build(op, n)
+    expect = po.Wire(ora).pack(op, of).tobytes()
+    rec = Records(halo)
+    dev.x[n:].fill_(-3.0)
+    send, got = rec.build(dev, n, df, capacity=g + 5)
+    assert got == g and _bytes(send) == expect
+    assert (host(dev.x)[n:] == -3.0).all()
+    # one record short
+    stride = layout(df, PARTICLE)[1]
+    small = _buffer(max(0, g - 1) * stride, dev.x.device)
+    arr = (_abi.Field * len(df))()
+    from dsdecompx_amd.halo import field_struct
+    for k, t in enumerate(df):
+        arr[k] = field_struct(t)
+    cnt = C.c_uint64()
+    pc = dev.c()
+    rc = _abi.lib().dsdx_halo_build_records(halo.plan, C.byref(halo._box), C.byref(pc), arr, len(df), n,
+                                            _ptr(small), max(0, g - 1),
+                                            torch.cuda.current_stream().cuda_stream, C.byref(cnt))
+    if g:
+        assert rc == _abi.ECAPACITY and cnt.value == g
+        assert _bytes(rec.pack(dev, n, df)) == expect
+    # the records feed the ghosts of the direct path
+    recv = rec.loopback(send, stride)
+    rec.unpack(recv, g, dev, n, df)
+    ref = to_device(synth(*case[1:3], 13), dev.capacity)
+    ref_f = [f.clone() for f in df]
+    halo.exchange_halo(ref, n, ref_f)
+    for k in F64 + I32:
+        assert_bits_equal(host(getattr(dev, k))[:n + g], host(getattr(ref, k))[:n + g], f"ghosts {k}")
