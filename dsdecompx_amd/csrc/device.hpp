@@ -82,7 +82,7 @@ struct ReplicaDesc {
 // lane (the chunk is kWave * items particles) and waves per block.
 struct Shape {
     int items = 16;
-    int waves = 2;
+    int waves = 4;
     int scan_threads = 1024;
     int xcd_run = 16;   // consecutive blocks per XCD turn (0: hardware order)
     // Bit mask of passes that walk their chunks / tiles last to first.  The

@@ -201,7 +201,7 @@ int dsdx_forward_exchange(const dsdx_plan* plan, const dsdx_field* fields,
 
 /* Launch shape of the plan's kernels: lane items per chunk (4, 8, 16; a
  * chunk is 64 x items particles, one wave each), waves per block (2, 4, 8)
- * and scan block threads (256, 512, 1024).  Defaults 8 / 4 / 1024, or the
+ * and scan block threads (256, 512, 1024).  Defaults 16 / 4 / 1024, or the
  * environment variables DSDX_ITEMS, DSDX_WAVES, DSDX_SCAN_THREADS; the
  * autotuner (dsdecompx_amd.autotune) picks them by measurement.  Changing
  * items drops the plan. */
