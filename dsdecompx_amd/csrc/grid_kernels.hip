@@ -47,8 +47,9 @@ __global__ __launch_bounds__(kWave) void exchange_counts_kernel(const ReplicaDes
     }
 }
 
-// owner_rank (src/migration.cpp:74-98) of an already wrapped position: the
-// cut interval by upper_bound - 1, clamped; ranks row-major.
+// owner_rank (src/migration.cpp:74-98) of a position whose periodic
+// coordinates owner_rank has wrapped itself: the cut interval by
+// upper_bound - 1, clamped; ranks row-major.
 __device__ __forceinline__ uint32_t owner_rank(const double (&v)[3], const MigrateParams& M)
 {
     int32_t c[3];
@@ -103,8 +104,14 @@ __global__ __launch_bounds__(WAVES * kWave) void migrate_count_kernel(
     for (int k = 0; k < ITEMS; ++k) {
         const uint32_t i = first + k * kWave + lane;
         if (i >= owned) continue;
+        // migration asks owner_rank for the wrapped record's position, and
+        // owner_rank wraps it again (src/migration.cpp:83-87, :440): a record
+        // that wrapped to exactly the upper face belongs to the lower one
         double v[3];
         wrapped<ITEMS>(d, px, py, pz, k, v);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            if (d->bp.periodic[a]) v[a] = wrap(v[a], d->bp.lower[a], d->bp.extent[a]);
         const uint32_t q = owner_rank(v, M);
         M.dest[slot0 + k * kWave + lane] = static_cast<uint8_t>(q);
         atomicAdd(&s_cnt[wave][q], 1u);

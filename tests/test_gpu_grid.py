@@ -339,3 +339,47 @@ def test_overflow_is_reported_and_writes_nothing(gpu):
     ghosts = og.exchange_halo(parts, owned)
     assert grid.exchange_halo() == ghosts
     _compare(grid, parts, owned, ghosts, "after overflow")
+
+
+def test_exact_ties_on_cuts_walls_and_slabs(gpu):
+    """Positions exactly on the cuts (owner by upper_bound - 1), on the box
+    faces, just below the lower face (wraps to exactly the upper face, no
+    fold-back, clamped to the last subdomain), and exactly on the slab
+    thresholds lower + w / upper - w of a subdomain: migration and halo
+    bit-exact with the oracle on a 2x3x2 grid (y non-periodic)."""
+    torch = gpu
+    from dsdecompx_amd.grid import SubdomainGrid
+    from dsdecompx_amd.halo import BoxConfig
+    say("grid ties")
+    upper, periodic, w, dims = (8.0, 9.0, 6.0), (True, False, True), 0.5, (2, 3, 2)
+    og = po.Grid(po.Box(upper=upper, periodic=periodic, width=w), dims)
+    grid = SubdomainGrid(BoxConfig(upper=upper, periodic=periodic, width=w), dims, 4096)
+    cuts = [og.cuts(a) for a in range(3)]
+    vals = []
+    for a in range(3):
+        c = cuts[a]
+        v = set(c) | {x + w for x in c} | {x - w for x in c}
+        v |= {np.nextafter(x, -np.inf) for x in c} | {np.nextafter(x, np.inf) for x in c}
+        if periodic[a]:
+            v |= {-1e-17, -0.0}
+        v = sorted(x for x in v if (periodic[a] or 0.0 <= x <= upper[a]))
+        vals.append(v)
+    rng = np.random.default_rng(2)
+    pos = np.array([(rng.choice(vals[0]), rng.choice(vals[1]), rng.choice(vals[2])) for _ in range(1200)])
+    n = len(pos) // og.ranks
+    parts, owned = [], []
+    for r in range(og.ranks):
+        p = po.Particles(4096)
+        chunk = pos[r * n:(r + 1) * n]
+        p.x[:n], p.y[:n], p.z[:n] = chunk.T
+        p.global_id[:n] = np.arange(r * n, (r + 1) * n)
+        p.vx[:n] = rng.standard_normal(n)
+        parts.append(p)
+        owned.append(n)
+    _load(torch, grid, parts, owned)
+    og.migrate(parts, owned)
+    assert grid.migrate() == owned
+    _compare(grid, parts, owned, [0] * og.ranks, "ties migrate")
+    ghosts = og.exchange_halo(parts, owned)
+    assert grid.exchange_halo() == ghosts
+    _compare(grid, parts, owned, ghosts, "ties halo")

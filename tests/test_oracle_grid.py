@@ -64,6 +64,22 @@ def test_particle_migration_example():
     assert parts[0].x[0] == 0.25 and parts[1].x[0] == 9.75
 
 
+def test_record_on_the_upper_face_belongs_to_the_first_subdomain():
+    """x = -1e-17 wraps to exactly 10.0 (no fold-back, wrapped_record), and
+    migrate asks owner_rank for that position, which wraps it again to 0.0
+    (src/migration.cpp:83-87, :440): the particle is stored at 10.0 but owned
+    by the first subdomain.  A single wrap would give it to the last one."""
+    g = po.Grid(po.Box(upper=(10.0, 10.0, 10.0)), (2, 1, 1))
+    parts = [po.Particles(8) for _ in range(2)]
+    parts[1].x[0], parts[1].y[0], parts[1].z[0] = -1e-17, 5.0, 5.0
+    parts[1].global_id[0] = 7
+    owned = [0, 1]
+    g.migrate(parts, owned)
+    assert owned == [1, 0]
+    assert parts[0].x[0] == 10.0 and parts[0].global_id[0] == 7
+    assert g.owner_rank(10.0, 5.0, 5.0) == 0
+
+
 @pytest.mark.parametrize("dims", [(2, 1, 1), (1, 3, 1), (2, 2, 2), (3, 2, 1), (4, 1, 2)])
 @pytest.mark.parametrize("periodic", [(True, True, True), (True, False, True)])
 def test_topology_closed_form(dims, periodic):
