@@ -6,6 +6,7 @@
 // replica; migration moves particles between replicas.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -147,11 +148,11 @@ int upload_tables(dsdx_grid* g, hipStream_t s)
     if (!g->staged) DSDX_HIP(hipEventCreateWithFlags(&g->staged, hipEventDisableTiming), "hipEventCreate");
     else DSDX_HIP(hipEventSynchronize(g->staged), "wait for the previous table upload");
     if (need > g->stage_bytes) {
+        const size_t grown = std::max(need, g->stage_bytes + g->stage_bytes / 2);
         if (g->stage) (void)hipHostFree(g->stage);
         g->stage = nullptr;
         g->stage_bytes = 0;
         void* host = nullptr;
-        const size_t grown = std::max(need, g->stage_bytes + g->stage_bytes / 2);
         DSDX_HIP(hipHostMalloc(&host, grown, hipHostMallocDefault), "hipHostMalloc");
         g->stage = static_cast<unsigned char*>(host);
         g->stage_bytes = grown;
