@@ -231,6 +231,32 @@ def measure_cpp(args, n, L, w, sigma, periodic):
     return res
 
 
+def measure_fields(args):
+    """cfg2 through the drop-in C++ API (comm_bench, mode "fields"):
+    1,048,576 particles in 64 Gaussian clusters (sigma 4) in a periodic box
+    L = 101.6, halo width 2.8, displacement N(0, 0.5) per step, and two
+    registered fields (add_field) -- density (f64) and vec (3 x f64), F = 32 B
+    per particle -- that travel into the ghosts with exchange_halo; per step
+    migrate, exchange_halo, reverse_halo, reverse_exchange(Sum) of both
+    fields.  Algorithmic bytes per step: migrate 48 N, halo 24 N + (168 + 2F) G,
+    reverse 24 G + 48 U, reverse_exchange F G + 2F U."""
+    n, L, w, sigma = 1048576, 101.6, 2.8, 0.5
+    res = run_comm_bench(args, n, L, w, sigma, (True, True, True), dict(os.environ), mode="fields")
+    ms, G, U = res["hot_ms_per_step"], res["ghosts_per_step"], res["owners"]
+    F = 32.0
+    step_bytes = 48.0 * n + 24.0 * n + (168.0 + 2 * F) * G + 24.0 * G + 48.0 * U + F * G + 2 * F * U
+    return {
+        "workload": f"cfg2: {n} particles in 64 Gaussian clusters (sigma 4) in a periodic box L={L}, "
+                    f"halo width {w}, displacement N(0,{sigma}) per axis per step, 2 registered "
+                    f"fields (f64 density, 3 x f64 vector), reverse_exchange(Sum) of both",
+        "value": n / (ms * 1e3), "unit": "Mparticles/s", "ms_per_step": ms,
+        "ghosts_per_step": G, "owners": U, "step_algorithmic_bytes": step_bytes,
+        "step_hbm_frac_of_peak": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "kernels": {k: {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"]}
+                    for k, v in res["kernels"].items() if v["launches"]},
+    }
+
+
 def measure_grid(args, n, L, w, sigma, dims=(2, 2, 2)):
     """The same particles split over a dims process grid, every subdomain on
     this GPU (dsdx_grid_*), driven by dsdecompx_amd/bin/grid_bench: migrate
@@ -759,6 +785,8 @@ def run(args):
                 "reverse_avg_ms": (r["kernels"]["reverse_halo"]["total_ms"] /
                                    r["kernels"]["reverse_halo"]["launches"]),
             }
+    if args.fields:
+        out["fields"] = measure_fields(args)
     if args.grid and all(periodic):
         out["grid"] = measure_grid(args, n, L, w, sigma)
     if args.wire and all(periodic):
@@ -800,6 +828,8 @@ def main():
                     help="skip the reused-plan (refresh) runs")
     ap.add_argument("--no-autograd", dest="autograd", action="store_false",
                     help="skip the cfg4 autograd-vs-plain comparison")
+    ap.add_argument("--no-fields", dest="fields", action="store_false",
+                    help="skip the cfg2 run (clustered particles with registered fields)")
     ap.add_argument("--no-wire", dest="wire", action="store_false",
                     help="skip the communication-record (pack / unpack) run")
     ap.add_argument("--no-grid", dest="grid", action="store_false",

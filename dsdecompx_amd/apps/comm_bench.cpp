@@ -21,6 +21,11 @@
 // (examples/communication_test/main.cu:232-236): the plan built in the first
 // warm-up step is reused, exchange_halo(rebuild = false), full records or
 // positions only, then reverse_halo; no migrate.
+// mode "fields": positions in 64 Gaussian clusters (sigma 4, centres uniform,
+// wrapped into the box) and two registered fields that travel with the
+// particles and into the ghosts -- "density" (f64, uniform [0, 2]) and "vec"
+// (3 x f64, uniform [-1, 1]); the step is the split one plus
+// reverse_exchange(Sum) of both fields after reverse_halo.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -61,13 +66,28 @@ __device__ __forceinline__ double unit(uint64_t seed, uint64_t stream, uint64_t 
 }
 
 __global__ void place(double* x, double* y, double* z, double* vx, double* vy, double* vz,
-                      double* m, double* q, int* t, int* id, uint64_t n, double L, uint64_t seed)
+                      double* m, double* q, int* t, int* id, uint64_t n, double L, uint64_t seed,
+                      int clusters)
 {
     const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) return;
-    x[i] = L * unit(seed, 0, i);
-    y[i] = L * unit(seed, 1, i);
-    z[i] = L * unit(seed, 2, i);
+    if (clusters) {
+        // a Gaussian (sigma 4) around one of `clusters` uniform centres, wrapped
+        const uint64_t c = static_cast<uint64_t>(unit(seed, 6, i) * clusters);
+        double* axis[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a) {
+            const double centre = L * unit(seed, 7 + a, c);
+            const double u1 = 1.0 - unit(seed, 10 + 2 * a, i);
+            const double u2 = unit(seed, 11 + 2 * a, i);
+            double v = centre + 4.0 * sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+            v -= floor(v / L) * L;
+            axis[a][i] = v < L ? v : 0.0;
+        }
+    } else {
+        x[i] = L * unit(seed, 0, i);
+        y[i] = L * unit(seed, 1, i);
+        z[i] = L * unit(seed, 2, i);
+    }
     vx[i] = unit(seed, 3, i) - 0.5;
     vy[i] = unit(seed, 4, i) - 0.5;
     vz[i] = unit(seed, 5, i) - 0.5;
@@ -96,6 +116,14 @@ __global__ void move(double* x, double* y, double* z, uint64_t n, double sigma, 
         }
         axis[a][i] = v;
     }
+}
+
+__global__ void fill_fields(double* density, double* vec, uint64_t n, uint64_t seed)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    density[i] = 2.0 * unit(seed, 30, i);
+    for (int c = 0; c < 3; ++c) vec[3 * i + c] = 2.0 * unit(seed, 31 + c, i) - 1.0;
 }
 
 __global__ void fill_forces(double* fx, double* fy, double* fz, uint64_t count, uint64_t seed,
@@ -141,7 +169,7 @@ int main(int argc, char** argv)
     if (argc < 9) {
         std::fprintf(stderr,
                      "usage: %s particles L width sigma z_periodic steps warmup seed [calibration_bytes "
-                     "[kernel_steps [split|fused|refresh|refresh_pos]]]\n",
+                     "[kernel_steps [split|fused|refresh|refresh_pos|fields]]]\n",
                      argv[0]);
         return 2;
     }
@@ -156,11 +184,12 @@ int main(int argc, char** argv)
     const uint64_t calibration = argc > 9 ? std::strtoull(argv[9], nullptr, 10) / 8 : 0;
     const int kernel_steps = argc > 10 ? std::atoi(argv[10]) : steps;
     const std::string mode = argc > 11 ? argv[11] : "split";
-    if (mode != "split" && mode != "fused" && mode != "refresh" && mode != "refresh_pos") {
+    if (mode != "split" && mode != "fused" && mode != "refresh" && mode != "refresh_pos" && mode != "fields") {
         std::fprintf(stderr, "unknown mode %s\n", mode.c_str());
         return 2;
     }
     const bool refresh = mode == "refresh" || mode == "refresh_pos";
+    const bool fields = mode == "fields";
 
     Config config;
     config.domain.upper = {L, L, L};
@@ -173,7 +202,20 @@ int main(int argc, char** argv)
     ParticleInfo p = ParticleInfo::allocate(MemorySpace::CudaDevice, n);
     hipLaunchKernelGGL(place, dim3(grid(n)), dim3(256), 0, s, p.x_data(), p.y_data(), p.z_data(),
                        p.vx_data(), p.vy_data(), p.vz_data(), p.mass_data(), p.charge_data(),
-                       p.type_data(), p.global_id_data(), n, L, seed);
+                       p.type_data(), p.global_id_data(), n, L, seed, fields ? 64 : 0);
+    // registered fields (mode "fields"): application-owned device arrays with
+    // room for the ghosts
+    double* density = nullptr;
+    double* vec = nullptr;
+    const uint64_t field_capacity = 2 * n + 4096;
+    if (fields) {
+        ok(hipMalloc(&density, field_capacity * sizeof(double)), "malloc");
+        ok(hipMalloc(&vec, 3 * field_capacity * sizeof(double)), "malloc");
+        hipLaunchKernelGGL(fill_fields, dim3(grid(n)), dim3(256), 0, s, density, vec, n, seed);
+        p.add_field<double>("density", density, field_capacity);
+        p.add_field<double>("vec", vec, field_capacity, 3);
+    }
+    const std::vector<ReverseExchangeField> reduced = {{"density", ReduceOp::Sum}, {"vec", ReduceOp::Sum}};
     ok(hipStreamSynchronize(s), "place");
 
     hipEvent_t ev[4];
@@ -203,6 +245,7 @@ int main(int argc, char** argv)
                                p.fz_data(), total, seed, step);
             if (timed) ok(hipEventRecord(ev[2], s), "record");
             decomp.reverse_halo(p, s);
+            if (fields) decomp.reverse_exchange(p, reduced, s);
             if (timed) {
                 ok(hipEventRecord(ev[3], s), "record");
                 ok(hipEventSynchronize(ev[3]), "sync");
@@ -257,7 +300,7 @@ int main(int argc, char** argv)
     }
 
     const char* kernels[] = {"migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "halo_copy",
-                             "reverse_halo", "migrate_classify", "halo_refresh"};
+                             "reverse_halo", "migrate_classify", "halo_refresh", "reverse_exchange"};
     std::printf("{\"particles\": %llu, \"steps\": %d, \"warmup\": %d, \"hot_ms_per_step\": %.6f, "
                 "\"wall_ms_per_step\": %.6f, \"ghosts_per_step\": %.1f, \"last_ghosts\": %llu, "
                 "\"owners\": %llu, \"kernel_steps\": %d, \"kernel_pass_hot_ms_per_step\": %.6f, "
@@ -265,7 +308,7 @@ int main(int argc, char** argv)
                 static_cast<unsigned long long>(n), steps, warmup, hot_ms / steps, wall_ms / steps,
                 ghosts_sum / steps, static_cast<unsigned long long>(last_ghosts), owners, kernel_steps,
                 kernel_steps > 0 ? kernel_pass_hot_ms / kernel_steps : 0.0);
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < 9; ++k) {
         double ms = 0.0;
         uint64_t launches = 0;
         dsdx_timing_read(kernels[k], &ms, &launches);
@@ -274,6 +317,8 @@ int main(int argc, char** argv)
     }
     std::printf("}}\n");
     for (auto& e : ev) (void)hipEventDestroy(e);
+    if (density) (void)hipFree(density);
+    if (vec) (void)hipFree(vec);
     (void)hipStreamDestroy(s);
     return 0;
 }
