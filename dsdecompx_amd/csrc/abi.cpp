@@ -149,6 +149,7 @@ int make_field_params(const dsdx_field* fields, uint32_t count, FieldParams* fp)
         if (bytes % 4 != 0) return fail(DSDX_EINVAL, "field size must be a multiple of 4 bytes");
         fp->words[f] = static_cast<uint32_t>(bytes / 4);
         fp->data[f] = fields[f].data;
+        if (bytes % 8 == 0 && reinterpret_cast<uintptr_t>(fields[f].data) % 8 == 0) fp->wide |= 1u << f;
     }
     fp->count = count;
     return DSDX_OK;

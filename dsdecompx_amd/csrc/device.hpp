@@ -158,6 +158,7 @@ struct MigrateParams {
 
 struct FieldParams {
     uint32_t count;
+    uint32_t wide;                     // bit f: field f moves as 8-byte units (even words, 8-byte aligned)
     uint32_t words[DSDX_MAX_FIELDS];   // 32-bit words per particle
     void* data[DSDX_MAX_FIELDS];
 };

@@ -289,6 +289,9 @@ int dsdx_grid_migrate(dsdx_grid* grid, const dsdx_particles* from, const dsdx_pa
     FieldParams ff, ft;
     if (int rc = make_field_params(fields_from, field_count, &ff)) return rc;
     if (int rc = make_field_params(fields_to, field_count, &ft)) return rc;
+    // 8-byte moves only where both sides allow them
+    ff.wide &= ft.wide;
+    ft.wide = ff.wide;
     for (uint32_t f = 0; f < field_count; ++f) {
         if (ff.words[f] != ft.words[f] || fields_from[f].type != fields_to[f].type)
             return fail(DSDX_EINVAL, "source and target fields differ in type or size");

@@ -184,6 +184,24 @@ __device__ __forceinline__ void move_column(const T* __restrict__ src, T* __rest
     }
 }
 
+// One registered field of a chunk moved to its new slots, `units` values of
+// T per particle.
+template <int ITEMS, typename T>
+__device__ __forceinline__ void move_field_units(const T* __restrict__ from, T* __restrict__ into,
+                                                 uint32_t units, uint32_t first, uint32_t owned,
+                                                 const uint32_t (&to)[ITEMS])
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        const uint32_t i = first + k * kWave + lane;
+        if (i >= owned) continue;
+        const uint64_t a = static_cast<uint64_t>(i) * units;
+        const uint64_t b = static_cast<uint64_t>(to[k]) * units;
+        for (uint32_t c = 0; c < units; ++c) into[b + c] = from[a + c];
+    }
+}
+
 struct Same {
     template <typename T>
     __device__ __forceinline__ T operator()(T v) const { return v; }
@@ -271,17 +289,12 @@ __global__ __launch_bounds__(WAVES * kWave) void migrate_scatter_kernel(
     move_column<ITEMS>(src.type + off, dst.type, first, owned, full, to, Same{});
     move_column<ITEMS>(src.global_id + off, dst.global_id, first, owned, full, to, Same{});
     for (uint32_t f = 0; f < ff.count; ++f) {
-        const uint32_t w = ff.words[f];
-        const uint32_t* from = static_cast<const uint32_t*>(ff.data[f]) + off * w;
-        uint32_t* into = static_cast<uint32_t*>(ft.data[f]);
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const uint32_t i = first + k * kWave + lane;
-            if (i >= owned) continue;
-            const uint64_t a = static_cast<uint64_t>(i) * w;
-            const uint64_t b = static_cast<uint64_t>(to[k]) * w;
-            for (uint32_t c = 0; c < w; ++c) into[b + c] = from[a + c];
-        }
+        if ((ff.wide >> f) & 1u)
+            move_field_units<ITEMS>(static_cast<const uint64_t*>(ff.data[f]) + off * (ff.words[f] / 2),
+                                    static_cast<uint64_t*>(ft.data[f]), ff.words[f] / 2, first, owned, to);
+        else
+            move_field_units<ITEMS>(static_cast<const uint32_t*>(ff.data[f]) + off * ff.words[f],
+                                    static_cast<uint32_t*>(ft.data[f]), ff.words[f], first, owned, to);
     }
 }
 

@@ -172,12 +172,21 @@ __global__ __launch_bounds__(kMaxScanThreads) void scan_kernel(const ReplicaDesc
 // image_start, owner_entry, ghost_list), from which the copy pass fills the
 // ghost records.  The capacity test is uniform over a replica: an
 // overflowing replica gets its status word set and no route.
+template <class T>
+__device__ __forceinline__ void copy_units(T* dd, uint32_t units, uint64_t from, uint64_t to)
+{
+    for (uint32_t k = 0; k < units; ++k) dd[to * units + k] = dd[from * units + k];
+}
+
+// Every registered field of particle `from` into particle `to`, in 8-byte
+// units where the field allows it.
 __device__ __forceinline__ void copy_field_words(const FieldParams& fp, uint64_t from, uint64_t to)
 {
     for (uint32_t f = 0; f < fp.count; ++f) {
-        const uint32_t w = fp.words[f];
-        uint32_t* dd = static_cast<uint32_t*>(fp.data[f]);
-        for (uint32_t k = 0; k < w; ++k) dd[to * w + k] = dd[from * w + k];
+        if ((fp.wide >> f) & 1u)
+            copy_units(static_cast<uint64_t*>(fp.data[f]), fp.words[f] / 2, from, to);
+        else
+            copy_units(static_cast<uint32_t*>(fp.data[f]), fp.words[f], from, to);
     }
 }
 
@@ -392,6 +401,17 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_fields_kernel(
     }
 }
 
+template <class T>
+__device__ __forceinline__ void forward_units(T* dd, uint32_t units, uint64_t i, const OwnerGhosts& ghosts)
+{
+    for (uint32_t c = 0; c < units; ++c) {
+        const T value = dd[i * units + c];
+        uint32_t k = 0;
+        for (uint32_t rest = ghosts.slots; rest; rest &= rest - 1u, ++k)
+            dd[ghosts.at(k, static_cast<uint32_t>(__builtin_ctz(rest))) * units + c] = value;
+    }
+}
+
 // forward_exchange: owner values copied into each of its ghost slots.
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void forward_fields_kernel(
@@ -403,14 +423,10 @@ __global__ __launch_bounds__(WAVES * kWave) void forward_fields_kernel(
     const uint64_t i = o.d->offset + B.owner_index[o.u];
     const OwnerGhosts ghosts = ghosts_of(desc, o, B);
     for (uint32_t f = 0; f < fp.count; ++f) {
-        const uint32_t w = fp.words[f];
-        uint32_t* dd = static_cast<uint32_t*>(fp.data[f]);
-        for (uint32_t c = 0; c < w; ++c) {
-            const uint32_t value = dd[i * w + c];
-            uint32_t k = 0;
-            for (uint32_t rest = ghosts.slots; rest; rest &= rest - 1u, ++k)
-                dd[ghosts.at(k, static_cast<uint32_t>(__builtin_ctz(rest))) * w + c] = value;
-        }
+        if ((fp.wide >> f) & 1u)
+            forward_units(static_cast<uint64_t*>(fp.data[f]), fp.words[f] / 2, i, ghosts);
+        else
+            forward_units(static_cast<uint32_t*>(fp.data[f]), fp.words[f], i, ghosts);
     }
 }
 
