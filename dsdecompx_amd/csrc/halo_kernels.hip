@@ -346,9 +346,35 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_forces_kernel(
     fx[i] = ax; fy[i] = ay; fz[i] = az;
 }
 
+// Up to C components at once: one pass over the images, each image's
+// components read together; every component still accumulates its images in
+// ascending send slot.
+template <class T, int C>
+__device__ __forceinline__ void reduce_components(T* data, int32_t op, uint64_t i, const OwnerGhosts& g)
+{
+    T acc[C];
+#pragma unroll
+    for (int e = 0; e < C; ++e) acc[e] = data[i * C + e];
+    uint32_t k = 0;
+    for (uint32_t rest = g.slots; rest; rest &= rest - 1u, ++k) {
+        const T* v = data + g.at(k, static_cast<uint32_t>(__builtin_ctz(rest))) * C;
+#pragma unroll
+        for (int e = 0; e < C; ++e) acc[e] = reduce_value<T>(acc[e], v[e], op);
+    }
+#pragma unroll
+    for (int e = 0; e < C; ++e) data[i * C + e] = acc[e];
+}
+
 template <class T>
 __device__ void reduce_field(T* data, uint32_t c, int32_t op, uint64_t i, const OwnerGhosts& g)
 {
+    switch (c) {
+    case 1: reduce_components<T, 1>(data, op, i, g); return;
+    case 2: reduce_components<T, 2>(data, op, i, g); return;
+    case 3: reduce_components<T, 3>(data, op, i, g); return;
+    case 4: reduce_components<T, 4>(data, op, i, g); return;
+    default: break;
+    }
     for (uint32_t e = 0; e < c; ++e) {
         T acc = data[i * c + e];
         uint32_t k = 0;
