@@ -383,3 +383,81 @@ def test_exact_ties_on_cuts_walls_and_slabs(gpu):
     ghosts = og.exchange_halo(parts, owned)
     assert grid.exchange_halo() == ghosts
     _compare(grid, parts, owned, ghosts, "ties halo")
+
+
+def _random_case(seed):
+    """A seeded grid configuration: dims (2..12 subdomains), a box with
+    non-zero lower bounds and irrational-looking extents, a width up to a
+    quarter of the smallest subdomain, random periodicity and neighbour
+    policy."""
+    rng = np.random.default_rng(seed)
+    while True:
+        dims = tuple(int(v) for v in rng.integers(1, 4, 3))
+        if 2 <= dims[0] * dims[1] * dims[2] <= 12:
+            break
+    lower = tuple(float(v) for v in rng.uniform(-50.0, 50.0, 3))
+    ext = rng.uniform(6.0, 14.0, 3) * np.array(dims)
+    upper = tuple(float(l + e) for l, e in zip(lower, ext))
+    w = float(rng.uniform(0.2, 0.25) * min(ext / np.array(dims)))
+    periodic = tuple(bool(v) for v in rng.integers(0, 2, 3))
+    if not any(periodic):
+        periodic = (True, periodic[1], periodic[2])
+    corners = bool(rng.integers(0, 2))
+    return dims, lower, upper, periodic, w, corners
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_grids_with_shifted_boxes(gpu, seed):
+    """Seeded grids on boxes that do not start at the origin: two steps of
+    migrate / halo / reverse, every array bit-exact with the oracle."""
+    torch = gpu
+    from dsdecompx_amd.grid import SubdomainGrid
+    from dsdecompx_amd.halo import BoxConfig
+    dims, lower, upper, periodic, w, corners = _random_case(seed)
+    say("random grid", seed, dims, periodic, corners)
+    og = po.Grid(po.Box(lower=lower, upper=upper, periodic=periodic, width=w, edges_and_corners=corners), dims)
+    n, cap = 800, 6000
+    grid = SubdomainGrid(BoxConfig(lower=lower, upper=upper, periodic=periodic, width=w,
+                                   edges_and_corners=corners), dims, cap)
+    rng = np.random.default_rng(100 + seed)
+    lo, hi = np.array(lower), np.array(upper)
+    parts, owned = [], []
+    for r in range(og.ranks):
+        p = po.Particles(cap)
+        pos = rng.uniform(lo - 0.1 * (hi - lo), hi + 0.1 * (hi - lo), (n, 3))
+        for a in range(3):
+            if not periodic[a]:
+                pos[:, a] = np.clip(pos[:, a], lo[a], hi[a])
+        p.x[:n], p.y[:n], p.z[:n] = pos.T
+        for k in ("vx", "vy", "vz", "charge"):
+            getattr(p, k)[:n] = rng.standard_normal(n)
+        p.mass[:n] = 1.0
+        p.type[:n] = rng.integers(0, 3, n)
+        p.global_id[:n] = np.arange(r * n, (r + 1) * n)
+        parts.append(p)
+        owned.append(n)
+    _load(torch, grid, parts, owned)
+    for step in range(2):
+        og.migrate(parts, owned)
+        assert grid.migrate() == owned, f"seed {seed} step {step} owned"
+        _compare(grid, parts, owned, [0] * og.ranks, f"seed {seed} step {step} migrate")
+        ghosts = og.exchange_halo(parts, owned)
+        assert grid.exchange_halo() == ghosts, f"seed {seed} step {step} ghosts"
+        _compare(grid, parts, owned, ghosts, f"seed {seed} step {step} halo")
+        f = rng.uniform(-1, 1, (og.ranks, 3, cap))
+        for r, p in enumerate(parts):
+            m = owned[r] + ghosts[r]
+            for a, k in enumerate(("fx", "fy", "fz")):
+                getattr(p, k)[:m] = f[r, a, :m]
+                getattr(grid.particles, k)[grid.slots(r, m)] = torch.from_numpy(f[r, a, :m]).cuda()
+        og.reverse_halo(parts, owned)
+        grid.reverse_halo()
+        _compare(grid, parts, owned, [0] * og.ranks, f"seed {seed} step {step} reverse", ("fx", "fy", "fz"))
+        # move every owned particle a little for the next step
+        for r, p in enumerate(parts):
+            for a, k in enumerate(("x", "y", "z")):
+                v = getattr(p, k)[:owned[r]]
+                v += rng.normal(0, 0.4, owned[r])
+                if not periodic[a]:
+                    np.clip(v, lo[a], hi[a], out=v)
+                getattr(grid.particles, k)[grid.slots(r, owned[r])] = torch.from_numpy(v.copy()).cuda()
