@@ -335,6 +335,7 @@ int wait_counts(Plan* plan)
     if (ghosts > d.ghost_list_capacity) status |= kOverflowGhostList;
     t[kStatusWord] = status;
     plan->totals_cached = true;
+    plan->note_live_tiles();
     return DSDX_OK;
 }
 
@@ -345,6 +346,7 @@ int read_totals(Plan* plan, hipStream_t stream)
              "copy halo counts to the host");
     DSDX_HIP(hipStreamSynchronize(stream), "wait for halo counts");
     plan->totals_cached = true;
+    plan->note_live_tiles();
     return DSDX_OK;
 }
 

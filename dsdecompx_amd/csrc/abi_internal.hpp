@@ -49,6 +49,21 @@ struct Plan {
     uint32_t chunks = 0;
     uint32_t max_replica_chunks = 0;
     uint32_t tiles = 0;
+    // Up to 64 replicas: live owner tiles of the last build whose counts the
+    // host read (0 before the first).  Owner-major launches size their grid
+    // from it; their waves walk however many tiles the device counts hold.
+    uint32_t live_tiles = 0;
+
+    void note_live_tiles()
+    {
+        if (desc.size() > static_cast<size_t>(kWave) || !host_totals) return;
+        uint32_t n = 0;
+        for (size_t r = 0; r < desc.size(); ++r) {
+            const uint32_t* t = host_totals + r * kTotals;
+            if (!t[kStatusWord]) n += (t[kColOwners] + kOwnerTile - 1) / kOwnerTile;
+        }
+        live_tiles = n;
+    }
 
     bool classified = false;   // classify + scan of the current table are on the device
     bool built = false;        // ghosts and owner map written: reverse/forward usable
@@ -86,6 +101,9 @@ struct Plan {
         L.max_replica_chunks = max_replica_chunks;
         L.chunk_size = chunk_size();
         L.tiles = tiles;
+        L.tile_waves = tiles;
+        if (desc.size() <= static_cast<size_t>(kWave) && live_tiles)
+            L.tile_waves = std::min(tiles, live_tiles + live_tiles / 16 + 8);
         L.xcd_run = static_cast<uint32_t>(shape.xcd_run);
         L.descending = 0;
         return L;

@@ -117,6 +117,7 @@ struct Layout {
     uint32_t max_replica_chunks;    // largest chunks_r (scan work per block)
     uint32_t chunk_size;            // kWave * items (a power of two)
     uint32_t tiles;                 // owner tiles over all replicas
+    uint32_t tile_waves;            // waves of an owner-major launch (each walks tiles w, w + tile_waves, ...)
     uint32_t xcd_run;               // block renumbering (see xcd_block)
     uint32_t descending;            // this launch walks its chunks / tiles last to first
 };

@@ -385,6 +385,7 @@ int dsdx_grid_halo_build(dsdx_grid* grid, const dsdx_particles* particles, const
             return fail(DSDX_ECAPACITY, "a subdomain cannot hold its ghosts");
     for (uint32_t r = 0; r < grid->ranks; ++r)
         if (ghosts) ghosts[r] = grid->host_totals[r * kTotals + kColReceived];
+    grid->note_live_tiles();
     grid->built = true;
     return DSDX_OK;
 }

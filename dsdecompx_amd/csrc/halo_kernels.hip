@@ -165,13 +165,7 @@ __global__ __launch_bounds__(kMaxScanThreads) void scan_kernel(const ReplicaDesc
 }
 
 // ---------------------------------------------------------------------------
-// Pass 3 (route): one wave per chunk.  Walks the chunk's boundary particles
-// in owned order; for each of its send slots s (ascending) a particle gets
-// the ghost at receive block 25-s, position prefix[s][chunk] + rank in the
-// chunk.  Writes the owner -> ghost map (owner_index, owner_code,
-// image_start, owner_entry, ghost_list), from which the copy pass fills the
-// ghost records.  The capacity test is uniform over a replica: an
-// overflowing replica gets its status word set and no route.
+// Copies of registered fields, in units of T.
 template <class T>
 __device__ __forceinline__ void copy_units(T* dd, uint32_t units, uint64_t from, uint64_t to)
 {
@@ -190,6 +184,13 @@ __device__ __forceinline__ void copy_field_words(const FieldParams& fp, uint64_t
     }
 }
 
+// Pass 3 (route): one wave per chunk.  Walks the chunk's boundary particles
+// in owned order; for each of its send slots s (ascending) a particle gets
+// the ghost at receive block 25-s, position prefix[s][chunk] + rank in the
+// chunk.  Writes the owner -> ghost map (owner_index, owner_code,
+// image_start, owner_entry, ghost_list), from which the owner-major passes
+// work.  The capacity test is uniform over a replica: an overflowing replica
+// gets its status word set and no route.
 template <int ITEMS, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
     const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ chunk_replica, Layout L,
@@ -289,61 +290,85 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
     dsdx_particles p, FieldParams fp, const ReplicaDesc* __restrict__ desc,
     const uint32_t* __restrict__ tile_replica, Layout L, PlanBuffers B)
 {
-    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
-    if (!o.live) return;
-    const ReplicaDesc* d = o.d;
-    const uint64_t u = o.u;
-    const uint64_t off = d->offset;
-    const uint64_t i = off + B.owner_index[u];
-    const uint32_t E = B.owner_entry[u];
-    const uint32_t first = B.image_start[u];
-    const uint32_t slots = d->bp.lut[B.owner_code[u]];
-    const double px = B.bpos[E];
-    const double py = B.bpos[B.bpos_stride + E];
-    const double pz = B.bpos[2 * B.bpos_stride + E];
-    const double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
-    const double m = p.mass[i], q = p.charge[i];
-    const int32_t ty = p.type[i], id = p.global_id[i];
-    const uint32_t* glist = B.ghost_list + d->ghost_list_offset;
-    uint32_t k = 0;
-    for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k) {
-        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
-        const uint64_t j = ghost_slot(desc, d, s, glist[first + k]);
-        p.x[j] = px + d->bp.shift[s][0];
-        p.y[j] = py + d->bp.shift[s][1];
-        p.z[j] = pz + d->bp.shift[s][2];
-        p.vx[j] = vx; p.vy[j] = vy; p.vz[j] = vz;
-        if (p.fx) p.fx[j] = 0.0;
-        if (p.fy) p.fy[j] = 0.0;
-        if (p.fz) p.fz[j] = 0.0;
-        p.mass[j] = m; p.charge[j] = q;
-        p.type[j] = ty; p.global_id[j] = id;
-        if (fp.count) copy_field_words(fp, i, j);
-    }
+    __shared__ SlotTable s_table[WAVES];
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        const SlotTable& t = fill_slot_table(s_table[threadIdx.x / kWave], desc, o.d);
+        if (!o.live) return;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const uint32_t E = B.owner_entry[o.u];
+        const OwnerGhosts g = ghosts_of(desc, o, B);
+        const double px = B.bpos[E];
+        const double py = B.bpos[B.bpos_stride + E];
+        const double pz = B.bpos[2 * B.bpos_stride + E];
+        const double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
+        const double m = p.mass[i], q = p.charge[i];
+        const int32_t ty = p.type[i], id = p.global_id[i];
+        const FirstImages f = first_images(g);
+        auto put = [&](uint32_t s, uint32_t rel) {
+            const uint64_t j = t.ghost0[s] + rel;
+            p.x[j] = px + t.shift[s][0];
+            p.y[j] = py + t.shift[s][1];
+            p.z[j] = pz + t.shift[s][2];
+            p.vx[j] = vx; p.vy[j] = vy; p.vz[j] = vz;
+            if (p.fx) p.fx[j] = 0.0;
+            if (p.fy) p.fy[j] = 0.0;
+            if (p.fz) p.fz[j] = 0.0;
+            p.mass[j] = m; p.charge[j] = q;
+            p.type[j] = ty; p.global_id[j] = id;
+            if (fp.count) copy_field_words(fp, i, j);
+        };
+#pragma unroll
+        for (uint32_t k = 0; k < kPreloaded; ++k) {
+            if (k >= f.n) break;
+            put(f.slot[k], f.rel[k]);
+        }
+        uint32_t k = kPreloaded;
+        for (uint32_t rest = f.rest; rest; rest &= rest - 1u, ++k)
+            put(static_cast<uint32_t>(__builtin_ctz(rest)), g.list[k]);
+    });
 }
 
 // reverse_halo: f[i] = ((f[i] + g_s1) + g_s2) + ... over the owner's send
 // slots s1 < s2 < ..., the order of the Host reduction (src/halo.cpp:1357-1370
-// for n ascending).  No atomics: one lane sums one owner.
+// for n ascending).  No atomics: one lane sums one owner.  The forces of the
+// first kPreloaded images are loaded together, then added in slot order.
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void reverse_forces_kernel(
     double* __restrict__ fx, double* __restrict__ fy, double* __restrict__ fz,
     const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ tile_replica, Layout L,
     PlanBuffers B)
 {
-    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
-    if (!o.live) return;
-    const uint32_t* glist = B.ghost_list + o.d->ghost_list_offset + B.image_start[o.u];
-    const uint64_t i = o.d->offset + B.owner_index[o.u];
-    uint32_t slots = o.d->bp.lut[B.owner_code[o.u]];
-    double ax = fx[i], ay = fy[i], az = fz[i];
-    for (uint32_t k = 0; slots; ++k, slots &= slots - 1u) {
-        const uint64_t j = ghost_slot(desc, o.d, static_cast<uint32_t>(__builtin_ctz(slots)), glist[k]);
-        ax = ax + fx[j];
-        ay = ay + fy[j];
-        az = az + fz[j];
-    }
-    fx[i] = ax; fy[i] = ay; fz[i] = az;
+    __shared__ SlotTable s_table[WAVES];
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        const SlotTable& t = fill_slot_table(s_table[threadIdx.x / kWave], desc, o.d);
+        if (!o.live) return;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const OwnerGhosts g = ghosts_of(desc, o, B);
+        double ax = fx[i], ay = fy[i], az = fz[i];
+        const FirstImages f = first_images(g);
+        double gx[kPreloaded], gy[kPreloaded], gz[kPreloaded];
+#pragma unroll
+        for (uint32_t k = 0; k < kPreloaded; ++k) {
+            const uint64_t j = t.ghost0[f.slot[k]] + f.rel[k];
+            gx[k] = fx[j]; gy[k] = fy[j]; gz[k] = fz[j];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPreloaded; ++k) {
+            if (k < f.n) {
+                ax = ax + gx[k];
+                ay = ay + gy[k];
+                az = az + gz[k];
+            }
+        }
+        uint32_t k = kPreloaded;
+        for (uint32_t rest = f.rest; rest; rest &= rest - 1u, ++k) {
+            const uint64_t j = t.ghost0[__builtin_ctz(rest)] + g.list[k];
+            ax = ax + fx[j];
+            ay = ay + fy[j];
+            az = az + fz[j];
+        }
+        fx[i] = ax; fy[i] = ay; fz[i] = az;
+    });
 }
 
 // Up to C components at once: one pass over the images, each image's
@@ -391,40 +416,41 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_fields_kernel(
     TypedFields tf, const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ tile_replica,
     Layout L, PlanBuffers B)
 {
-    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
-    if (!o.live) return;
-    const uint64_t i = o.d->offset + B.owner_index[o.u];
-    const OwnerGhosts ghosts = ghosts_of(desc, o, B);
-    for (uint32_t f = 0; f < tf.count; ++f) {
-        const uint32_t cc = tf.components[f];
-        const int32_t op = tf.op[f];
-        switch (tf.type[f]) {
-        case DSDX_INT32:
-            if (op == DSDX_SUM)
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        if (!o.live) return;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const OwnerGhosts ghosts = ghosts_of(desc, o, B);
+        for (uint32_t f = 0; f < tf.count; ++f) {
+            const uint32_t cc = tf.components[f];
+            const int32_t op = tf.op[f];
+            switch (tf.type[f]) {
+            case DSDX_INT32:
+                if (op == DSDX_SUM)
+                    reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, ghosts);
+                else
+                    reduce_field<int32_t>(static_cast<int32_t*>(tf.data[f]), cc, op, i, ghosts);
+                break;
+            case DSDX_INT64:
+                if (op == DSDX_SUM)
+                    reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, ghosts);
+                else
+                    reduce_field<int64_t>(static_cast<int64_t*>(tf.data[f]), cc, op, i, ghosts);
+                break;
+            case DSDX_UINT32:
                 reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, ghosts);
-            else
-                reduce_field<int32_t>(static_cast<int32_t*>(tf.data[f]), cc, op, i, ghosts);
-            break;
-        case DSDX_INT64:
-            if (op == DSDX_SUM)
+                break;
+            case DSDX_UINT64:
                 reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, ghosts);
-            else
-                reduce_field<int64_t>(static_cast<int64_t*>(tf.data[f]), cc, op, i, ghosts);
-            break;
-        case DSDX_UINT32:
-            reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, ghosts);
-            break;
-        case DSDX_UINT64:
-            reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, ghosts);
-            break;
-        case DSDX_FLOAT32:
-            reduce_field<float>(static_cast<float*>(tf.data[f]), cc, op, i, ghosts);
-            break;
-        default:
-            reduce_field<double>(static_cast<double*>(tf.data[f]), cc, op, i, ghosts);
-            break;
+                break;
+            case DSDX_FLOAT32:
+                reduce_field<float>(static_cast<float*>(tf.data[f]), cc, op, i, ghosts);
+                break;
+            default:
+                reduce_field<double>(static_cast<double*>(tf.data[f]), cc, op, i, ghosts);
+                break;
+            }
         }
-    }
+    });
 }
 
 template <class T>
@@ -444,49 +470,66 @@ __global__ __launch_bounds__(WAVES * kWave) void forward_fields_kernel(
     FieldParams fp, const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ tile_replica,
     Layout L, PlanBuffers B)
 {
-    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
-    if (!o.live) return;
-    const uint64_t i = o.d->offset + B.owner_index[o.u];
-    const OwnerGhosts ghosts = ghosts_of(desc, o, B);
-    for (uint32_t f = 0; f < fp.count; ++f) {
-        if ((fp.wide >> f) & 1u)
-            forward_units(static_cast<uint64_t*>(fp.data[f]), fp.words[f] / 2, i, ghosts);
-        else
-            forward_units(static_cast<uint32_t*>(fp.data[f]), fp.words[f], i, ghosts);
-    }
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        if (!o.live) return;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const OwnerGhosts ghosts = ghosts_of(desc, o, B);
+        for (uint32_t f = 0; f < fp.count; ++f) {
+            if ((fp.wide >> f) & 1u)
+                forward_units(static_cast<uint64_t*>(fp.data[f]), fp.words[f] / 2, i, ghosts);
+            else
+                forward_units(static_cast<uint32_t*>(fp.data[f]), fp.words[f], i, ghosts);
+        }
+    });
 }
 
 // Reused-plan refresh: each ghost re-reads its owner; the slot of the k-th
-// ghost of an owner is the k-th set bit of its slot set.
+// ghost of an owner is the k-th set bit of its slot set.  The owner's values
+// and its first kPreloaded ghost-list entries are loaded before its stores.
 template <int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(
     dsdx_particles p, FieldParams fp, int32_t positions_only, const ReplicaDesc* __restrict__ desc,
     const uint32_t* __restrict__ tile_replica, Layout L, PlanBuffers B)
 {
-    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
-    if (!o.live) return;
-    const ReplicaDesc* d = o.d;
-    const uint64_t i = d->offset + B.owner_index[o.u];
-    const OwnerGhosts ghosts = ghosts_of(desc, o, B);
-    uint32_t slots = ghosts.slots;
-    const double px = p.x[i], py = p.y[i], pz = p.z[i];
-    for (uint32_t k = 0; slots; ++k) {
-        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(slots));
-        slots &= slots - 1u;
-        const uint64_t j = ghosts.at(k, s);
-        p.x[j] = px + d->bp.shift[s][0];
-        p.y[j] = py + d->bp.shift[s][1];
-        p.z[j] = pz + d->bp.shift[s][2];
-        if (p.fx) p.fx[j] = 0.0;
-        if (p.fy) p.fy[j] = 0.0;
-        if (p.fz) p.fz[j] = 0.0;
+    __shared__ SlotTable s_table[WAVES];
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        const SlotTable& t = fill_slot_table(s_table[threadIdx.x / kWave], desc, o.d);
+        if (!o.live) return;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const OwnerGhosts g = ghosts_of(desc, o, B);
+        const double px = p.x[i], py = p.y[i], pz = p.z[i];
+        double vx = 0.0, vy = 0.0, vz = 0.0, m = 0.0, q = 0.0;
+        int32_t ty = 0, id = 0;
         if (!positions_only) {
-            p.vx[j] = p.vx[i]; p.vy[j] = p.vy[i]; p.vz[j] = p.vz[i];
-            p.mass[j] = p.mass[i]; p.charge[j] = p.charge[i];
-            p.type[j] = p.type[i]; p.global_id[j] = p.global_id[i];
-            if (fp.count) copy_field_words(fp, i, j);
+            vx = p.vx[i]; vy = p.vy[i]; vz = p.vz[i];
+            m = p.mass[i]; q = p.charge[i];
+            ty = p.type[i]; id = p.global_id[i];
         }
-    }
+        const FirstImages f = first_images(g);
+        auto put = [&](uint32_t s, uint32_t rel) {
+            const uint64_t j = t.ghost0[s] + rel;
+            p.x[j] = px + t.shift[s][0];
+            p.y[j] = py + t.shift[s][1];
+            p.z[j] = pz + t.shift[s][2];
+            if (p.fx) p.fx[j] = 0.0;
+            if (p.fy) p.fy[j] = 0.0;
+            if (p.fz) p.fz[j] = 0.0;
+            if (!positions_only) {
+                p.vx[j] = vx; p.vy[j] = vy; p.vz[j] = vz;
+                p.mass[j] = m; p.charge[j] = q;
+                p.type[j] = ty; p.global_id[j] = id;
+                if (fp.count) copy_field_words(fp, i, j);
+            }
+        };
+#pragma unroll
+        for (uint32_t k = 0; k < kPreloaded; ++k) {
+            if (k >= f.n) break;
+            put(f.slot[k], f.rel[k]);
+        }
+        uint32_t k = kPreloaded;
+        for (uint32_t rest = f.rest; rest; rest &= rest - 1u, ++k)
+            put(static_cast<uint32_t>(__builtin_ctz(rest)), g.list[k]);
+    });
 }
 
 // Owner of every ghost: out[offset_q + rel] = owned index within the
@@ -496,17 +539,18 @@ __global__ __launch_bounds__(WAVES * kWave) void ghost_owners_kernel(
     int32_t* __restrict__ out, const ReplicaDesc* __restrict__ desc,
     const uint32_t* __restrict__ tile_replica, Layout L, PlanBuffers B)
 {
-    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
-    if (!o.live) return;
-    const uint32_t i = B.owner_index[o.u];
-    const OwnerGhosts ghosts = ghosts_of(desc, o, B);
-    uint32_t k = 0;
-    for (uint32_t rest = ghosts.slots; rest; rest &= rest - 1u, ++k) {
-        // ghost slot relative to the receiving replica's first owned slot
-        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
-        const ReplicaDesc* to = desc + o.d->dest[s];
-        out[ghosts.at(k, s) - to->owned] = static_cast<int32_t>(i);
-    }
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        if (!o.live) return;
+        const uint32_t i = B.owner_index[o.u];
+        const OwnerGhosts ghosts = ghosts_of(desc, o, B);
+        uint32_t k = 0;
+        for (uint32_t rest = ghosts.slots; rest; rest &= rest - 1u, ++k) {
+            // ghost slot relative to the receiving replica's first owned slot
+            const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
+            const ReplicaDesc* to = desc + o.d->dest[s];
+            out[ghosts.at(k, s) - to->owned] = static_cast<int32_t>(i);
+        }
+    });
 }
 
 // ---------------------------------------------------------------------------

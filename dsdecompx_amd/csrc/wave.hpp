@@ -242,7 +242,10 @@ inline uint32_t wave_blocks(uint32_t chunks, int waves)
 }
 
 inline dim3 chunk_grid(const Layout& L, int waves) { return dim3(wave_blocks(L.chunks, waves)); }
-inline dim3 tile_grid(const Layout& L, int waves) { return dim3(wave_blocks(L.tiles, waves)); }
+inline dim3 tile_grid(const Layout& L, int waves)
+{
+    return dim3(wave_blocks(L.tile_waves ? L.tile_waves : L.tiles, waves));
+}
 
 // The layout of one launch of a pass (its walking order from Shape::descending).
 inline Layout ordered(const Layout& L, const Shape& s, int bit)

@@ -79,53 +79,64 @@ __global__ __launch_bounds__(WAVES * kWave) void pack_halo_kernel(
 {
     __shared__ uint32_t s_delta[WAVES][kWave];
     const uint32_t* delta = wave_deltas<WAVES>(s_delta, B);
-    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
-    if (!o.live) return;
-    const BoxParams& bp = o.d->bp;
-    const uint64_t i = o.d->offset + B.owner_index[o.u];
-    const OwnerGhosts g = ghosts_of(desc, o, B);
-    double x, y, z;
-    if (fresh) {
-        const uint32_t E = B.owner_entry[o.u];
-        x = B.bpos[E];
-        y = B.bpos[B.bpos_stride + E];
-        z = B.bpos[2 * B.bpos_stride + E];
-    } else {
-        x = p.x[i];
-        y = p.y[i];
-        z = p.z[i];
-    }
-    if (positions_only) {
-        uint32_t k = 0;
-        for (uint32_t rest = g.slots; rest; rest &= rest - 1u, ++k) {
-            const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
-            double* r = reinterpret_cast<double*>(out) + 3ull * (g.list[k] + delta[s]);
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        if (!o.live) return;
+        const BoxParams& bp = o.d->bp;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const OwnerGhosts g = ghosts_of(desc, o, B);
+        double x, y, z;
+        if (fresh) {
+            const uint32_t E = B.owner_entry[o.u];
+            x = B.bpos[E];
+            y = B.bpos[B.bpos_stride + E];
+            z = B.bpos[2 * B.bpos_stride + E];
+        } else {
+            x = p.x[i];
+            y = p.y[i];
+            z = p.z[i];
+        }
+        // every image's record, first the preloaded ones
+        auto each_image = [&](auto&& put) {
+            const FirstImages f = first_images(g);
+#pragma unroll
+            for (uint32_t k = 0; k < kPreloaded; ++k) {
+                if (k >= f.n) break;
+                put(f.slot[k], f.rel[k] + delta[f.slot[k]]);
+            }
+            uint32_t k = kPreloaded;
+            for (uint32_t rest = f.rest; rest; rest &= rest - 1u, ++k) {
+                const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
+                put(s, g.list[k] + delta[s]);
+            }
+        };
+        if (positions_only) {
+            each_image([&](uint32_t s, uint32_t at) {
+                double* r = reinterpret_cast<double*>(out) + 3ull * at;
+                r[0] = x + bp.shift[s][0];
+                r[1] = y + bp.shift[s][1];
+                r[2] = z + bp.shift[s][2];
+            });
+            return;
+        }
+        const double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
+        const double m = p.mass[i], q = p.charge[i];
+        const uint64_t ids = static_cast<uint32_t>(p.type[i]) |
+                             (static_cast<uint64_t>(static_cast<uint32_t>(p.global_id[i])) << 32);
+        each_image([&](uint32_t s, uint32_t at) {
+            unsigned char* rec = out + static_cast<uint64_t>(at) * rf.stride;
+            double* r = reinterpret_cast<double*>(rec);
             r[0] = x + bp.shift[s][0];
             r[1] = y + bp.shift[s][1];
             r[2] = z + bp.shift[s][2];
-        }
-        return;
-    }
-    const double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
-    const double m = p.mass[i], q = p.charge[i];
-    const uint64_t ids = static_cast<uint32_t>(p.type[i]) |
-                         (static_cast<uint64_t>(static_cast<uint32_t>(p.global_id[i])) << 32);
-    uint32_t k = 0;
-    for (uint32_t rest = g.slots; rest; rest &= rest - 1u, ++k) {
-        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
-        unsigned char* rec = out + static_cast<uint64_t>(g.list[k] + delta[s]) * rf.stride;
-        double* r = reinterpret_cast<double*>(rec);
-        r[0] = x + bp.shift[s][0];
-        r[1] = y + bp.shift[s][1];
-        r[2] = z + bp.shift[s][2];
-        r[3] = vx; r[4] = vy; r[5] = vz;
-        r[6] = 0.0; r[7] = 0.0; r[8] = 0.0;
-        r[9] = m; r[10] = q;
-        reinterpret_cast<uint64_t*>(rec)[11] = ids;
-        for (uint32_t f = 0; f < rf.count; ++f)
-            copy_words(static_cast<const uint32_t*>(rf.data[f]) + i * rf.words[f],
-                       reinterpret_cast<uint32_t*>(rec + rf.offset[f]), rf.words[f]);
-    }
+            r[3] = vx; r[4] = vy; r[5] = vz;
+            r[6] = 0.0; r[7] = 0.0; r[8] = 0.0;
+            r[9] = m; r[10] = q;
+            reinterpret_cast<uint64_t*>(rec)[11] = ids;
+            for (uint32_t f = 0; f < rf.count; ++f)
+                copy_words(static_cast<const uint32_t*>(rf.data[f]) + i * rf.words[f],
+                           reinterpret_cast<uint32_t*>(rec + rf.offset[f]), rf.words[f]);
+        });
+    });
 }
 
 // unpack_kernel (src/cuda/unpack.cu:16-31): record k into particle dest + k,
@@ -231,21 +242,40 @@ __global__ __launch_bounds__(WAVES * kWave) void accumulate_reverse_kernel(
 {
     __shared__ uint32_t s_delta[WAVES][kWave];
     const uint32_t* delta = wave_deltas<WAVES>(s_delta, B);
-    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
-    if (!o.live) return;
-    const uint64_t i = o.d->offset + B.owner_index[o.u];
-    const OwnerGhosts g = ghosts_of(desc, o, B);
-    double ax = fx[i], ay = fy[i], az = fz[i];
-    uint32_t k = 0;
-    for (uint32_t rest = g.slots; rest; rest &= rest - 1u, ++k) {
-        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
-        const double* r = reinterpret_cast<const double*>(in + static_cast<uint64_t>(g.list[k] + delta[s]) *
-                                                                   kRecordBytes);
-        ax = ax + r[6];
-        ay = ay + r[7];
-        az = az + r[8];
-    }
-    fx[i] = ax; fy[i] = ay; fz[i] = az;
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        if (!o.live) return;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const OwnerGhosts g = ghosts_of(desc, o, B);
+        double ax = fx[i], ay = fy[i], az = fz[i];
+        auto forces = [&](uint32_t at) {
+            return reinterpret_cast<const double*>(in + static_cast<uint64_t>(at) * kRecordBytes) + 6;
+        };
+        // the forces of the first kPreloaded images loaded together, then
+        // added in slot order
+        const FirstImages f = first_images(g);
+        double gx[kPreloaded], gy[kPreloaded], gz[kPreloaded];
+#pragma unroll
+        for (uint32_t k = 0; k < kPreloaded; ++k) {
+            const double* r = forces(f.rel[k] + delta[f.slot[k]]);
+            gx[k] = r[0]; gy[k] = r[1]; gz[k] = r[2];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kPreloaded; ++k) {
+            if (k < f.n) {
+                ax = ax + gx[k];
+                ay = ay + gy[k];
+                az = az + gz[k];
+            }
+        }
+        uint32_t k = kPreloaded;
+        for (uint32_t rest = f.rest; rest; rest &= rest - 1u, ++k) {
+            const double* r = forces(g.list[k] + delta[__builtin_ctz(rest)]);
+            ax = ax + r[0];
+            ay = ay + r[1];
+            az = az + r[2];
+        }
+        fx[i] = ax; fy[i] = ay; fz[i] = az;
+    });
 }
 
 // pack_reverse_fields_kernel / scatter_fields_kernel
@@ -282,18 +312,19 @@ __global__ __launch_bounds__(WAVES * kWave) void gather_field_records_kernel(
 {
     __shared__ uint32_t s_delta[WAVES][kWave];
     const uint32_t* delta = wave_deltas<WAVES>(s_delta, B);
-    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
-    if (!o.live) return;
-    const uint64_t i = o.d->offset + B.owner_index[o.u];
-    const OwnerGhosts g = ghosts_of(desc, o, B);
-    uint32_t k = 0;
-    for (uint32_t rest = g.slots; rest; rest &= rest - 1u, ++k) {
-        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
-        unsigned char* rec = out + static_cast<uint64_t>(g.list[k] + delta[s]) * rf.stride;
-        for (uint32_t f = 0; f < rf.count; ++f)
-            copy_words(static_cast<const uint32_t*>(rf.data[f]) + i * rf.words[f],
-                       reinterpret_cast<uint32_t*>(rec + rf.offset[f]), rf.words[f]);
-    }
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        if (!o.live) return;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const OwnerGhosts g = ghosts_of(desc, o, B);
+        uint32_t k = 0;
+        for (uint32_t rest = g.slots; rest; rest &= rest - 1u, ++k) {
+            const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
+            unsigned char* rec = out + static_cast<uint64_t>(g.list[k] + delta[s]) * rf.stride;
+            for (uint32_t f = 0; f < rf.count; ++f)
+                copy_words(static_cast<const uint32_t*>(rf.data[f]) + i * rf.words[f],
+                           reinterpret_cast<uint32_t*>(rec + rf.offset[f]), rf.words[f]);
+        }
+    });
 }
 
 template <class T>
@@ -325,41 +356,42 @@ __global__ __launch_bounds__(WAVES * kWave) void reduce_field_records_kernel(
 {
     __shared__ uint32_t s_delta[WAVES][kWave];
     const uint32_t* delta = wave_deltas<WAVES>(s_delta, B);
-    const OwnerLane o = owner_of_lane<WAVES>(desc, tile_replica, L, B);
-    if (!o.live) return;
-    const uint64_t i = o.d->offset + B.owner_index[o.u];
-    const OwnerGhosts g = ghosts_of(desc, o, B);
-    for (uint32_t f = 0; f < rf.count; ++f) {
-        const uint32_t c = rf.components[f];
-        const int32_t op = rf.op[f];
-        const uint32_t off = rf.offset[f];
-        switch (rf.type[f]) {
-        case DSDX_INT32:
-            if (op == DSDX_SUM)
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        if (!o.live) return;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const OwnerGhosts g = ghosts_of(desc, o, B);
+        for (uint32_t f = 0; f < rf.count; ++f) {
+            const uint32_t c = rf.components[f];
+            const int32_t op = rf.op[f];
+            const uint32_t off = rf.offset[f];
+            switch (rf.type[f]) {
+            case DSDX_INT32:
+                if (op == DSDX_SUM)
+                    reduce_records<uint32_t>(static_cast<uint32_t*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
+                else
+                    reduce_records<int32_t>(static_cast<int32_t*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
+                break;
+            case DSDX_INT64:
+                if (op == DSDX_SUM)
+                    reduce_records<uint64_t>(static_cast<uint64_t*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
+                else
+                    reduce_records<int64_t>(static_cast<int64_t*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
+                break;
+            case DSDX_UINT32:
                 reduce_records<uint32_t>(static_cast<uint32_t*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
-            else
-                reduce_records<int32_t>(static_cast<int32_t*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
-            break;
-        case DSDX_INT64:
-            if (op == DSDX_SUM)
+                break;
+            case DSDX_UINT64:
                 reduce_records<uint64_t>(static_cast<uint64_t*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
-            else
-                reduce_records<int64_t>(static_cast<int64_t*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
-            break;
-        case DSDX_UINT32:
-            reduce_records<uint32_t>(static_cast<uint32_t*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
-            break;
-        case DSDX_UINT64:
-            reduce_records<uint64_t>(static_cast<uint64_t*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
-            break;
-        case DSDX_FLOAT32:
-            reduce_records<float>(static_cast<float*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
-            break;
-        default:
-            reduce_records<double>(static_cast<double*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
-            break;
+                break;
+            case DSDX_FLOAT32:
+                reduce_records<float>(static_cast<float*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
+                break;
+            default:
+                reduce_records<double>(static_cast<double*>(rf.data[f]), c, op, i, g, delta, in, rf.stride, off);
+                break;
+            }
         }
-    }
+    });
 }
 
 // The self messages of a loopback box (src/backends/mpi_backend.cpp:366-375):

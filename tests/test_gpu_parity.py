@@ -261,6 +261,35 @@ def test_multi_step_cfg1(gpu):
         compare_particles(dev, op, n + g, f"step {step}")
 
 
+def test_one_plan_across_sizes(gpu):
+    """One plan rebuilt for 1,500, then 60,000, then 1,500 particles in the
+    same box.  The owner-major passes size their launch from the previous
+    build's owner count, so the second build's copy runs with far fewer waves
+    than owner tiles (each wave walks several) and the third with more (the
+    extra waves have no tile); every array and reverse sum bit-exact."""
+    import torch
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    L, w = 32.0, 2.8
+    box_o = po.Box(upper=(L,) * 3, width=w)
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
+    rng = np.random.default_rng(321)
+    for step, n in enumerate((1500, 60000, 1500)):
+        say("sizes", n)
+        d = synth(n, L, 400 + step)
+        ora = po.Oracle(box_o)
+        op = to_oracle(d, n * 3)
+        dev = to_device(d, n * 3)
+        g = ora.exchange_halo(op, n)
+        assert halo.exchange_halo(dev, n) == g
+        f = rng.uniform(-1, 1, (3, n + g))
+        for a, k in enumerate(("fx", "fy", "fz")):
+            getattr(op, k)[:n + g] = f[a]
+            getattr(dev, k)[:n + g] = torch.from_numpy(f[a]).cuda()
+        ora.reverse_halo(op, n)
+        halo.reverse_halo(dev, n)
+        compare_particles(dev, op, n + g, f"sizes n={n}")
+
+
 @pytest.mark.parametrize("case", [c for c in CASES if c[0] in ("cfg1", "z_nonperiodic",
                                                                  "shifted_box", "empty")],
                          ids=lambda c: c[0])
