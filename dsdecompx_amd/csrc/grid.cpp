@@ -233,6 +233,12 @@ int dsdx_grid_create(const dsdx_box* box, const int32_t* dims, uint64_t capacity
         for (int i = 0; i <= dims[a]; ++i)
             g->mp.cuts[a][i] = box->lower[a] + extent * static_cast<double>(i) / static_cast<double>(dims[a]);
     }
+    // Chunks of 8 items per lane: the migration's count and scatter passes
+    // hold every item of a chunk in registers, and at 16 items they run at
+    // 4-5 waves per SIMD against 7-8 (2x2x2 grid of 16 M particles: count
+    // 103 -> 82 us, scatter 683 -> 651 us per step; the halo passes are no
+    // slower).
+    g->shape.items = 8;
     int rc = shape_from_env(&g->shape);
     g->desc.resize(g->ranks);
     for (uint32_t r = 0; r < g->ranks; ++r) rank_desc(g, r, &g->desc[r]);
