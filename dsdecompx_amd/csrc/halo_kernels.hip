@@ -371,40 +371,65 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_forces_kernel(
     });
 }
 
+// The particle slots of an owner's images: the first kPreloaded from the
+// entries loaded with the owner, the rest from its ghost list.
+struct Images {
+    const OwnerGhosts& g;
+    const FirstImages& f;
+    const SlotTable& t;
+    __device__ uint64_t first(uint32_t k) const { return t.ghost0[f.slot[k]] + f.rel[k]; }
+    __device__ uint64_t later(uint32_t k, uint32_t s) const { return t.ghost0[s] + g.list[k]; }
+};
+
 // Up to C components at once: one pass over the images, each image's
-// components read together; every component still accumulates its images in
+// components read together, those of the first kPreloaded images all before
+// the first reduction; every component still accumulates its images in
 // ascending send slot.
 template <class T, int C>
-__device__ __forceinline__ void reduce_components(T* data, int32_t op, uint64_t i, const OwnerGhosts& g)
+__device__ __forceinline__ void reduce_components(T* data, int32_t op, uint64_t i, const Images& im)
 {
     T acc[C];
 #pragma unroll
     for (int e = 0; e < C; ++e) acc[e] = data[i * C + e];
-    uint32_t k = 0;
-    for (uint32_t rest = g.slots; rest; rest &= rest - 1u, ++k) {
-        const T* v = data + g.at(k, static_cast<uint32_t>(__builtin_ctz(rest))) * C;
+    T v[kPreloaded][C];
 #pragma unroll
-        for (int e = 0; e < C; ++e) acc[e] = reduce_value<T>(acc[e], v[e], op);
+    for (uint32_t k = 0; k < kPreloaded; ++k) {
+        const T* src = data + im.first(k) * C;
+#pragma unroll
+        for (int e = 0; e < C; ++e) v[k][e] = src[e];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPreloaded; ++k) {
+        if (k < im.f.n) {
+#pragma unroll
+            for (int e = 0; e < C; ++e) acc[e] = reduce_value<T>(acc[e], v[k][e], op);
+        }
+    }
+    uint32_t k = kPreloaded;
+    for (uint32_t rest = im.f.rest; rest; rest &= rest - 1u, ++k) {
+        const T* src = data + im.later(k, static_cast<uint32_t>(__builtin_ctz(rest))) * C;
+#pragma unroll
+        for (int e = 0; e < C; ++e) acc[e] = reduce_value<T>(acc[e], src[e], op);
     }
 #pragma unroll
     for (int e = 0; e < C; ++e) data[i * C + e] = acc[e];
 }
 
 template <class T>
-__device__ void reduce_field(T* data, uint32_t c, int32_t op, uint64_t i, const OwnerGhosts& g)
+__device__ __forceinline__ void reduce_field(T* data, uint32_t c, int32_t op, uint64_t i, const Images& im)
 {
     switch (c) {
-    case 1: reduce_components<T, 1>(data, op, i, g); return;
-    case 2: reduce_components<T, 2>(data, op, i, g); return;
-    case 3: reduce_components<T, 3>(data, op, i, g); return;
-    case 4: reduce_components<T, 4>(data, op, i, g); return;
+    case 1: reduce_components<T, 1>(data, op, i, im); return;
+    case 2: reduce_components<T, 2>(data, op, i, im); return;
+    case 3: reduce_components<T, 3>(data, op, i, im); return;
+    case 4: reduce_components<T, 4>(data, op, i, im); return;
     default: break;
     }
     for (uint32_t e = 0; e < c; ++e) {
         T acc = data[i * c + e];
         uint32_t k = 0;
-        for (uint32_t rest = g.slots; rest; rest &= rest - 1u, ++k)
-            acc = reduce_value<T>(acc, data[g.at(k, static_cast<uint32_t>(__builtin_ctz(rest))) * c + e], op);
+        for (uint32_t rest = im.g.slots; rest; rest &= rest - 1u, ++k)
+            acc = reduce_value<T>(acc, data[im.later(k, static_cast<uint32_t>(__builtin_ctz(rest))) * c + e], op);
         data[i * c + e] = acc;
     }
 }
@@ -416,37 +441,41 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_fields_kernel(
     TypedFields tf, const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ tile_replica,
     Layout L, PlanBuffers B)
 {
+    __shared__ SlotTable s_table[WAVES];
     for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        const SlotTable& t = fill_slot_table(s_table[threadIdx.x / kWave], desc, o.d);
         if (!o.live) return;
         const uint64_t i = o.d->offset + B.owner_index[o.u];
-        const OwnerGhosts ghosts = ghosts_of(desc, o, B);
-        for (uint32_t f = 0; f < tf.count; ++f) {
-            const uint32_t cc = tf.components[f];
-            const int32_t op = tf.op[f];
-            switch (tf.type[f]) {
+        const OwnerGhosts g = ghosts_of(desc, o, B);
+        const FirstImages f = first_images(g);
+        const Images ghosts{g, f, t};
+        for (uint32_t fi = 0; fi < tf.count; ++fi) {
+            const uint32_t cc = tf.components[fi];
+            const int32_t op = tf.op[fi];
+            switch (tf.type[fi]) {
             case DSDX_INT32:
                 if (op == DSDX_SUM)
-                    reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, ghosts);
+                    reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[fi]), cc, op, i, ghosts);
                 else
-                    reduce_field<int32_t>(static_cast<int32_t*>(tf.data[f]), cc, op, i, ghosts);
+                    reduce_field<int32_t>(static_cast<int32_t*>(tf.data[fi]), cc, op, i, ghosts);
                 break;
             case DSDX_INT64:
                 if (op == DSDX_SUM)
-                    reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, ghosts);
+                    reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[fi]), cc, op, i, ghosts);
                 else
-                    reduce_field<int64_t>(static_cast<int64_t*>(tf.data[f]), cc, op, i, ghosts);
+                    reduce_field<int64_t>(static_cast<int64_t*>(tf.data[fi]), cc, op, i, ghosts);
                 break;
             case DSDX_UINT32:
-                reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[f]), cc, op, i, ghosts);
+                reduce_field<uint32_t>(static_cast<uint32_t*>(tf.data[fi]), cc, op, i, ghosts);
                 break;
             case DSDX_UINT64:
-                reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[f]), cc, op, i, ghosts);
+                reduce_field<uint64_t>(static_cast<uint64_t*>(tf.data[fi]), cc, op, i, ghosts);
                 break;
             case DSDX_FLOAT32:
-                reduce_field<float>(static_cast<float*>(tf.data[f]), cc, op, i, ghosts);
+                reduce_field<float>(static_cast<float*>(tf.data[fi]), cc, op, i, ghosts);
                 break;
             default:
-                reduce_field<double>(static_cast<double*>(tf.data[f]), cc, op, i, ghosts);
+                reduce_field<double>(static_cast<double*>(tf.data[fi]), cc, op, i, ghosts);
                 break;
             }
         }

@@ -52,10 +52,14 @@ def _snapshot(b):
     return {k: host(getattr(b, k)).copy() for k in F64 + I32}
 
 
-def test_batch_matches_oracle_and_loop(gpu):
+# 12 replicas: the owner-major passes walk the live tiles they count on the
+# device; 80 replicas (more than a wave has lanes): the tiles numbered on the
+# host, one per wave.
+@pytest.mark.parametrize("count,seed,nmax", [(12, 5, 30000), (80, 31, 3000)], ids=["12", "80"])
+def test_batch_matches_oracle_and_loop(gpu, count, seed, nmax):
     torch = gpu
     from dsdecompx_amd.batch import LoopedReplicas, ReplicaBatch
-    specs = _specs(12, 5)
+    specs = _specs(count, seed, nmax=nmax)
     say("batch", [s.owned for s in specs])
     b = ReplicaBatch(specs)
     _fill(torch, b, 6)
