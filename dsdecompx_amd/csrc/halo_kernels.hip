@@ -280,6 +280,33 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
     }
 }
 
+// The particle slots of an owner's images: the first kPreloaded from the
+// entries loaded with the owner, the rest from its ghost list.
+struct Images {
+    const OwnerGhosts& g;
+    const FirstImages& f;
+    const SlotTable& t;
+    __device__ uint64_t first(uint32_t k) const { return t.ghost0[f.slot[k]] + f.rel[k]; }
+    __device__ uint64_t later(uint32_t k, uint32_t s) const { return t.ghost0[s] + g.list[k]; }
+};
+
+// Calls put(send slot, particle slot) for every image of an owner, in
+// ascending send slot.
+template <class Put>
+__device__ __forceinline__ void each_image(const Images& im, Put&& put)
+{
+#pragma unroll
+    for (uint32_t k = 0; k < kPreloaded; ++k) {
+        if (k >= im.f.n) break;
+        put(im.f.slot[k], im.first(k));
+    }
+    uint32_t k = kPreloaded;
+    for (uint32_t rest = im.f.rest; rest; rest &= rest - 1u, ++k) {
+        const uint32_t slot = static_cast<uint32_t>(__builtin_ctz(rest));
+        put(slot, im.later(k, slot));
+    }
+}
+
 // Pass 4 (copy): one wave per owner tile, one lane per boundary owner, in
 // owned order (one ascending sweep over the owner arrays).  Each owner's
 // ghosts get position + shift[slot] (the owner's position comes from the
@@ -304,8 +331,7 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
         const double m = p.mass[i], q = p.charge[i];
         const int32_t ty = p.type[i], id = p.global_id[i];
         const FirstImages f = first_images(g);
-        auto put = [&](uint32_t s, uint32_t rel) {
-            const uint64_t j = t.ghost0[s] + rel;
+        each_image(Images{g, f, t}, [&](uint32_t s, uint64_t j) {
             p.x[j] = px + t.shift[s][0];
             p.y[j] = py + t.shift[s][1];
             p.z[j] = pz + t.shift[s][2];
@@ -316,15 +342,7 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
             p.mass[j] = m; p.charge[j] = q;
             p.type[j] = ty; p.global_id[j] = id;
             if (fp.count) copy_field_words(fp, i, j);
-        };
-#pragma unroll
-        for (uint32_t k = 0; k < kPreloaded; ++k) {
-            if (k >= f.n) break;
-            put(f.slot[k], f.rel[k]);
-        }
-        uint32_t k = kPreloaded;
-        for (uint32_t rest = f.rest; rest; rest &= rest - 1u, ++k)
-            put(static_cast<uint32_t>(__builtin_ctz(rest)), g.list[k]);
+        });
     });
 }
 
@@ -346,10 +364,11 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_forces_kernel(
         const OwnerGhosts g = ghosts_of(desc, o, B);
         double ax = fx[i], ay = fy[i], az = fz[i];
         const FirstImages f = first_images(g);
+        const Images im{g, f, t};
         double gx[kPreloaded], gy[kPreloaded], gz[kPreloaded];
 #pragma unroll
         for (uint32_t k = 0; k < kPreloaded; ++k) {
-            const uint64_t j = t.ghost0[f.slot[k]] + f.rel[k];
+            const uint64_t j = im.first(k);
             gx[k] = fx[j]; gy[k] = fy[j]; gz[k] = fz[j];
         }
 #pragma unroll
@@ -362,7 +381,7 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_forces_kernel(
         }
         uint32_t k = kPreloaded;
         for (uint32_t rest = f.rest; rest; rest &= rest - 1u, ++k) {
-            const uint64_t j = t.ghost0[__builtin_ctz(rest)] + g.list[k];
+            const uint64_t j = im.later(k, static_cast<uint32_t>(__builtin_ctz(rest)));
             ax = ax + fx[j];
             ay = ay + fy[j];
             az = az + fz[j];
@@ -370,16 +389,6 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_forces_kernel(
         fx[i] = ax; fy[i] = ay; fz[i] = az;
     });
 }
-
-// The particle slots of an owner's images: the first kPreloaded from the
-// entries loaded with the owner, the rest from its ghost list.
-struct Images {
-    const OwnerGhosts& g;
-    const FirstImages& f;
-    const SlotTable& t;
-    __device__ uint64_t first(uint32_t k) const { return t.ghost0[f.slot[k]] + f.rel[k]; }
-    __device__ uint64_t later(uint32_t k, uint32_t s) const { return t.ghost0[s] + g.list[k]; }
-};
 
 // Up to C components at once: one pass over the images, each image's
 // components read together, those of the first kPreloaded images all before
@@ -482,14 +491,33 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_fields_kernel(
     });
 }
 
-template <class T>
-__device__ __forceinline__ void forward_units(T* dd, uint32_t units, uint64_t i, const OwnerGhosts& ghosts)
+
+// A field of U units: the owner's units loaded once, then stored to every image.
+template <class T, int U>
+__device__ __forceinline__ void forward_record(T* dd, uint64_t i, const Images& im)
 {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = dd[i * U + u];
+    each_image(im, [&](uint32_t, uint64_t j) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) dd[j * U + u] = v[u];
+    });
+}
+
+template <class T>
+__device__ __forceinline__ void forward_units(T* dd, uint32_t units, uint64_t i, const Images& im)
+{
+    switch (units) {
+    case 1: forward_record<T, 1>(dd, i, im); return;
+    case 2: forward_record<T, 2>(dd, i, im); return;
+    case 3: forward_record<T, 3>(dd, i, im); return;
+    case 4: forward_record<T, 4>(dd, i, im); return;
+    default: break;
+    }
     for (uint32_t c = 0; c < units; ++c) {
         const T value = dd[i * units + c];
-        uint32_t k = 0;
-        for (uint32_t rest = ghosts.slots; rest; rest &= rest - 1u, ++k)
-            dd[ghosts.at(k, static_cast<uint32_t>(__builtin_ctz(rest))) * units + c] = value;
+        each_image(im, [&](uint32_t, uint64_t j) { dd[j * units + c] = value; });
     }
 }
 
@@ -499,15 +527,19 @@ __global__ __launch_bounds__(WAVES * kWave) void forward_fields_kernel(
     FieldParams fp, const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ tile_replica,
     Layout L, PlanBuffers B)
 {
+    __shared__ SlotTable s_table[WAVES];
     for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        const SlotTable& t = fill_slot_table(s_table[threadIdx.x / kWave], desc, o.d);
         if (!o.live) return;
         const uint64_t i = o.d->offset + B.owner_index[o.u];
-        const OwnerGhosts ghosts = ghosts_of(desc, o, B);
-        for (uint32_t f = 0; f < fp.count; ++f) {
-            if ((fp.wide >> f) & 1u)
-                forward_units(static_cast<uint64_t*>(fp.data[f]), fp.words[f] / 2, i, ghosts);
+        const OwnerGhosts g = ghosts_of(desc, o, B);
+        const FirstImages f = first_images(g);
+        const Images ghosts{g, f, t};
+        for (uint32_t fi = 0; fi < fp.count; ++fi) {
+            if ((fp.wide >> fi) & 1u)
+                forward_units(static_cast<uint64_t*>(fp.data[fi]), fp.words[fi] / 2, i, ghosts);
             else
-                forward_units(static_cast<uint32_t*>(fp.data[f]), fp.words[f], i, ghosts);
+                forward_units(static_cast<uint32_t*>(fp.data[fi]), fp.words[fi], i, ghosts);
         }
     });
 }
@@ -535,8 +567,7 @@ __global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(
             ty = p.type[i]; id = p.global_id[i];
         }
         const FirstImages f = first_images(g);
-        auto put = [&](uint32_t s, uint32_t rel) {
-            const uint64_t j = t.ghost0[s] + rel;
+        each_image(Images{g, f, t}, [&](uint32_t s, uint64_t j) {
             p.x[j] = px + t.shift[s][0];
             p.y[j] = py + t.shift[s][1];
             p.z[j] = pz + t.shift[s][2];
@@ -549,15 +580,7 @@ __global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(
                 p.type[j] = ty; p.global_id[j] = id;
                 if (fp.count) copy_field_words(fp, i, j);
             }
-        };
-#pragma unroll
-        for (uint32_t k = 0; k < kPreloaded; ++k) {
-            if (k >= f.n) break;
-            put(f.slot[k], f.rel[k]);
-        }
-        uint32_t k = kPreloaded;
-        for (uint32_t rest = f.rest; rest; rest &= rest - 1u, ++k)
-            put(static_cast<uint32_t>(__builtin_ctz(rest)), g.list[k]);
+        });
     });
 }
 

@@ -290,6 +290,46 @@ def test_one_plan_across_sizes(gpu):
         compare_particles(dev, op, n + g, f"sizes n={n}")
 
 
+@pytest.mark.parametrize("items,waves,descending", [(16, 4, 63), (4, 2, 0), (8, 8, 24)])
+def test_launch_shapes_and_walk_orders(gpu, monkeypatch, items, waves, descending):
+    """Launch shape and walk order (DSDX_ITEMS, DSDX_WAVES, DSDX_DESCENDING,
+    read when a plan is made) change no result: migrate, build, reverse and
+    a full refresh bit-exact with the oracle with every pass walking last to
+    first, and at other chunk sizes."""
+    import torch
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    monkeypatch.setenv("DSDX_ITEMS", str(items))
+    monkeypatch.setenv("DSDX_WAVES", str(waves))
+    monkeypatch.setenv("DSDX_DESCENDING", str(descending))
+    n, L, w = 20000, 27.0, 2.8
+    say("shape", items, waves, descending)
+    d = synth(n, L, 500 + items)
+    rng = np.random.default_rng(501)
+    for k in ("x", "y", "z"):
+        d[k] = d[k] + rng.normal(0, 0.3, n)
+    ora = po.Oracle(po.Box(upper=(L,) * 3, width=w))
+    op = to_oracle(d, n * 3)
+    dev = to_device(d, n * 3)
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
+    ora.migrate(op, n)
+    halo.migrate(dev, n)
+    g = ora.exchange_halo(op, n)
+    assert halo.exchange_halo(dev, n) == g
+    f = rng.uniform(-1, 1, (3, n + g))
+    for a, k in enumerate(("fx", "fy", "fz")):
+        getattr(op, k)[:n + g] = f[a]
+        getattr(dev, k)[:n + g] = torch.from_numpy(f[a]).cuda()
+    ora.reverse_halo(op, n)
+    halo.reverse_halo(dev, n)
+    compare_particles(dev, op, n + g, f"shape {items}x{waves} order {descending}")
+    # a full refresh rewrites every ghost from its owner, as a rebuild does
+    op.vx[:n] += 1.0
+    dev.vx[:n] = torch.from_numpy(op.vx[:n].copy()).cuda()
+    halo.refresh(dev, n, positions_only=False)
+    ora.write_ghosts(op, n)
+    compare_particles(dev, op, n + g, f"refresh {items}x{waves} order {descending}")
+
+
 @pytest.mark.parametrize("case", [c for c in CASES if c[0] in ("cfg1", "z_nonperiodic",
                                                                  "shifted_box", "empty")],
                          ids=lambda c: c[0])
