@@ -379,7 +379,8 @@ int dsdx_loopback_exchange(dsdx_plan* plan, const void* send, void* recv,
  * [r * capacity_per_rank, (r + 1) * capacity_per_rank), owned first, then its
  * ghosts; a registered field's array (dsdx_field, AoS of `components`
  * values per slot) has the same slots.  All 13 particle arrays are required.
- * At most 64 subdomains. */
+ * At most 64 subdomains.  A grid's kernels take chunks of 8 items per lane
+ * (DSDX_ITEMS and the other shape variables apply as for a plan). */
 typedef struct dsdx_grid dsdx_grid;
 
 int dsdx_grid_create(const dsdx_box* box, const int32_t* dims, uint64_t capacity_per_rank,
