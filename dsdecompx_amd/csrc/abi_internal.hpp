@@ -1,5 +1,6 @@
-// abi_internal.hpp -- host-side plan state shared by the C ABI (abi.cpp) and
-// the autotuner (autotune.cpp).
+// abi_internal.hpp -- host-side plan state shared by the C ABI's single-box
+// and batched entry points (abi.cpp), process grids (grid.cpp) and
+// communication records (wire.cpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
