@@ -91,9 +91,20 @@ def cpu_baseline(name, n, L, w, sigma, periodic, steps, seed=7):
             timed += 1
     ms = 1e3 * total / timed
     return {"value": n / (ms * 1e3), "unit": "Mparticles/s", "cores": 1, "kind": "port",
-            "ms_per_step": ms,
+            "ms_per_step": ms, "cpu_model": _cpu_model(),
             "sample": f"{name} workload ({n} particles), {timed} timed steps after 1 warm-up, "
                       f"oracle (C, -O2, one thread) on {os.cpu_count()} visible host cores"}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def measure_abi(args, n, L, w, sigma, periodic):
