@@ -118,6 +118,7 @@ struct Layout {
     uint32_t chunk_size;            // kWave * items (a power of two)
     uint32_t tiles;                 // owner tiles over all replicas
     uint32_t tile_waves;            // waves of an owner-major launch (each walks tiles w, w + tile_waves, ...)
+    uint32_t stream_owners;         // owner gathers load nontemporally (see owner_load)
     uint32_t xcd_run;               // block renumbering (see xcd_block)
     uint32_t descending;            // this launch walks its chunks / tiles last to first
 };
@@ -180,6 +181,7 @@ struct RecordFields {
 };
 
 constexpr uint32_t kRecordBytes = 96;   // ParticleRecord: 11 doubles, 2 int32
+constexpr uint64_t kMallBytes = 256ull << 20;   // MI355X memory-side (Infinity) cache
 
 // Runs of one loopback exchange: block b copies words[b] 8-byte words.
 struct BlockMap {

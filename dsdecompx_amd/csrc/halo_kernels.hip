@@ -312,7 +312,7 @@ __device__ __forceinline__ void each_image(const Images& im, Put&& put)
 // ghosts get position + shift[slot] (the owner's position comes from the
 // classify pass, densely), velocity, mass, charge, type, id and the
 // registered fields of the owner, forces zero (src/halo.cpp:1232-1262).
-template <int WAVES>
+template <int WAVES, bool STREAM>
 __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
     dsdx_particles p, FieldParams fp, const ReplicaDesc* __restrict__ desc,
     const uint32_t* __restrict__ tile_replica, Layout L, PlanBuffers B)
@@ -327,9 +327,9 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
         const double px = B.bpos[E];
         const double py = B.bpos[B.bpos_stride + E];
         const double pz = B.bpos[2 * B.bpos_stride + E];
-        const double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
-        const double m = p.mass[i], q = p.charge[i];
-        const int32_t ty = p.type[i], id = p.global_id[i];
+        const double vx = owner_load<STREAM>(p.vx + i), vy = owner_load<STREAM>(p.vy + i), vz = owner_load<STREAM>(p.vz + i);
+        const double m = owner_load<STREAM>(p.mass + i), q = owner_load<STREAM>(p.charge + i);
+        const int32_t ty = owner_load<STREAM>(p.type + i), id = owner_load<STREAM>(p.global_id + i);
         const FirstImages f = first_images(g);
         each_image(Images{g, f, t}, [&](uint32_t s, uint64_t j) {
             p.x[j] = px + t.shift[s][0];
@@ -547,7 +547,7 @@ __global__ __launch_bounds__(WAVES * kWave) void forward_fields_kernel(
 // Reused-plan refresh: each ghost re-reads its owner; the slot of the k-th
 // ghost of an owner is the k-th set bit of its slot set.  The owner's values
 // and its first kPreloaded ghost-list entries are loaded before its stores.
-template <int WAVES>
+template <int WAVES, bool STREAM>
 __global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(
     dsdx_particles p, FieldParams fp, int32_t positions_only, const ReplicaDesc* __restrict__ desc,
     const uint32_t* __restrict__ tile_replica, Layout L, PlanBuffers B)
@@ -562,9 +562,9 @@ __global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(
         double vx = 0.0, vy = 0.0, vz = 0.0, m = 0.0, q = 0.0;
         int32_t ty = 0, id = 0;
         if (!positions_only) {
-            vx = p.vx[i]; vy = p.vy[i]; vz = p.vz[i];
-            m = p.mass[i]; q = p.charge[i];
-            ty = p.type[i]; id = p.global_id[i];
+            vx = owner_load<STREAM>(p.vx + i); vy = owner_load<STREAM>(p.vy + i); vz = owner_load<STREAM>(p.vz + i);
+            m = owner_load<STREAM>(p.mass + i); q = owner_load<STREAM>(p.charge + i);
+            ty = owner_load<STREAM>(p.type + i); id = owner_load<STREAM>(p.global_id + i);
         }
         const FirstImages f = first_images(g);
         each_image(Images{g, f, t}, [&](uint32_t s, uint64_t j) {
@@ -691,9 +691,12 @@ hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Lay
     const Layout O = ordered(L, s, kOrderCopy);
     if (L.tiles == 0) return hipSuccess;
     return dispatch_waves(s, [&](auto waves) {
-        constexpr int W = decltype(waves)::value;
-        return launch(copy_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, p, fp, L.desc,
-                      L.tile_replica, O, B);
+        return dispatch_stream(L.stream_owners, [&](auto streaming) {
+            constexpr int W = decltype(waves)::value;
+            constexpr bool S = decltype(streaming)::value;
+            return launch(copy_kernel<W, S>, tile_grid(L, W), dim3(W * kWave), stream, p, fp, L.desc,
+                          L.tile_replica, O, B);
+        });
     });
 }
 
@@ -739,9 +742,12 @@ hipError_t launch_refresh(const dsdx_particles& p, const FieldParams& fp, int32_
     const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
     return dispatch_waves(s, [&](auto waves) {
-        constexpr int W = decltype(waves)::value;
-        return launch(refresh_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, p, fp,
-                      positions_only, L.desc, L.tile_replica, O, B);
+        return dispatch_stream(L.stream_owners, [&](auto streaming) {
+            constexpr int W = decltype(waves)::value;
+            constexpr bool S = decltype(streaming)::value;
+            return launch(refresh_kernel<W, S>, tile_grid(L, W), dim3(W * kWave), stream, p, fp,
+                          positions_only, L.desc, L.tile_replica, O, B);
+        });
     });
 }
 

@@ -21,6 +21,28 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Number of set bits of `mask` below this lane.
+// An owner value that a pass gathers once and only reads, and that no recent
+// kernel wrote (the velocities, masses, charges, types and ids a ghost copy
+// reads).  STREAM (the particle arrays are larger than the 256 MB memory-side
+// cache, Layout::stream_owners): the load is nontemporal, so the gathered
+// lines do not displace what the caches hold.  Otherwise the arrays may be
+// cache-resident and the load is ordinary.  A compile-time choice: the
+// compiler merges a run-time select of the two loads into one ordinary load.
+template <bool STREAM, class T>
+__device__ __forceinline__ T owner_load(const T* p)
+{
+    if constexpr (STREAM) return __builtin_nontemporal_load(p);
+    return *p;
+}
+
+// Calls f(std::integral_constant<bool, stream>) for the run-time flag.
+template <class F>
+hipError_t dispatch_stream(uint32_t stream, F&& f)
+{
+    if (stream) return f(std::integral_constant<bool, true>{});
+    return f(std::integral_constant<bool, false>{});
+}
+
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
 {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),

@@ -71,7 +71,7 @@ __device__ __forceinline__ double double_of(uint64_t v) { return __longlong_as_d
 // boundary entry (bpos); read them there instead of gathering x, y, z.  The
 // owner gathers bound it (one lane per owner, sparse in owned order);
 // staging the records through LDS for full-wave stores measured no faster.
-template <int WAVES>
+template <int WAVES, bool STREAM>
 __global__ __launch_bounds__(WAVES * kWave) void pack_halo_kernel(
     dsdx_particles p, RecordFields rf, int32_t positions_only, int32_t fresh, unsigned char* __restrict__ out,
     const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ tile_replica, Layout L,
@@ -118,10 +118,10 @@ __global__ __launch_bounds__(WAVES * kWave) void pack_halo_kernel(
             });
             return;
         }
-        const double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
-        const double m = p.mass[i], q = p.charge[i];
-        const uint64_t ids = static_cast<uint32_t>(p.type[i]) |
-                             (static_cast<uint64_t>(static_cast<uint32_t>(p.global_id[i])) << 32);
+        const double vx = owner_load<STREAM>(p.vx + i), vy = owner_load<STREAM>(p.vy + i), vz = owner_load<STREAM>(p.vz + i);
+        const double m = owner_load<STREAM>(p.mass + i), q = owner_load<STREAM>(p.charge + i);
+        const uint64_t ids = static_cast<uint32_t>(owner_load<STREAM>(p.type + i)) |
+                             (static_cast<uint64_t>(static_cast<uint32_t>(owner_load<STREAM>(p.global_id + i))) << 32);
         each_image([&](uint32_t s, uint32_t at) {
             unsigned char* rec = out + static_cast<uint64_t>(at) * rf.stride;
             double* r = reinterpret_cast<double*>(rec);
@@ -417,9 +417,12 @@ hipError_t launch_pack_halo(const dsdx_particles& p, const RecordFields& rf, int
     if (L.tiles == 0) return hipSuccess;
     const Layout O = ordered(L, s, kOrderOwners);
     return dispatch_waves(s, [&](auto waves) {
-        constexpr int W = decltype(waves)::value;
-        return launch(pack_halo_kernel<W>, tile_grid(L, W), dim3(W * kWave), stream, p, rf, positions_only,
-                      fresh, out, L.desc, L.tile_replica, O, B);
+        return dispatch_stream(L.stream_owners, [&](auto streaming) {
+            constexpr int W = decltype(waves)::value;
+            constexpr bool S = decltype(streaming)::value;
+            return launch(pack_halo_kernel<W, S>, tile_grid(L, W), dim3(W * kWave), stream, p, rf,
+                          positions_only, fresh, out, L.desc, L.tile_replica, O, B);
+        });
     });
 }
 
