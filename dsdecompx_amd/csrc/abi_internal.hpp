@@ -105,13 +105,14 @@ struct Plan {
         L.tile_waves = tiles;
         if (desc.size() <= static_cast<size_t>(kWave) && live_tiles)
             L.tile_waves = std::min(tiles, live_tiles + live_tiles / 16 + 8);
-        // Owner gathers stream past the caches once the owned records no
-        // longer fit the 256 MB memory-side cache (16 M particles: copy 168 ->
-        // 146 us); below that the gathered lines may be cache hits, and
-        // streaming them measured slower (1 M particles: 27.5 -> 33 us).
+        // Reads that a pass makes once stream past the caches when the owned
+        // records no longer fit the 256 MB memory-side cache (16 M particles:
+        // copy 168 -> 146 us, classify 81 -> 66 us); below that the lines may
+        // be cache hits, and streaming them measured slower (1 M particles:
+        // copy 27.5 -> 33 us).
         uint64_t owned_total = 0;
         for (const ReplicaDesc& d : desc) owned_total += d.owned;
-        L.stream_owners = owned_total * kRecordBytes > kMallBytes ? 1u : 0u;
+        L.streaming = owned_total * kRecordBytes > kMallBytes ? 1u : 0u;
         L.xcd_run = static_cast<uint32_t>(shape.xcd_run);
         L.descending = 0;
         return L;

@@ -118,7 +118,8 @@ struct Layout {
     uint32_t chunk_size;            // kWave * items (a power of two)
     uint32_t tiles;                 // owner tiles over all replicas
     uint32_t tile_waves;            // waves of an owner-major launch (each walks tiles w, w + tile_waves, ...)
-    uint32_t stream_owners;         // owner gathers load nontemporally (see owner_load)
+    uint32_t streaming;             // the owned records exceed the memory-side cache: passes that
+                                    // read an array once load it nontemporally (read_once, load_items)
     uint32_t xcd_run;               // block renumbering (see xcd_block)
     uint32_t descending;            // this launch walks its chunks / tiles last to first
 };

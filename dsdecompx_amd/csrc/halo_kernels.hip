@@ -39,7 +39,7 @@ __device__ __forceinline__ uint32_t slab_code(double px, double py, double pz, c
 // With WRAP the pass first applies migrate's periodic wrap to the positions
 // it loaded (storing the changed ones) and classifies the wrapped values:
 // migrate followed by a rebuild with one read of x, y, z.
-template <int ITEMS, int WAVES, bool WRAP>
+template <int ITEMS, int WAVES, bool WRAP, bool STREAM>
 __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
     double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
     const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ chunk_replica, Layout L,
@@ -62,9 +62,9 @@ __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
     const uint32_t first = (chunk - d->first_chunk) * K;
     const bool full = first + K <= owned;
     double px[ITEMS], py[ITEMS], pz[ITEMS];
-    load_items<ITEMS>(x + off, first, owned, full, px);
-    load_items<ITEMS>(y + off, first, owned, full, py);
-    load_items<ITEMS>(z + off, first, owned, full, pz);
+    load_items<ITEMS, STREAM>(x + off, first, owned, full, px);
+    load_items<ITEMS, STREAM>(y + off, first, owned, full, py);
+    load_items<ITEMS, STREAM>(z + off, first, owned, full, pz);
     if (WRAP) {
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
@@ -327,9 +327,9 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
         const double px = B.bpos[E];
         const double py = B.bpos[B.bpos_stride + E];
         const double pz = B.bpos[2 * B.bpos_stride + E];
-        const double vx = owner_load<STREAM>(p.vx + i), vy = owner_load<STREAM>(p.vy + i), vz = owner_load<STREAM>(p.vz + i);
-        const double m = owner_load<STREAM>(p.mass + i), q = owner_load<STREAM>(p.charge + i);
-        const int32_t ty = owner_load<STREAM>(p.type + i), id = owner_load<STREAM>(p.global_id + i);
+        const double vx = read_once<STREAM>(p.vx + i), vy = read_once<STREAM>(p.vy + i), vz = read_once<STREAM>(p.vz + i);
+        const double m = read_once<STREAM>(p.mass + i), q = read_once<STREAM>(p.charge + i);
+        const int32_t ty = read_once<STREAM>(p.type + i), id = read_once<STREAM>(p.global_id + i);
         const FirstImages f = first_images(g);
         each_image(Images{g, f, t}, [&](uint32_t s, uint64_t j) {
             p.x[j] = px + t.shift[s][0];
@@ -562,9 +562,9 @@ __global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(
         double vx = 0.0, vy = 0.0, vz = 0.0, m = 0.0, q = 0.0;
         int32_t ty = 0, id = 0;
         if (!positions_only) {
-            vx = owner_load<STREAM>(p.vx + i); vy = owner_load<STREAM>(p.vy + i); vz = owner_load<STREAM>(p.vz + i);
-            m = owner_load<STREAM>(p.mass + i); q = owner_load<STREAM>(p.charge + i);
-            ty = owner_load<STREAM>(p.type + i); id = owner_load<STREAM>(p.global_id + i);
+            vx = read_once<STREAM>(p.vx + i); vy = read_once<STREAM>(p.vy + i); vz = read_once<STREAM>(p.vz + i);
+            m = read_once<STREAM>(p.mass + i); q = read_once<STREAM>(p.charge + i);
+            ty = read_once<STREAM>(p.type + i); id = read_once<STREAM>(p.global_id + i);
         }
         const FirstImages f = first_images(g);
         each_image(Images{g, f, t}, [&](uint32_t s, uint64_t j) {
@@ -659,12 +659,15 @@ hipError_t launch_classify(double* x, double* y, double* z, bool wrap_first, con
     const Layout O = ordered(L, s, wrap_first ? kOrderMigrate : kOrderClassify);
     if (L.chunks == 0) return hipSuccess;
     return dispatch(s, [&](auto items, auto waves) {
-        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
-        if (wrap_first)
-            return launch(classify_kernel<I, W, true>, chunk_grid(L, W), dim3(W * kWave), stream,
-                          x, y, z, L.desc, L.chunk_replica, O, B);
-        return launch(classify_kernel<I, W, false>, chunk_grid(L, W), dim3(W * kWave), stream,
-                      x, y, z, L.desc, L.chunk_replica, O, B);
+        return dispatch_stream(L.streaming, [&](auto streaming) {
+            constexpr int I = decltype(items)::value, W = decltype(waves)::value;
+            constexpr bool S = decltype(streaming)::value;
+            if (wrap_first)
+                return launch(classify_kernel<I, W, true, S>, chunk_grid(L, W), dim3(W * kWave),
+                              stream, x, y, z, L.desc, L.chunk_replica, O, B);
+            return launch(classify_kernel<I, W, false, S>, chunk_grid(L, W), dim3(W * kWave),
+                          stream, x, y, z, L.desc, L.chunk_replica, O, B);
+        });
     });
 }
 
@@ -691,7 +694,7 @@ hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Lay
     const Layout O = ordered(L, s, kOrderCopy);
     if (L.tiles == 0) return hipSuccess;
     return dispatch_waves(s, [&](auto waves) {
-        return dispatch_stream(L.stream_owners, [&](auto streaming) {
+        return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr int W = decltype(waves)::value;
             constexpr bool S = decltype(streaming)::value;
             return launch(copy_kernel<W, S>, tile_grid(L, W), dim3(W * kWave), stream, p, fp, L.desc,
@@ -742,7 +745,7 @@ hipError_t launch_refresh(const dsdx_particles& p, const FieldParams& fp, int32_
     const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
     return dispatch_waves(s, [&](auto waves) {
-        return dispatch_stream(L.stream_owners, [&](auto streaming) {
+        return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr int W = decltype(waves)::value;
             constexpr bool S = decltype(streaming)::value;
             return launch(refresh_kernel<W, S>, tile_grid(L, W), dim3(W * kWave), stream, p, fp,

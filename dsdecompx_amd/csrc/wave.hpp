@@ -21,15 +21,15 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Number of set bits of `mask` below this lane.
-// An owner value that a pass gathers once and only reads, and that no recent
-// kernel wrote (the velocities, masses, charges, types and ids a ghost copy
-// reads).  STREAM (the particle arrays are larger than the 256 MB memory-side
-// cache, Layout::stream_owners): the load is nontemporal, so the gathered
-// lines do not displace what the caches hold.  Otherwise the arrays may be
-// cache-resident and the load is ordinary.  A compile-time choice: the
+// A value that the pass reads once and only reads: the owner records a ghost
+// copy gathers (no recent kernel wrote them), the positions the
+// classification reads.  STREAM (the particle arrays are larger than the
+// 256 MB memory-side cache, Layout::streaming): the load is nontemporal, so
+// these lines do not displace what the caches hold.  Otherwise the arrays may
+// be cache-resident and the load is ordinary.  A compile-time choice: the
 // compiler merges a run-time select of the two loads into one ordinary load.
 template <bool STREAM, class T>
-__device__ __forceinline__ T owner_load(const T* p)
+__device__ __forceinline__ T read_once(const T* p)
 {
     if constexpr (STREAM) return __builtin_nontemporal_load(p);
     return *p;
@@ -133,19 +133,21 @@ __device__ __forceinline__ void wrap_store(double& v, double* dst, bool valid, i
 
 // Loads v[k] = src[first + 64 k + lane] for the chunk's items (0 past
 // `owned`); a chunk entirely below `owned` takes the unpredicated loads.
-template <int ITEMS>
+// STREAM: nontemporal loads (read_once).
+template <int ITEMS, bool STREAM = false>
 __device__ __forceinline__ void load_items(const double* __restrict__ src, uint32_t first,
                                            uint32_t owned, bool full, double (&v)[ITEMS])
 {
     const uint32_t lane = lane_id();
+    auto ld = [&](uint32_t i) { return read_once<STREAM>(src + i); };
     if (full) {
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) v[k] = src[first + k * kWave + lane];
+        for (int k = 0; k < ITEMS; ++k) v[k] = ld(first + k * kWave + lane);
     } else {
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
             const uint32_t i = first + k * kWave + lane;
-            v[k] = i < owned ? src[i] : 0.0;
+            v[k] = i < owned ? ld(i) : 0.0;
         }
     }
 }

@@ -118,10 +118,10 @@ __global__ __launch_bounds__(WAVES * kWave) void pack_halo_kernel(
             });
             return;
         }
-        const double vx = owner_load<STREAM>(p.vx + i), vy = owner_load<STREAM>(p.vy + i), vz = owner_load<STREAM>(p.vz + i);
-        const double m = owner_load<STREAM>(p.mass + i), q = owner_load<STREAM>(p.charge + i);
-        const uint64_t ids = static_cast<uint32_t>(owner_load<STREAM>(p.type + i)) |
-                             (static_cast<uint64_t>(static_cast<uint32_t>(owner_load<STREAM>(p.global_id + i))) << 32);
+        const double vx = read_once<STREAM>(p.vx + i), vy = read_once<STREAM>(p.vy + i), vz = read_once<STREAM>(p.vz + i);
+        const double m = read_once<STREAM>(p.mass + i), q = read_once<STREAM>(p.charge + i);
+        const uint64_t ids = static_cast<uint32_t>(read_once<STREAM>(p.type + i)) |
+                             (static_cast<uint64_t>(static_cast<uint32_t>(read_once<STREAM>(p.global_id + i))) << 32);
         each_image([&](uint32_t s, uint32_t at) {
             unsigned char* rec = out + static_cast<uint64_t>(at) * rf.stride;
             double* r = reinterpret_cast<double*>(rec);
@@ -417,7 +417,7 @@ hipError_t launch_pack_halo(const dsdx_particles& p, const RecordFields& rf, int
     if (L.tiles == 0) return hipSuccess;
     const Layout O = ordered(L, s, kOrderOwners);
     return dispatch_waves(s, [&](auto waves) {
-        return dispatch_stream(L.stream_owners, [&](auto streaming) {
+        return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr int W = decltype(waves)::value;
             constexpr bool S = decltype(streaming)::value;
             return launch(pack_halo_kernel<W, S>, tile_grid(L, W), dim3(W * kWave), stream, p, rf,
