@@ -558,7 +558,7 @@ __global__ __launch_bounds__(WAVES * kWave) void refresh_kernel(
         if (!o.live) return;
         const uint64_t i = o.d->offset + B.owner_index[o.u];
         const OwnerGhosts g = ghosts_of(desc, o, B);
-        const double px = p.x[i], py = p.y[i], pz = p.z[i];
+        const double px = read_once<STREAM>(p.x + i), py = read_once<STREAM>(p.y + i), pz = read_once<STREAM>(p.z + i);
         double vx = 0.0, vy = 0.0, vz = 0.0, m = 0.0, q = 0.0;
         int32_t ty = 0, id = 0;
         if (!positions_only) {

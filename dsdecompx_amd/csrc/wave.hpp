@@ -22,8 +22,7 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgc
 
 // Number of set bits of `mask` below this lane.
 // A value that the pass reads once and only reads: the owner records a ghost
-// copy gathers (no recent kernel wrote them), the positions the
-// classification reads.  STREAM (the particle arrays are larger than the
+// copy or refresh gathers, the positions the classification reads.  STREAM (the particle arrays are larger than the
 // 256 MB memory-side cache, Layout::streaming): the load is nontemporal, so
 // these lines do not displace what the caches hold.  Otherwise the arrays may
 // be cache-resident and the load is ordinary.  A compile-time choice: the

@@ -91,9 +91,9 @@ __global__ __launch_bounds__(WAVES * kWave) void pack_halo_kernel(
             y = B.bpos[B.bpos_stride + E];
             z = B.bpos[2 * B.bpos_stride + E];
         } else {
-            x = p.x[i];
-            y = p.y[i];
-            z = p.z[i];
+            x = read_once<STREAM>(p.x + i);
+            y = read_once<STREAM>(p.y + i);
+            z = read_once<STREAM>(p.z + i);
         }
         // every image's record, first the preloaded ones
         auto each_image = [&](auto&& put) {
