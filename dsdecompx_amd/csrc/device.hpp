@@ -78,6 +78,14 @@ struct ReplicaDesc {
     uint32_t pad;
 };
 
+// Bits of Shape::descending.
+constexpr int kOrderMigrate = 1;
+constexpr int kOrderClassify = 2;
+constexpr int kOrderRoute = 4;
+constexpr int kOrderCopy = 8;
+constexpr int kOrderOwners = 16;
+constexpr int kOrderMigrateScatter = 32;   // process grids: the migration's record moves
+
 // Launch shape of the classify / scatter / owner-major kernels: items per
 // lane (the chunk is kWave * items particles) and waves per block.
 struct Shape {
@@ -90,17 +98,12 @@ struct Shape {
     // the fused migrate + classify) walks backwards: the most recently
     // written lines are the ones still held by the 256 MB memory-side cache.
     // The classify pass that follows migrate then walks forwards, into the
-    // lines migrate read last.
-    int descending = 1;
+    // lines migrate read last.  The owner-major passes after the rebuild
+    // (reverse, refresh, field exchanges) walk backwards for the same reason:
+    // the application has just written the forces (or positions) in index
+    // order, and the highest owners' lines are the cached ones.
+    int descending = kOrderMigrate | kOrderOwners;
 };
-
-// Bits of Shape::descending.
-constexpr int kOrderMigrate = 1;
-constexpr int kOrderClassify = 2;
-constexpr int kOrderRoute = 4;
-constexpr int kOrderCopy = 8;
-constexpr int kOrderOwners = 16;
-constexpr int kOrderMigrateScatter = 32;   // process grids: the migration's record moves
 
 // Owner tiles: the owner slots [0, bound_r) of replica r (relative to
 // first_chunk_r * K), bound_r = min(owned_r, capacity_r - owned_r,
