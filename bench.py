@@ -96,6 +96,73 @@ def cpu_baseline(name, n, L, w, sigma, periodic, steps, seed=7):
                       f"oracle (C, -O2, one thread) on {os.cpu_count()} visible host cores"}
 
 
+def cpu_all_cores(name, n, L, w, sigma, steps, seed=11):
+    """The oracle on every host core at once: P independent periodic
+    sub-boxes of n / P particles each at the workload's density (the work of a
+    P-rank Host run without its MPI transfers), one thread per sub-box, wall
+    clock per step over all P.  A reported figure beside `cpu_baseline`."""
+    import concurrent.futures as cf
+    import numpy as np
+    from oracle import pyoracle as po
+
+    P = max(1, min(16, os.cpu_count() or 1))
+    m = n // P
+    Ls = L / P ** (1.0 / 3.0)
+    rng = np.random.default_rng(seed)
+    work = []
+    for _ in range(P):
+        ora = po.Oracle(po.Box(upper=(Ls, Ls, Ls), width=w))
+        cap = int(m * 1.5) + 4096
+        p = po.Particles(cap)
+        p.x[:m], p.y[:m], p.z[:m] = (rng.uniform(0, Ls, m) for _ in range(3))
+        p.vx[:m], p.vy[:m], p.vz[:m] = (rng.standard_normal(m) for _ in range(3))
+        p.mass[:m] = 1.0
+        p.global_id[:m] = np.arange(m, dtype=np.int32)
+        work.append((ora, p, rng.normal(0.0, sigma, (3, m)), rng.uniform(-1, 1, (3, cap))))
+
+    ghosts = [0] * P
+
+    def displace(r, k):
+        ora, p, disp, force = work[r]
+        for a, key in enumerate(("x", "y", "z")):
+            getattr(p, key)[:m] += np.roll(disp[a], (k * 7919) % m)
+
+    def build(r):
+        ora, p, disp, force = work[r]
+        ora.migrate(p, m)
+        ghosts[r] = ora.exchange_halo(p, m)
+
+    def fill(r):
+        ora, p, disp, force = work[r]
+        for a, key in enumerate(("fx", "fy", "fz")):
+            getattr(p, key)[:m + ghosts[r]] = force[a, :m + ghosts[r]]
+
+    def reverse(r):
+        ora, p, disp, force = work[r]
+        ora.reverse_halo(p, m)
+
+    # Like cpu_baseline, the displacement and the force fill (numpy, under the
+    # interpreter lock) are outside the timed phases.
+    timed = 0.0
+    with cf.ThreadPoolExecutor(P) as pool:
+        for k in range(steps + 1):
+            list(pool.map(lambda r: displace(r, k), range(P)))
+            t0 = time.perf_counter()
+            list(pool.map(build, range(P)))
+            t1 = time.perf_counter()
+            list(pool.map(fill, range(P)))
+            t2 = time.perf_counter()
+            list(pool.map(reverse, range(P)))
+            t3 = time.perf_counter()
+            if k > 0:
+                timed += (t1 - t0) + (t3 - t2)
+    ms = 1e3 * timed / steps
+    return {"value": P * m / (ms * 1e3), "unit": "Mparticles/s", "cores": P, "kind": "port",
+            "ms_per_step": ms,
+            "sample": f"{name} workload split into {P} independent periodic boxes of {m} particles "
+                      f"(L = {Ls:.1f}), one oracle thread each, {steps} timed steps after 1 warm-up"}
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -804,6 +871,8 @@ def run(args):
         out["wire"] = measure_wire(args, n, L, w)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(name, n, L, w, sigma, periodic,
                                                                  args.cpu_steps)
+    if not args.no_cpu and all(periodic):
+        out["cpu_all_cores"] = cpu_all_cores(name, n, L, w, sigma, args.cpu_steps)
     if not args.no_batched:
         out["batched"] = measure_batched(args)
     if args.autograd:
