@@ -20,13 +20,13 @@ __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
 __device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Number of set bits of `mask` below this lane.
 // A value that the pass reads once and only reads: the owner records a ghost
-// copy or refresh gathers, the positions the classification reads.  STREAM (the particle arrays are larger than the
-// 256 MB memory-side cache, Layout::streaming): the load is nontemporal, so
-// these lines do not displace what the caches hold.  Otherwise the arrays may
-// be cache-resident and the load is ordinary.  A compile-time choice: the
-// compiler merges a run-time select of the two loads into one ordinary load.
+// copy or refresh gathers, the positions the classification reads.  STREAM
+// (the particle arrays are larger than the 256 MB memory-side cache,
+// Layout::streaming): the load is nontemporal, so these lines do not displace
+// what the caches hold.  Otherwise the arrays may be cache-resident and the
+// load is ordinary.  A compile-time choice: the compiler merges a run-time
+// select of the two loads into one ordinary load.
 template <bool STREAM, class T>
 __device__ __forceinline__ T read_once(const T* p)
 {
@@ -42,6 +42,7 @@ hipError_t dispatch_stream(uint32_t stream, F&& f)
     return f(std::integral_constant<bool, false>{});
 }
 
+// Number of set bits of `mask` below this lane.
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
 {
     return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
