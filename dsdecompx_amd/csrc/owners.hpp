@@ -49,7 +49,7 @@ __device__ __forceinline__ const SlotTable& fill_slot_table(SlotTable& t,
 // Ghost-list entries an owner-major kernel loads before its ghost loop.  On
 // gfx9 the vector memory counter counts stores as well as loads, so a load
 // issued inside a loop of stores makes the wave wait for the stores too.
-constexpr uint32_t kPreloaded = 4;
+constexpr uint32_t kPreloaded = 2;
 
 // Owner-major kernels: one wave per owner tile, one lane per boundary owner
 // (the owner slots of a replica are dense, in owned order).  A replica whose
