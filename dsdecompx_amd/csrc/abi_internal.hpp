@@ -54,6 +54,9 @@ struct Plan {
     // host read (0 before the first).  Owner-major launches size their grid
     // from it; their waves walk however many tiles the device counts hold.
     uint32_t live_tiles = 0;
+    // Single box: the chunk size follows the particle count (auto_items) until
+    // a caller or DSDX_ITEMS fixes it.
+    bool items_auto = true;
 
     void note_live_tiles()
     {
