@@ -309,15 +309,20 @@ int auto_items(uint64_t owned)
     return 4;
 }
 
+// The chunk size of a single-box plan for `owned` particles, unless fixed.
+void apply_auto_items(Plan* plan, uint64_t owned)
+{
+    if (!plan->items_auto || plan->shape.items == auto_items(owned)) return;
+    plan->shape.items = auto_items(owned);
+    plan->classified = plan->built = plan->totals_cached = false;
+    plan->desc_on_device = false;
+    plan->live_tiles = 0;
+}
+
 int set_single(Plan* plan, const BoxParams& bp, uint64_t owned, uint64_t capacity,
                uint64_t ghost_list_elems, hipStream_t stream)
 {
-    if (plan->items_auto && plan->shape.items != auto_items(owned)) {
-        plan->shape.items = auto_items(owned);
-        plan->classified = plan->built = plan->totals_cached = false;
-        plan->desc_on_device = false;
-        plan->live_tiles = 0;
-    }
+    apply_auto_items(plan, owned);
     ReplicaDesc d;
     std::memset(&d, 0, sizeof(d));
     d.bp = bp;
@@ -494,6 +499,7 @@ int dsdx_plan_reserve(dsdx_plan* plan, uint64_t owned, uint64_t ghosts)
 {
     if (!plan) return fail(DSDX_EINVAL, "null plan");
     if (owned > kMaxOwned) return fail(DSDX_EINVAL, "too many particles for one device plan");
+    apply_auto_items(plan, owned);   // the chunk size a build of `owned` will use
     const uint32_t K = plan->chunk_size();
     const uint32_t saved = plan->chunks;
     plan->chunks = static_cast<uint32_t>((owned + K - 1) / K);

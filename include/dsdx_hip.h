@@ -201,10 +201,12 @@ int dsdx_forward_exchange(const dsdx_plan* plan, const dsdx_field* fields,
 
 /* Launch shape of the plan's kernels: lane items per chunk (4, 8, 16; a
  * chunk is 64 x items particles, one wave each), waves per block (2, 4, 8)
- * and scan block threads (256, 512, 1024).  Defaults 16 / 4 / 1024, or the
- * environment variables DSDX_ITEMS, DSDX_WAVES, DSDX_SCAN_THREADS; the
- * autotuner (dsdecompx_amd.autotune) picks them by measurement.  Changing
- * items drops the plan. */
+ * and scan block threads (256, 512, 1024).  Defaults: items by the owned
+ * count of each build (16 from 4 M particles, 8 from 2 M, else 4) / 4 /
+ * 1024, or the environment variables DSDX_ITEMS, DSDX_WAVES,
+ * DSDX_SCAN_THREADS; the autotuner (dsdecompx_amd.autotune) picks them by
+ * measurement.  Setting a shape fixes the items; changing items drops the
+ * plan. */
 int dsdx_plan_set_shape(dsdx_plan* plan, int32_t items, int32_t waves,
                         int32_t scan_threads);
 int dsdx_plan_get_shape(const dsdx_plan* plan, int32_t* items, int32_t* waves,
