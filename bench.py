@@ -31,9 +31,9 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
     # name: (particles, box length, halo width, displacement sigma, periodic)
-    "cfg3": (16_777_216, 256.0, 2.8, 0.1, (True, True, True)),
-    "cfg3z": (16_777_216, 256.0, 2.8, 0.1, (True, True, False)),
-    "cfg1": (32_768, 32.0, 2.8, 0.05, (True, True, True)),
+    "uniform-16m": (16_777_216, 256.0, 2.8, 0.1, (True, True, True)),
+    "uniform-16m-zwalls": (16_777_216, 256.0, 2.8, 0.1, (True, True, False)),
+    "uniform-32k": (32_768, 32.0, 2.8, 0.05, (True, True, True)),
 }
 
 
@@ -310,7 +310,8 @@ def measure_cpp(args, n, L, w, sigma, periodic):
 
 
 def measure_fields(args):
-    """cfg2 through the drop-in C++ API (comm_bench, mode "fields"):
+    """The clustered workload with registered fields, through the drop-in C++
+    API (comm_bench, mode "fields"):
     1,048,576 particles in 64 Gaussian clusters (sigma 4) in a periodic box
     L = 101.6, halo width 2.8, displacement N(0, 0.5) per step, and two
     registered fields (add_field) -- density (f64) and vec (3 x f64), F = 32 B
@@ -324,7 +325,7 @@ def measure_fields(args):
     F = 32.0
     step_bytes = 48.0 * n + 24.0 * n + (168.0 + 2 * F) * G + 24.0 * G + 48.0 * U + F * G + 2 * F * U
     return {
-        "workload": f"cfg2: {n} particles in 64 Gaussian clusters (sigma 4) in a periodic box L={L}, "
+        "workload": f"{n} particles in 64 Gaussian clusters (sigma 4) in a periodic box L={L}, "
                     f"halo width {w}, displacement N(0,{sigma}) per axis per step, 2 registered "
                     f"fields (f64 density, 3 x f64 vector), reverse_exchange(Sum) of both",
         "value": n / (ms * 1e3), "unit": "Mparticles/s", "ms_per_step": ms,
@@ -371,7 +372,7 @@ def measure_grid(args, n, L, w, sigma, dims=(2, 2, 2)):
 
 
 def measure_batched(args):
-    """cfg5: 512 replica boxes, particle counts log-uniform in [2048, 262144],
+    """512 replica boxes, particle counts log-uniform in [2048, 262144],
     density U[0.6, 1.2], w = 2.8, displacement N(0, 0.1).  The same steps
     (application displacement + migrate + exchange_halo + forces +
     reverse_halo) run as the plain per-box loop and as the batched form
@@ -467,7 +468,7 @@ def measure_batched(args):
     ghosts, owners, status = b.counts()
     total = int(n.sum())
     return {
-        "workload": f"cfg5: {R} periodic replica boxes, {total} particles (log-uniform "
+        "workload": f"{R} periodic replica boxes, {total} particles (log-uniform "
                     f"2048..262144 per box, density U[0.6,1.2]), halo width 2.8, "
                     f"displacement N(0,0.1)",
         "replicas": R, "particles": total, "steps": steps,
@@ -557,7 +558,7 @@ def measure_traffic(n, L, w, sigma, periodic, env, seed):
 
 
 def measure_wire(args, n, L, w):
-    """The cfg3 halo rebuild and reverse through the communication records,
+    """The 16 M-particle halo rebuild and reverse through the communication records,
     as each rank of a transport-based multi-GPU run would execute them:
     dsdx_halo_build_records (classify, route, 96-byte ParticleRecords into the
     send buffer; no ghost written), the transfer (dsdx_loopback_exchange
@@ -666,7 +667,7 @@ def measure_wire(args, n, L, w):
 
 
 def measure_autograd(args):
-    """cfg4 (65,536 particles, L = 40.3, w = 2.8, one rebuild): the autograd
+    """65,536 particles, L = 40.3, w = 2.8, one rebuild: the autograd
     form -- owned positions through HaloPositions (positions-only refresh
     along the saved plan), a differentiable scalar sum(out * weight) of all
     positions, backward through the reverse_halo kernel -- against the plain
@@ -729,7 +730,7 @@ def measure_autograd(args):
     ms_plain = timed(plain_step, iters)
     same = torch.equal(autograd_step().view(torch.int64), plain_step().view(torch.int64))
     return {
-        "workload": f"cfg4: {n} particles uniform in L={L} (periodic), halo width {w}, {g} ghosts",
+        "workload": f"{n} particles uniform in L={L} (periodic), halo width {w}, {g} ghosts",
         "autograd_ms_per_iter": ms_auto, "plain_ms_per_iter": ms_plain,
         "plain_vs_autograd": ms_auto / ms_plain,
         "bit_exact_gradient": bool(same),
@@ -894,22 +895,22 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", default="cfg3", choices=sorted(WORKLOADS))
+    ap.add_argument("--config", default="uniform-16m", choices=sorted(WORKLOADS))
     ap.add_argument("--particles", type=int, default=0, help="override the particle count")
     ap.add_argument("--path", default="cpp", choices=("cpp", "abi"),
                     help="cpp: the C++ Decomp API driver; abi: the C ABI from Python")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-batched", action="store_true", help="skip the cfg5 batched comparison")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched comparison (512 replica boxes)")
     ap.add_argument("--no-fused", dest="fused", action="store_false",
                     help="skip the fused migrate_and_exchange_halo run")
     ap.add_argument("--no-refresh", dest="refresh", action="store_false",
                     help="skip the reused-plan (refresh) runs")
     ap.add_argument("--no-autograd", dest="autograd", action="store_false",
-                    help="skip the cfg4 autograd-vs-plain comparison")
+                    help="skip the autograd-vs-plain comparison (65,536 particles)")
     ap.add_argument("--no-fields", dest="fields", action="store_false",
-                    help="skip the cfg2 run (clustered particles with registered fields)")
+                    help="skip the run of 1,048,576 clustered particles with registered fields")
     ap.add_argument("--no-wire", dest="wire", action="store_false",
                     help="skip the communication-record (pack / unpack) run")
     ap.add_argument("--no-grid", dest="grid", action="store_false",

@@ -150,7 +150,7 @@ struct HostState {
     std::vector<int32_t> i32[2];
 };
 
-// cfg1-style loop through the C++ API, every array compared with the oracle
+// 32,768-particle loop through the C++ API, every array compared with the oracle
 // after every step; the halo is rebuilt each step after migrate; every third
 // step also refreshes the reused plan (positions only, then full).  With
 // `fused` the step calls migrate_and_exchange_halo instead of migrate +
@@ -257,7 +257,7 @@ static void test_steps_against_oracle(hipStream_t s, std::size_t n, double L, do
         hip_ok(hipMemcpy(p.fz_data(), fz.data(), (n + g) * 8, hipMemcpyHostToDevice), "up");
         decomp.reverse_halo(p, s);
         dsdxo_reverse_halo(&nb, &plan, static_cast<int64_t>(n), fx.data(), fy.data(), fz.data());
-        // density: ghost contributions summed back (cfg2's reverse_exchange)
+        // density: ghost contributions summed back (the clustered workload's reverse_exchange)
         for (std::size_t i = n; i < n + g; ++i) density[i] = uniform(seed);
         hip_ok(hipMemcpy(d_density, density.data(), (n + g) * 8, hipMemcpyHostToDevice), "up");
         decomp.reverse_exchange(p, {{"density", ReduceOp::Sum}}, s);

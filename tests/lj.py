@@ -1,5 +1,5 @@
 """Lennard-Jones pair energy and forces over owned + ghost particles (torch,
-FP64): the application side of the cfg4 differentiation test.
+FP64): the application side of the differentiation test (65,536 particles).
 
 E = 1/2 sum_{i owned} sum_{j owned or ghost, j != i, r_ij < rc} phi(r_ij),
 phi(r) = 4 eps ((sigma/r)^12 - (sigma/r)^6) - phi(rc)  (energy-shifted so the

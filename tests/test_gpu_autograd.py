@@ -1,4 +1,4 @@
-"""cfg4: differentiation through the halo.  exchange_halo runs inside a
+"""Differentiation through the halo (65,536 particles, L = 40.3).  exchange_halo runs inside a
 torch.autograd.Function whose backward is the reverse_halo kernel; a
 Lennard-Jones energy over owned + ghost particles is differentiated, checked
 by central finite differences (h = 1e-6) on 256 random particles, and
@@ -63,12 +63,12 @@ def test_gradcheck_functions(gpu):
         out.sum().backward()
 
 
-def test_lj_forces_cfg4(gpu):
+def test_lj_forces_65k(gpu):
     torch = gpu
     from dsdecompx_amd import autograd as ag
     from tests import lj
     n, L, w, rc = 65536, 40.3, 2.8, 2.5
-    say("cfg4 LJ", n)
+    say("LJ", n)
     d, p, halo, g = _setup(n, L, w, 404)
     pos = torch.stack([p.x[:n], p.y[:n], p.z[:n]], 1).clone().requires_grad_(True)
     pos_all = ag.halo_positions(pos, halo, p)
@@ -127,7 +127,7 @@ def test_forward_reverse_adjoint_vec3(gpu):
     torch = gpu
     from dsdecompx_amd import autograd as ag
     n, L, w = 65536, 40.3, 2.8
-    say("cfg4 adjointness", n)
+    say("adjointness", n)
     d, p, halo, g = _setup(n, L, w, 405)
     gen = torch.Generator(device="cuda").manual_seed(11)
     x = torch.randn((n, 3), dtype=torch.float64, device="cuda", generator=gen)

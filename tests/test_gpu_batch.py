@@ -1,4 +1,4 @@
-"""cfg5-style batching: many replica boxes of uneven size in one launch per
+"""Batching: many replica boxes of uneven size in one launch per
 phase.  Every replica's ghosts, positions and reverse sums are bit-exact with
 the oracle run on that replica alone and with the plain per-box loop; the
 full step captured in a hipGraph replays to the same bits as the eager step

@@ -1,12 +1,12 @@
 """The workloads of the benchmark at their real sizes, bit-exact against the
 oracle on identical inputs:
 
-- cfg2: 1,048,576 particles in 64 Gaussian clusters (sigma 4, L = 101.6),
+- clustered, 1,048,576 particles in 64 Gaussian clusters (sigma 4, L = 101.6),
   displacement N(0, 0.5) per step (particles cross the box faces and wrap),
   two registered
   fields (density f64, vec3 f64) travelling with the ghosts and returned by
   reverse_exchange(Sum); three steps.
-- cfg3: 16,777,216 particles in L = 256, periodic and with z non-periodic
+- uniform, 16,777,216 particles in L = 256, periodic and with z non-periodic
   (reflecting walls applied by the application); one full step.
 """
 import numpy as np
@@ -28,12 +28,12 @@ def gpu():
     return torch
 
 
-def test_cfg2_clusters_fields(gpu):
+def test_clustered_fields_1m(gpu):
     torch = gpu
     from dsdecompx_amd import _abi
     from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
     n, L, w = 1_048_576, 101.6, 2.8
-    say("cfg2", n)
+    say("clustered 1M", n)
     d = synth(n, L, 2002, clusters=64, sigma=4.0)
     rng = np.random.default_rng(2003)
     cap = int(n * 1.6)
@@ -48,7 +48,7 @@ def test_cfg2_clusters_fields(gpu):
     tv = torch.from_numpy(vec.copy()).cuda()
     halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
     for step in range(3):
-        say("cfg2 step", step)
+        say("clustered 1M step", step)
         disp = rng.normal(0, 0.5, (3, n))
         for a, k in enumerate(("x", "y", "z")):
             getattr(op, k)[:n] += disp[a]
@@ -60,7 +60,7 @@ def test_cfg2_clusters_fields(gpu):
         halo.migrate(dev, n)
         g = ora.exchange_halo(op, n, fields=[po.Field(dens), po.Field(vec)])
         assert halo.exchange_halo(dev, n, fields=[td, tv]) == g
-        compare_particles(dev, op, n + g, f"cfg2 step {step}")
+        compare_particles(dev, op, n + g, f"clustered step {step}")
         assert_bits_equal(host(td[:n + g]), dens[:n + g], "density")
         assert_bits_equal(host(tv[:n + g]).reshape(-1), vec[:n + g].reshape(-1), "vec")
         contrib = rng.uniform(0, 1, g)
@@ -80,16 +80,16 @@ def test_cfg2_clusters_fields(gpu):
         ora.reverse_halo(op, n)
         halo.reverse_halo(dev, n)
         for k in ("fx", "fy", "fz"):
-            assert_bits_equal(host(getattr(dev, k)[:n]), getattr(op, k)[:n], f"cfg2 reverse {k}")
+            assert_bits_equal(host(getattr(dev, k)[:n]), getattr(op, k)[:n], f"clustered reverse {k}")
 
 
 @pytest.mark.parametrize("zper", [True, False], ids=["periodic", "z_walls"])
-def test_cfg3_full_size_step(gpu, zper):
+def test_uniform_16m_full_size_step(gpu, zper):
     torch = gpu
     from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
     n, L, w = 16_777_216, 256.0, 2.8
     per = (True, True, zper)
-    say("cfg3 full size", per)
+    say("uniform 16M full size", per)
     rng = np.random.default_rng(303 if zper else 304)
     cap = int(n * 1.1)
     d = {k: rng.uniform(0, L, n) for k in ("x", "y", "z")}
@@ -115,7 +115,7 @@ def test_cfg3_full_size_step(gpu, zper):
     assert halo.exchange_halo(dev, n) == g
     expect = ((L + 2 * w) ** 2 * (L + (2 * w if zper else 0)) - L ** 3) / L ** 3 * n
     assert abs(g - expect) < 0.01 * expect, (g, expect)
-    compare_particles(dev, op, n + g, "cfg3 halo")
+    compare_particles(dev, op, n + g, "uniform 16M halo")
     f = rng.uniform(-1, 1, (3, n + g))
     for a, k in enumerate(("fx", "fy", "fz")):
         getattr(op, k)[:n + g] = f[a]
@@ -123,4 +123,4 @@ def test_cfg3_full_size_step(gpu, zper):
     ora.reverse_halo(op, n)
     halo.reverse_halo(dev, n)
     for k in ("fx", "fy", "fz"):
-        assert_bits_equal(host(getattr(dev, k)[:n]), getattr(op, k)[:n], f"cfg3 reverse {k}")
+        assert_bits_equal(host(getattr(dev, k)[:n]), getattr(op, k)[:n], f"uniform 16M reverse {k}")

@@ -17,7 +17,7 @@ CASES = [
     ("one", 1, 8.0, 1.0, (True, True, True), True, 0.0),
     ("small", 1000, 10.0, 1.5, (True, True, True), True, 0.0),
     ("ragged_chunk", 4099, 16.0, 2.8, (True, True, True), True, 0.0),
-    ("cfg1", 32768, 32.0, 2.8, (True, True, True), True, 0.0),
+    ("uniform_32k", 32768, 32.0, 2.8, (True, True, True), True, 0.0),
     ("z_nonperiodic", 20000, 27.0, 2.8, (True, True, False), True, 0.0),
     ("faces_only", 20000, 27.0, 2.8, (True, True, True), False, 0.0),
     ("shifted_box", 15000, 24.6, 2.8, (True, True, True), True, -7.3),
@@ -229,8 +229,8 @@ def test_refresh_matches_rewrite(gpu):
         compare_particles(dev, ref, n + g, f"refresh full={full}")
 
 
-def test_multi_step_cfg1(gpu):
-    """cfg1 loop: displace N(0,0.05), migrate, exchange_halo, forces U[-1,1]
+def test_multi_step_uniform_32k(gpu):
+    """32,768 particles in L = 32, 10 steps: displace N(0,0.05), migrate, exchange_halo, forces U[-1,1]
     on owned+ghosts, reverse_halo; every array bit-exact after every step."""
     import torch
     from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
@@ -243,7 +243,7 @@ def test_multi_step_cfg1(gpu):
     halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
     rng = np.random.default_rng(1234)
     for step in range(10):
-        say("cfg1 step", step)
+        say("32k step", step)
         disp = rng.normal(0, 0.05, (3, n))
         for a, k in enumerate(("x", "y", "z")):
             getattr(op, k)[:n] += disp[a]
@@ -330,7 +330,7 @@ def test_launch_shapes_and_walk_orders(gpu, monkeypatch, items, waves, descendin
     compare_particles(dev, op, n + g, f"refresh {items}x{waves} order {descending}")
 
 
-@pytest.mark.parametrize("case", [c for c in CASES if c[0] in ("cfg1", "z_nonperiodic",
+@pytest.mark.parametrize("case", [c for c in CASES if c[0] in ("uniform_32k", "z_nonperiodic",
                                                                  "shifted_box", "empty")],
                          ids=lambda c: c[0])
 def test_fused_migrate_build_equals_separate(gpu, case):
