@@ -514,7 +514,7 @@ def measure_traffic(n, L, w, sigma, periodic, env, seed):
     """HBM bytes per dispatch of every kernel of the step, from two rocprofv3
     counter runs of the C++ driver on the same workload (FETCH_SIZE, then
     WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
-    for the driver's plain copy of CALIBRATION_BYTES in the same run.
+    for comm_bench's plain copy of CALIBRATION_BYTES in the same run.
     None when rocprofv3 is not on PATH."""
     import glob
     import shutil
@@ -839,7 +839,7 @@ def run(args):
         zms = zr["hot_ms_per_step"]
         zab = algorithmic_bytes(n, zr["ghosts_per_step"], zr["owners"], 2)
         out["z_walls"] = {
-            "workload": f"{name} with z non-periodic (reflective walls at 0 and L in the driver)",
+            "workload": f"{name} with z non-periodic (reflective walls at 0 and L applied by comm_bench)",
             "value": n / (zms * 1e3), "unit": "Mparticles/s", "ms_per_step": zms,
             "ghosts_per_step": zr["ghosts_per_step"], "owners_per_step": zr["owners"],
             "step_hbm_frac_of_peak": sum(zab.values()) / (zms * 1e-3) / 1e9 / HBM_PEAK_GBS,
