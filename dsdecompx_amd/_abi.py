@@ -22,7 +22,9 @@ SUM, MIN, MAX, REPLACE = range(4)
 class Box(C.Structure):
     _fields_ = [("lower", C.c_double * 3), ("upper", C.c_double * 3),
                 ("periodic", C.c_int32 * 3), ("width", C.c_double),
-                ("edges_and_corners", C.c_int32)]
+                ("edges_and_corners", C.c_int32),
+                ("local_lower", C.c_double * 3), ("local_upper", C.c_double * 3),
+                ("local_given", C.c_int32)]
 
 
 _VP = C.c_void_p

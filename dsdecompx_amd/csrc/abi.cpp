@@ -8,7 +8,6 @@
 #include <cstring>
 #include <new>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "abi_internal.hpp"
@@ -84,12 +83,21 @@ int make_box_params(const dsdx_box* box, BoxParams* bp)
             return fail(DSDX_EINVAL, "global-domain extent must be positive");
     }
     if (!(box->width >= 0.0)) return fail(DSDX_EINVAL, "halo width cannot be negative");
+    if (box->local_given) {
+        for (int a = 0; a < 3; ++a)
+            if (!(box->local_upper[a] > box->local_lower[a]))
+                return fail(DSDX_EINVAL, "local-domain extent must be positive");
+    }
     std::memset(bp, 0, sizeof(*bp));
     double extent[3];
     for (int a = 0; a < 3; ++a) {
         extent[a] = box->upper[a] - box->lower[a];
-        bp->lo_thr[a] = box->lower[a] + box->width;
-        bp->hi_thr[a] = box->upper[a] - box->width;
+        // the local domain of the 1x1x1 grid: the cuts lower + extent * i / 1
+        // (src/topology.cpp:66-72) unless the caller gives them
+        const double cut_lo = box->local_given ? box->local_lower[a] : box->lower[a] + extent[a] * 0.0 / 1.0;
+        const double cut_hi = box->local_given ? box->local_upper[a] : box->lower[a] + extent[a] * 1.0 / 1.0;
+        bp->lo_thr[a] = cut_lo + box->width;
+        bp->hi_thr[a] = cut_hi - box->width;
         bp->lower[a] = box->lower[a];
         bp->extent[a] = extent[a];
         bp->periodic[a] = box->periodic[a] ? 1 : 0;
@@ -267,10 +275,18 @@ void number_tiles(Plan* plan)
     plan->tiles = next;
 }
 
-// Copies the table to the device and waits (the host copy may change after).
+// Copies the table to the device.  A single box's descriptor goes down as the
+// argument of a store kernel (stream-ordered, no host wait, capturable); a
+// batch's tables are copied and waited for (the host copy may change after).
 int upload_table(Plan* plan, hipStream_t stream)
 {
     DSDX_HIP(plan->d_desc.reserve(plan->desc.size() * sizeof(ReplicaDesc)), "hipMalloc");
+    if (plan->desc.size() == 1) {
+        DSDX_HIP(launch_store_desc(plan->desc[0], plan->d_desc.as<ReplicaDesc>(), stream),
+                 "store replica table");
+        plan->desc_on_device = true;
+        return DSDX_OK;
+    }
     DSDX_HIP(hipMemcpyAsync(plan->d_desc.ptr, plan->desc.data(), plan->desc.size() * sizeof(ReplicaDesc),
                             hipMemcpyHostToDevice, stream),
              "upload replica table");
@@ -389,9 +405,15 @@ int run_classify(Plan* plan, const dsdx_particles* p, bool wrap_first, hipStream
         DSDX_HIP(launch_scan(L, B, plan->shape, s), "launch halo scan");
     }
     if (plan->desc.size() == 1) {
-        if (!plan->counted)
-            DSDX_HIP(hipEventCreateWithFlags(&plan->counted, hipEventDisableTiming), "hipEventCreate");
-        DSDX_HIP(hipEventRecord(plan->counted, s), "record halo counts");
+        // the host waits for the counts on this event; a captured build is
+        // never waited for, so it records none
+        hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+        DSDX_HIP(hipStreamIsCapturing(s, &capture), "query stream capture");
+        if (capture == hipStreamCaptureStatusNone) {
+            if (!plan->counted)
+                DSDX_HIP(hipEventCreateWithFlags(&plan->counted, hipEventDisableTiming), "hipEventCreate");
+            DSDX_HIP(hipEventRecord(plan->counted, s), "record halo counts");
+        }
     }
     plan->classified = true;
     return DSDX_OK;
@@ -505,7 +527,10 @@ int dsdx_plan_reserve(dsdx_plan* plan, uint64_t owned, uint64_t ghosts)
     plan->chunks = static_cast<uint32_t>((owned + K - 1) / K);
     const int rc = reserve_buffers(plan, std::max<uint64_t>(ghosts, 1));
     plan->chunks = saved;
-    return rc;
+    if (rc) return rc;
+    // the single-box descriptor the builds store
+    DSDX_HIP(plan->d_desc.reserve(sizeof(ReplicaDesc)), "hipMalloc");
+    return DSDX_OK;
 }
 
 int dsdx_plan_set_shape(dsdx_plan* plan, int32_t items, int32_t waves, int32_t scan_threads)
@@ -539,28 +564,22 @@ int dsdx_migrate_wrap(const dsdx_box* box, double* x, double* y, double* z, uint
     if (owned && (!x || !y || !z)) return fail(DSDX_EINVAL, "incomplete device particle view");
     if (owned > kMaxOwned) return fail(DSDX_EINVAL, "too many particles for one device plan");
     if (owned == 0) return DSDX_OK;
-    // One-replica tables (the wrap needs no plan buffers), cached per thread
-    // by box and count so that a loop over many boxes uploads each table
-    // once.  Never freed: they live until the process ends, so no HIP call
-    // runs from a static destructor.
-    thread_local auto* cache = new std::unordered_map<std::string, Plan*>();
-    std::string key(reinterpret_cast<const char*>(&bp), sizeof(bp));
-    key.append(reinterpret_cast<const char*>(&owned), sizeof(owned));
-    const hipStream_t s = as_stream(stream);
-    Plan* wrap_plan = nullptr;
-    const auto found = cache->find(key);
-    if (found != cache->end()) {
-        wrap_plan = found->second;
-    } else {
-        if (cache->size() > 4096) cache->clear();   // bounded; the old tables stay allocated
-        wrap_plan = new Plan();
-        if (int rc = shape_from_env(&wrap_plan->shape)) return rc;
-        wrap_plan->items_auto = std::getenv("DSDX_ITEMS") == nullptr;
-        if (int rc = set_single(wrap_plan, bp, owned, owned, 0, s)) return rc;
-        (*cache)[key] = wrap_plan;
+    // The box goes to the kernel as an argument: no table, no device memory,
+    // no host wait.  Launch shape: the library default (or DSDX_*), chunk
+    // size by the particle count as for a plan.
+    Shape shape;
+    if (int rc = shape_from_env(&shape)) return rc;
+    if (!std::getenv("DSDX_ITEMS")) shape.items = auto_items(owned);
+    WrapBox wb;
+    for (int a = 0; a < 3; ++a) {
+        wb.lower[a] = bp.lower[a];
+        wb.extent[a] = bp.extent[a];
+        wb.periodic[a] = bp.periodic[a];
     }
+    wb.owned = static_cast<uint32_t>(owned);
+    const hipStream_t s = as_stream(stream);
     timing::Scope timed(timing::kMigrate, s);
-    DSDX_HIP(launch_migrate_wrap(x, y, z, wrap_plan->layout(), wrap_plan->shape, s), "launch migrate wrap");
+    DSDX_HIP(launch_wrap_box(x, y, z, wb, shape, s), "launch migrate wrap");
     return DSDX_OK;
 }
 
@@ -598,7 +617,14 @@ static int halo_build(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles
     const hipStream_t s = as_stream(stream);
 
     plan->classified = plan->built = plan->totals_cached = false;
-    const uint64_t guess = std::max<uint64_t>(plan->ghost_list_capacity, owned + 1024);
+    // The ghost list keeps its size (dsdx_plan_reserve, or the last growth);
+    // a build whose ghosts do not fit it raises the overflow status and the
+    // list grows to the known count in rescatter.  A fresh plan starts from
+    // the ghosts the storage could hold.
+    const uint64_t guess =
+        plan->ghost_list_capacity
+            ? plan->ghost_list_capacity
+            : std::max<uint64_t>(1024, std::min<uint64_t>(capacity - owned, owned + 1024));
     if (int rc = set_single(plan, bp, owned, capacity, guess, s)) return rc;
     if (int rc = run_classify(plan, particles, wrap_first, s)) return rc;
     if (int rc = run_scatter(plan, particles, fp, s)) return rc;

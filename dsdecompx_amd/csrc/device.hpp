@@ -78,6 +78,14 @@ struct ReplicaDesc {
     uint32_t pad;
 };
 
+// The geometry migrate's wrap needs for one box, passed by value.
+struct WrapBox {
+    double lower[3];
+    double extent[3];
+    int32_t periodic[3];
+    uint32_t owned;
+};
+
 // Bits of Shape::descending.
 constexpr int kOrderMigrate = 1;
 constexpr int kOrderClassify = 2;

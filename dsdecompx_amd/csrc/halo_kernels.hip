@@ -609,6 +609,32 @@ __global__ __launch_bounds__(WAVES * kWave) void ghost_owners_kernel(
 // migrate at P = 1 (the wrap itself is defined above classify).  One wave per
 // chunk, loads before compute; only a changed coordinate is stored.
 
+// The wrap of one chunk: particles [first, first + 64 * ITEMS) of a box with
+// `owned` particles whose arrays start at x, y, z.
+template <int ITEMS>
+__device__ __forceinline__ void wrap_chunk(double* x, double* y, double* z, uint32_t owned, uint32_t first,
+                                           const int32_t (&periodic)[3], const double (&lower)[3],
+                                           const double (&extent)[3])
+{
+    constexpr uint32_t K = ITEMS * kWave;
+    const bool full = first + K <= owned;
+    double* axis[3] = {x, y, z};
+    double val[3][ITEMS];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (periodic[a]) load_items<ITEMS>(axis[a], first, owned, full, val[a]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!periodic[a]) continue;
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t i = first + k * kWave + lane_id();
+            const double w = wrap(val[a][k], lower[a], extent[a]);
+            if (i < owned && __double_as_longlong(w) != __double_as_longlong(val[a][k])) axis[a][i] = w;
+        }
+    }
+}
+
 template <int ITEMS, int WAVES>
 __global__ __launch_bounds__(WAVES * kWave) void migrate_wrap_kernel(
     double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
@@ -618,27 +644,34 @@ __global__ __launch_bounds__(WAVES * kWave) void migrate_wrap_kernel(
     const uint32_t chunk = wave_chunk<WAVES>(L, L.chunks);
     if (chunk >= L.chunks) return;
     const ReplicaDesc* d = desc + replica_of(chunk_replica, chunk);
-    const uint32_t owned = d->owned;
     const uint64_t off = d->offset;
-    const uint32_t first = (chunk - d->first_chunk) * K;
-    const bool full = first + K <= owned;
-    double* axis[3] = {x + off, y + off, z + off};
-    double val[3][ITEMS];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-        if (d->bp.periodic[a]) load_items<ITEMS>(axis[a], first, owned, full, val[a]);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if (!d->bp.periodic[a]) continue;
-        const double lower = d->bp.lower[a];
-        const double extent = d->bp.extent[a];
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const uint32_t i = first + k * kWave + lane_id();
-            const double w = wrap(val[a][k], lower, extent);
-            if (i < owned && __double_as_longlong(w) != __double_as_longlong(val[a][k])) axis[a][i] = w;
-        }
-    }
+    wrap_chunk<ITEMS>(x + off, y + off, z + off, d->owned, (chunk - d->first_chunk) * K, d->bp.periodic,
+                      d->bp.lower, d->bp.extent);
+}
+
+// The same wrap for one box whose geometry comes in as a kernel argument:
+// no replica table, so the call needs no device memory of its own.
+template <int ITEMS, int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void wrap_box_kernel(
+    double* __restrict__ x, double* __restrict__ y, double* __restrict__ z, WrapBox b, Layout L)
+{
+    constexpr uint32_t K = ITEMS * kWave;
+    const uint32_t chunk = wave_chunk<WAVES>(L, L.chunks);
+    if (chunk >= L.chunks) return;
+    wrap_chunk<ITEMS>(x, y, z, b.owned, chunk * K, b.periodic, b.lower, b.extent);
+}
+
+// One replica descriptor into device memory, by one wave in stream order:
+// the launch carries the descriptor, so the host copy may change as soon as
+// the launch returns, no host wait is needed, and the store can be captured
+// into a graph.
+__global__ __launch_bounds__(kWave) void store_desc_kernel(ReplicaDesc d, ReplicaDesc* __restrict__ out)
+{
+    static_assert(sizeof(ReplicaDesc) % 8 == 0, "descriptor copied in 8-byte words");
+    constexpr uint32_t kWords = sizeof(ReplicaDesc) / 8;
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&d);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(out);
+    for (uint32_t w = lane_id(); w < kWords; w += kWave) dst[w] = src[w];
 }
 
 }  // namespace
@@ -776,6 +809,27 @@ hipError_t launch_migrate_wrap(double* x, double* y, double* z, const Layout& L,
         return launch(migrate_wrap_kernel<I, W>, chunk_grid(L, W), dim3(W * kWave), stream,
                       x, y, z, L.desc, L.chunk_replica, O);
     });
+}
+
+hipError_t launch_wrap_box(double* x, double* y, double* z, const WrapBox& b, const Shape& s,
+                           hipStream_t stream)
+{
+    const uint32_t K = static_cast<uint32_t>(s.items) * kWave;
+    Layout L{};
+    L.chunks = (b.owned + K - 1) / K;
+    L.chunk_size = K;
+    L.xcd_run = static_cast<uint32_t>(s.xcd_run);
+    const Layout O = ordered(L, s, kOrderMigrate);
+    if (L.chunks == 0) return hipSuccess;
+    return dispatch(s, [&](auto items, auto waves) {
+        constexpr int I = decltype(items)::value, W = decltype(waves)::value;
+        return launch(wrap_box_kernel<I, W>, chunk_grid(L, W), dim3(W * kWave), stream, x, y, z, b, O);
+    });
+}
+
+hipError_t launch_store_desc(const ReplicaDesc& d, ReplicaDesc* out, hipStream_t stream)
+{
+    return launch(store_desc_kernel, dim3(1), dim3(kWave), stream, d, out);
 }
 
 }  // namespace dsdx

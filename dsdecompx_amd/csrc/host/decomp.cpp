@@ -214,25 +214,42 @@ struct Decomp::Impl {
         if (plan) dsdx_plan_destroy(plan);
     }
 
+    // The two cut planes of each axis of the 1x1x1 grid, as a fresh topology
+    // places them: lower + extent * i / 1 (src/topology.cpp:66-72).  The
+    // upper cut can lie one ulp away from the domain's upper bound.
+    void place_cuts()
+    {
+        auto& cuts = detail::TopologyAccess::axis_cuts(topology);
+        for (int a = 0; a < 3; ++a) {
+            const double extent = config.domain.upper[a] - config.domain.lower[a];
+            cuts[a] = {config.domain.lower[a] + extent * 0.0 / 1.0,
+                       config.domain.lower[a] + extent * 1.0 / 1.0};
+        }
+    }
+
+    // Neighbours, shifts and the local domain from the domain and the cuts
+    // (the cuts themselves are placed once, then only rescaled).
     void refresh_geometry()
     {
         auto& dims = detail::TopologyAccess::dimensions(topology);
         auto& coords = detail::TopologyAccess::coordinates(topology);
         auto& local = detail::TopologyAccess::local_domain(topology);
         auto& nbs = detail::TopologyAccess::neighbors(topology);
-        auto& cuts = detail::TopologyAccess::axis_cuts(topology);
+        const auto& cuts = detail::TopologyAccess::axis_cuts(topology);
         dims = {1, 1, 1};
         coords = {0, 0, 0};
         detail::TopologyAccess::rank(topology) = 0;
         detail::TopologyAccess::size(topology) = 1;
         for (int a = 0; a < 3; ++a) {
-            local.lower[a] = config.domain.lower[a];
-            local.upper[a] = config.domain.upper[a];
-            cuts[a] = {config.domain.lower[a], config.domain.upper[a]};
+            local.lower[a] = cuts[a][0];
+            local.upper[a] = cuts[a][1];
             box.lower[a] = config.domain.lower[a];
             box.upper[a] = config.domain.upper[a];
             box.periodic[a] = config.domain.periodic[a] ? 1 : 0;
+            box.local_lower[a] = cuts[a][0];
+            box.local_upper[a] = cuts[a][1];
         }
+        box.local_given = 1;
         box.width = config.halo.width;
         box.edges_and_corners = config.halo.include_edges_and_corners ? 1 : 0;
         // src/topology.cpp:85-166 with every direction a self-neighbour
@@ -312,6 +329,7 @@ Decomp::Decomp(MPI_Comm communicator, Config config) : impl_(std::make_unique<Im
         throw std::runtime_error("dsDecompX: hipSetDevice failed");
     config.process_grid.dimensions = {1, 1, 1};
     impl_->config = std::move(config);
+    impl_->place_cuts();
     impl_->refresh_geometry();
     check(dsdx_plan_create(&impl_->plan));
 }
@@ -338,6 +356,16 @@ void Decomp::rescale_domain(const GlobalDomain& domain)
     for (int a = 0; a < 3; ++a)
         if (!(domain.upper[a] > domain.lower[a]))
             throw std::invalid_argument("dsDecompX: global-domain extent must be positive");
+    // The cuts move with the domain: each keeps its fraction of the extent
+    // (src/decomp.cpp:117-124); the plan keeps its ghost lists and takes the
+    // new extent's shifts at the next refresh.
+    auto& cuts = detail::TopologyAccess::axis_cuts(impl_->topology);
+    const GlobalDomain previous = impl_->config.domain;
+    for (int a = 0; a < 3; ++a) {
+        const double old_extent = previous.upper[a] - previous.lower[a];
+        const double new_extent = domain.upper[a] - domain.lower[a];
+        for (double& cut : cuts[a]) cut = domain.lower[a] + (cut - previous.lower[a]) / old_extent * new_extent;
+    }
     impl_->config.domain = domain;
     impl_->refresh_geometry();
     impl_->shifts_changed = true;

@@ -28,6 +28,11 @@ hipError_t launch_ghost_owners(int32_t* out, const Layout& L, const PlanBuffers&
                                hipStream_t stream);
 hipError_t launch_migrate_wrap(double* x, double* y, double* z, const Layout& L, const Shape& s,
                                hipStream_t stream);
+// migrate's wrap of one box given by value (no replica table).
+hipError_t launch_wrap_box(double* x, double* y, double* z, const WrapBox& b, const Shape& s,
+                           hipStream_t stream);
+// One replica descriptor stored into device memory by a kernel (stream-ordered, no host wait).
+hipError_t launch_store_desc(const ReplicaDesc& d, ReplicaDesc* out, hipStream_t stream);
 
 // Process grids (grid_kernels.hip).
 hipError_t launch_exchange_counts(const Layout& L, const PlanBuffers& B, uint32_t* host,

@@ -36,12 +36,16 @@ def _stream_ptr(stream) -> int:
 
 @dataclass
 class BoxConfig:
-    """GlobalDomain + HaloConfig (include/domain.hpp:11-30)."""
+    """GlobalDomain + HaloConfig (include/domain.hpp:11-30).  `local` =
+    (lower, upper) of the local domain whose faces the slab tests use; None:
+    the cuts of a fresh 1x1x1 topology, lower + extent * i / 1
+    (src/topology.cpp:66-72)."""
     lower: tuple = (0.0, 0.0, 0.0)
     upper: tuple = (1.0, 1.0, 1.0)
     periodic: tuple = (True, True, True)
     width: float = 0.0
     edges_and_corners: bool = True
+    local: tuple = None
 
     def c(self) -> _abi.Box:
         b = _abi.Box()
@@ -51,6 +55,11 @@ class BoxConfig:
             b.periodic[a] = 1 if self.periodic[a] else 0
         b.width = float(self.width)
         b.edges_and_corners = 1 if self.edges_and_corners else 0
+        if self.local is not None:
+            for a in range(3):
+                b.local_lower[a] = float(self.local[0][a])
+                b.local_upper[a] = float(self.local[1][a])
+            b.local_given = 1
         return b
 
 

@@ -72,14 +72,23 @@ typedef struct dsdx_field {
 } dsdx_field;
 
 /* Global domain and halo settings (GlobalDomain + HaloConfig,
- * include/domain.hpp:11-30).  On the 1x1x1 grid the local domain is the
- * global one. */
+ * include/domain.hpp:11-30).  The migrate wrap and the periodic shifts use
+ * the global domain.  The slab tests use the local domain of the 1x1x1 grid
+ * (Topology::local_domain, src/halo.cpp:200-205): with local_given == 0 it
+ * is the fresh topology's cut planes lower + extent * 0 / 1 and
+ * lower + extent * 1 / 1, extent = upper - lower (src/topology.cpp:66-72;
+ * the upper cut can differ from `upper` by one ulp); with local_given != 0
+ * it is local_lower / local_upper, e.g. the cuts that
+ * Decomp::rescale_domain rescaled in place (src/decomp.cpp:117-124). */
 typedef struct dsdx_box {
     double lower[3];
     double upper[3];
     int32_t periodic[3];
     double width;
     int32_t edges_and_corners;
+    double local_lower[3];
+    double local_upper[3];
+    int32_t local_given;
 } dsdx_box;
 
 /* Device-resident halo plan: per-chunk slab classification, per-slot send
@@ -97,16 +106,22 @@ int dsdx_plan_destroy(dsdx_plan* plan);
  * src/migration.cpp:355-359): reverse and forward exchanges become no-ops. */
 int dsdx_plan_reset(dsdx_plan* plan);
 
-/* Pre-sizes the plan's buffers for `owned` particles and `ghosts` ghosts, so
- * that a following dsdx_halo_build with a null `ghosts` pointer (e.g. inside
- * a captured hipGraph) allocates nothing. */
+/* Pre-sizes the plan's buffers for builds of `owned` particles with at most
+ * `ghosts` ghosts.  A following dsdx_halo_build / dsdx_migrate_halo_build of
+ * that many particles with a null `ghosts` pointer then allocates no memory
+ * and waits for nothing on the host (the box descriptor goes down as a kernel
+ * argument), so it can be captured into a hipGraph together with
+ * dsdx_migrate_wrap and dsdx_reverse_halo.  A build whose ghosts exceed the
+ * reserved count writes none of them and records the overflow
+ * (dsdx_plan_query); the reverse and forward exchanges then skip it. */
 int dsdx_plan_reserve(dsdx_plan* plan, uint64_t owned, uint64_t ghosts);
 
 /* Periodic wrap of the owned positions in place; replaces
  * launch_classify_destinations + launch_build_migration_layout + the unpack
  * (src/cuda/kernels.hpp:152-176) at P = 1, where every particle is retained
  * in order (src/migration.cpp:102-116, :437-479).  Non-periodic axes are left
- * unchanged. */
+ * unchanged.  One kernel launch with the box as its argument: no device
+ * memory of its own, no host wait (capturable). */
 int dsdx_migrate_wrap(const dsdx_box* box, double* x, double* y, double* z,
                       uint64_t owned, void* stream);
 

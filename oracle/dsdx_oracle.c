@@ -91,24 +91,32 @@ static int push_index(dsdxo_plan* plan, int n, int32_t value)
 }
 
 /* src/halo.cpp:164-238 (rebuild_halo_plan): six slab tests per owned
- * particle, strict comparisons against lower+width and upper-width, then
- * every active neighbour whose direction is covered.  Send lists keep owned
- * order.  Receive counts: src/backends/mpi_backend.cpp:366-375 (receive block
- * n is the send list of the neighbour in the opposite slot). */
+ * particle, strict comparisons against the local domain's lower+width and
+ * upper-width, then every active neighbour whose direction is covered.  Send
+ * lists keep owned order.  Receive counts: src/backends/mpi_backend.cpp:
+ * 366-375 (receive block n is the send list of the neighbour in the opposite
+ * slot). */
 int dsdxo_halo_build(const dsdxo_box* box, const dsdxo_neighbors* nb,
                      int64_t owned, const double* x, const double* y,
                      const double* z, dsdxo_plan* plan)
 {
     for (int n = 0; n < DSDXO_MAX_NEIGHBORS; ++n) plan->send_count[n] = 0;
     const double width = box->width;
+    /* local domain: the topology's cut planes (src/topology.cpp:66-72) */
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        const double extent = box->upper[a] - box->lower[a];
+        lo[a] = box->local_given ? box->local_lower[a] : box->lower[a] + extent * 0.0 / 1.0;
+        hi[a] = box->local_given ? box->local_upper[a] : box->lower[a] + extent * 1.0 / 1.0;
+    }
     for (int64_t i = 0; i < owned; ++i) {
         const double px = x[i], py = y[i], pz = z[i];
-        const int in_xlo = px < box->lower[0] + width;
-        const int in_xhi = px > box->upper[0] - width;
-        const int in_ylo = py < box->lower[1] + width;
-        const int in_yhi = py > box->upper[1] - width;
-        const int in_zlo = pz < box->lower[2] + width;
-        const int in_zhi = pz > box->upper[2] - width;
+        const int in_xlo = px < lo[0] + width;
+        const int in_xhi = px > hi[0] - width;
+        const int in_ylo = py < lo[1] + width;
+        const int in_yhi = py > hi[1] - width;
+        const int in_zlo = pz < lo[2] + width;
+        const int in_zhi = pz > hi[2] - width;
         if (!in_xlo && !in_xhi && !in_ylo && !in_yhi && !in_zlo && !in_zhi)
             continue;
         for (int n = 0; n < nb->count; ++n) {

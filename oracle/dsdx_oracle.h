@@ -41,13 +41,21 @@ extern "C" {
 #define DSDXO_MAX_NEIGHBORS 26
 #define DSDXO_MAX_FIELDS 16
 
-/* Global box, halo width and neighbour policy (include/domain.hpp:11-30). */
+/* Global box, halo width and neighbour policy (include/domain.hpp:11-30),
+ * and the local domain whose faces the slab tests use: with local_given == 0
+ * the cuts of a fresh 1x1x1 topology, lower + extent * 0 / 1 and
+ * lower + extent * 1 / 1 (src/topology.cpp:66-72), else local_lower /
+ * local_upper (a subdomain of a process grid, or cuts rescaled by
+ * rescale_domain, src/decomp.cpp:117-124). */
 typedef struct {
     double lower[3];
     double upper[3];
     int32_t periodic[3];
     double width;
     int32_t edges_and_corners;
+    double local_lower[3];
+    double local_upper[3];
+    int32_t local_given;
 } dsdxo_box;
 
 /* Neighbour table of the 1x1x1 grid (include/topology.hpp:14-20). */

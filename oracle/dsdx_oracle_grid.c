@@ -58,7 +58,10 @@ int dsdxo_grid_build(const dsdxo_box* box, const int32_t dims[3], int32_t lift_u
                     g->coord[r][a] = c[a];
                     local->lower[a] = g->cuts[a][c[a]];
                     local->upper[a] = g->cuts[a][c[a] + 1];
+                    local->local_lower[a] = local->lower[a];
+                    local->local_upper[a] = local->upper[a];
                 }
+                local->local_given = 1;
                 dsdxo_neighbors* nb = &g->nb[r];
                 int n = 0;
                 for (int dx = -1; dx <= 1; ++dx)

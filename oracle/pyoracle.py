@@ -33,7 +33,9 @@ FIELD_TYPE = {np.dtype(v): k for k, v in NUMPY_TYPE.items()}
 class _Box(C.Structure):
     _fields_ = [("lower", C.c_double * 3), ("upper", C.c_double * 3),
                 ("periodic", C.c_int32 * 3), ("width", C.c_double),
-                ("edges_and_corners", C.c_int32)]
+                ("edges_and_corners", C.c_int32),
+                ("local_lower", C.c_double * 3), ("local_upper", C.c_double * 3),
+                ("local_given", C.c_int32)]
 
 
 class _Neighbors(C.Structure):
@@ -177,12 +179,16 @@ def _ip(a):
 
 @dataclass
 class Box:
-    """Global domain + halo configuration (include/domain.hpp:11-30)."""
+    """Global domain + halo configuration (include/domain.hpp:11-30).
+    `local` = (lower, upper) of the local domain whose faces the slab tests
+    use; None: the cuts of a fresh 1x1x1 topology, lower + extent * i / 1
+    (src/topology.cpp:66-72)."""
     lower: tuple = (0.0, 0.0, 0.0)
     upper: tuple = (1.0, 1.0, 1.0)
     periodic: tuple = (True, True, True)
     width: float = 0.0
     edges_and_corners: bool = True
+    local: tuple = None
 
     def c(self) -> _Box:
         b = _Box()
@@ -192,6 +198,11 @@ class Box:
             b.periodic[a] = 1 if self.periodic[a] else 0
         b.width = float(self.width)
         b.edges_and_corners = 1 if self.edges_and_corners else 0
+        if self.local is not None:
+            for a in range(3):
+                b.local_lower[a] = float(self.local[0][a])
+                b.local_upper[a] = float(self.local[1][a])
+            b.local_given = 1
         return b
 
 

@@ -89,6 +89,38 @@ def test_reverse_halo_bit_exact(gpu, case):
         assert_bits_equal(host(getattr(dev, k)[:n + g]), getattr(op, k)[:n + g], k)
 
 
+def test_upper_cut_tie_known_answer(gpu):
+    """Box [9.8, 53.6), w = 1.1: the local domain's upper cut is
+    lower + extent = 53.599999999999994 (src/topology.cpp:66-72), so the
+    high threshold is 52.49999999999999 and particles at exactly 52.5 are in
+    the high slab.  Given cuts (local=) use the domain's own bounds instead,
+    where 52.5 is on the threshold.  Bit-exact against the oracle both ways,
+    with the known ghost counts of the tie particles."""
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    say("upper cut tie KAT")
+    lo, up, w = 9.8, 53.6, 1.1
+    n = 3000
+    d = synth(n, up - lo, 21, lower=lo)
+    mid = 30.0
+    ties = [(52.5, mid, mid), (mid, 52.5, mid), (mid, mid, 52.5), (52.5, 52.5, mid),
+            (52.49999999999999, mid, mid), (10.9, mid, mid)]
+    for k, (x, y, z) in enumerate(ties):
+        d["x"][k], d["y"][k], d["z"][k] = x, y, z
+    for local, tie_images in ((None, [1, 1, 1, 3, 0, 0]), (((lo,) * 3, (up,) * 3), [0, 0, 0, 0, 0, 0])):
+        box_o = po.Box(lower=(lo,) * 3, upper=(up,) * 3, width=w, local=local)
+        box_d = BoxConfig(lower=(lo,) * 3, upper=(up,) * 3, width=w, local=local)
+        ora = po.Oracle(box_o)
+        op = to_oracle(d, 3 * n)
+        g_ref = ora.exchange_halo(op, n)
+        dev = to_device(d, 3 * n)
+        halo = LoopbackHalo(box_d)
+        assert halo.exchange_halo(dev, n) == g_ref
+        compare_particles(dev, op, n + g_ref, f"tie box local={local}")
+        own = host(halo.ghost_owners(g_ref))
+        assert_bits_equal(own, ora.ghost_owners(), "ghost owners")
+        assert [int((own == k).sum()) for k in range(len(ties))] == tie_images
+
+
 def test_reverse_order_known_answer(gpu):
     """Catastrophic cancellation makes only ascending send-slot order give 4.75."""
     import torch

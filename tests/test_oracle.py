@@ -5,6 +5,7 @@ and the invariants the reference's own tests assert
 (examples/sanity_check/mpi/regression.cpp:46-101,
 examples/sanity_check/cuda/forward_exchange.cu:103-167)."""
 import itertools
+import math
 from fractions import Fraction
 
 import numpy as np
@@ -91,6 +92,38 @@ def test_exact_ties_follow_host_comparisons(oracle_lib):
     for pos, expected in cases:
         o = po.Oracle(box)
         assert o.exchange_halo(_make(box, [pos]), 1) == expected, pos
+
+
+def _rn(q):
+    """The double nearest to the rational q (ties to even): int / int true
+    division in CPython is correctly rounded."""
+    q = Fraction(q)
+    return q.numerator / q.denominator
+
+
+def test_upper_cut_is_lower_plus_extent(oracle_lib):
+    """The slab tests use the local domain's faces, the cuts lower + extent *
+    i / 1 (src/topology.cpp:66-72), not the domain's upper bound: for lower
+    9.8, upper 53.6 the upper cut is one ulp below 53.6, so with w = 1.1 the
+    high threshold is 52.49999999999999 and a particle at exactly 52.5 is in
+    the high slab.  Each value below is derived with one exact rounding per
+    operation (Fractions), not with the oracle's formula."""
+    lo, up, w = 9.8, 53.6, 1.1
+    extent = _rn(Fraction(up) - Fraction(lo))
+    cut_hi = _rn(Fraction(lo) + Fraction(_rn(Fraction(extent) * 1) / 1))
+    assert cut_hi == np.nextafter(up, 0.0)           # the cut differs from `upper`
+    hi_t = _rn(Fraction(cut_hi) - Fraction(w))
+    assert hi_t < 52.5 == _rn(Fraction(up) - Fraction(w))
+    box = po.Box(lower=(lo,) * 3, upper=(up,) * 3, width=w)
+    mid = 30.0
+    for pos, expected in (((52.5, mid, mid), 1), ((hi_t, mid, mid), 0),
+                          ((mid, 52.5, mid), 1), ((mid, mid, 52.5), 1),
+                          ((_rn(Fraction(lo) + Fraction(w)), mid, mid), 0)):
+        assert po.Oracle(box).exchange_halo(_make(box, [pos]), 1) == expected, pos
+    # given cuts replace the derived ones: with the domain's own bounds 52.5 is
+    # exactly on the threshold and has no image
+    given = po.Box(lower=(lo,) * 3, upper=(up,) * 3, width=w, local=((lo,) * 3, (up,) * 3))
+    assert po.Oracle(given).exchange_halo(_make(given, [(52.5, mid, mid)]), 1) == 0
 
 
 def test_wide_halo_particle_in_both_slabs(oracle_lib):
@@ -355,6 +388,45 @@ def test_migrate_wrap_congruence_exact(oracle_lib):
     w = orig.copy()
     po.lib().dsdxo_migrate_wrap(C_box(box0), len(w), po._dp(w), po._dp(w.copy()), po._dp(w.copy()))
     assert np.array_equal(w[inside], orig[inside])        # lower == 0: in-box values unchanged
+
+
+def test_migrate_wrap_rounding_sequence(oracle_lib):
+    """The wrap of src/migration.cpp:106-113 at a box whose extent is not a
+    power of two and whose lower bound is not zero (lower -3.7, upper 12.1):
+    v = x - lower; v -= floor(v / extent) * extent; v + lower, every
+    operation rounded on its own.  Expected values come from exact rationals
+    with one rounding per step.  The inputs are chosen so that a fused
+    multiply-subtract, or subtracting (lower + q * extent) in one go, gives a
+    different double for at least one of them, so this test fails for either
+    mistake."""
+    lower, upper = -3.7, 12.1
+    ext = _rn(Fraction(upper) - Fraction(lower))
+
+    def seq(x):
+        v = _rn(Fraction(x) - Fraction(lower))
+        q = math.floor(_rn(Fraction(v) / Fraction(ext)))
+        t = _rn(q * Fraction(ext))
+        return _rn(Fraction(_rn(Fraction(v) - Fraction(t))) + Fraction(lower)), v, q
+
+    def fused(x):
+        _, v, q = seq(x)
+        return _rn(Fraction(_rn(Fraction(v) - q * Fraction(ext))) + Fraction(lower))
+
+    def reordered(x):
+        _, v, q = seq(x)
+        return _rn(Fraction(x) - Fraction(_rn(Fraction(lower) + Fraction(_rn(q * Fraction(ext))))))
+
+    rng = np.random.default_rng(31)
+    xs = list(rng.uniform(lower - 200.0, upper + 200.0, 400)) + [-1e-17, lower, upper, 1000.3, -987.65]
+    expected = [seq(x)[0] for x in xs]
+    assert any(fused(x) != e for x, e in zip(xs, expected))
+    assert any(reordered(x) != e for x, e in zip(xs, expected))
+    x = np.array(xs)
+    box = po.Box(lower=(lower,) * 3, upper=(upper,) * 3)
+    y, z = x.copy(), x.copy()
+    po.lib().dsdxo_migrate_wrap(C_box(box), len(x), po._dp(x), po._dp(y), po._dp(z))
+    assert x.tolist() == expected
+    assert np.array_equal(x, y) and np.array_equal(x, z)
 
 
 def C_box(box):
