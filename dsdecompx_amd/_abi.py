@@ -85,6 +85,8 @@ def lib():
                                      u32, u64, u64, P, C.POINTER(u64)], C.c_int),
         "dsdx_halo_scatter": ([P, C.POINTER(Particles), C.POINTER(Field), u32, u64, u64, P],
                               C.c_int),
+        "dsdx_halo_build_noalloc": ([C.POINTER(Box), C.POINTER(Particles), C.POINTER(Field), u32,
+                                     u64, u64, P, P, C.POINTER(u64)], C.c_int),
         "dsdx_halo_refresh": ([P, C.POINTER(Box), C.POINTER(Particles), C.POINTER(Field), u32,
                                u64, i32, P], C.c_int),
         "dsdx_plan_query": ([P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u64),
@@ -167,7 +169,7 @@ def exported_symbols():
             "dsdx_unpack_records", "dsdx_unpack_positions", "dsdx_pack_reverse_records",
             "dsdx_accumulate_reverse_records", "dsdx_pack_field_records",
             "dsdx_scatter_field_records", "dsdx_gather_field_records", "dsdx_reduce_field_records",
-            "dsdx_loopback_exchange"]
+            "dsdx_loopback_exchange", "dsdx_halo_build_noalloc"]
 
 
 KERNELS = ("migrate_wrap", "halo_classify", "halo_scan", "halo_scatter", "halo_copy",

@@ -157,6 +157,29 @@ int dsdx_migrate_halo_build(dsdx_plan* plan, const dsdx_box* box,
                             uint64_t owned, uint64_t capacity, void* stream,
                             uint64_t* ghosts);
 
+/* dsdx_halo_build without a plan and without allocating: no counterpart in
+ * the reference.  For callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same kernels write the same ghosts, bit for bit; their plan state (per
+ * chunk entries and counts, the owner map, the ghost list) lives in one fixed
+ * device array of the library, about 13 MB, which each GPU the library runs
+ * on holds until the process ends.  Limits: at most DSDX_NOALLOC_MAX_OWNED
+ * owned particles (DSDX_EINVAL beyond) and DSDX_NOALLOC_MAX_GHOSTS ghosts
+ * (DSDX_ECAPACITY beyond, nothing written).  Every call on a GPU shares that
+ * array and keeps the last call's plan in it, so calls that overlap on one
+ * GPU race: order them on one stream.  The stream must belong to the
+ * current device.  `ghost_owners`, if non-null, is a device array of at
+ * least capacity - owned int32 that receives the owned index of every ghost
+ * in ghost order (as dsdx_plan_ghost_owners), enough for a caller's own
+ * reverse sum.  `ghosts` as for dsdx_halo_build: non-null waits for the count
+ * (and reports an overflow as DSDX_ECAPACITY), null waits for nothing. */
+#define DSDX_NOALLOC_MAX_OWNED 262144
+#define DSDX_NOALLOC_MAX_GHOSTS 524288
+int dsdx_halo_build_noalloc(const dsdx_box* box, const dsdx_particles* particles,
+                            const dsdx_field* fields, uint32_t field_count,
+                            uint64_t owned, uint64_t capacity,
+                            int32_t* ghost_owners, void* stream, uint64_t* ghosts);
+
 /* Writes the ghosts of the plan's last classification (after a
  * DSDX_ECAPACITY, with storage of at least owned + ghosts). */
 int dsdx_halo_scatter(dsdx_plan* plan, const dsdx_particles* particles,

@@ -58,6 +58,12 @@ def test_validation_errors_without_device(lib):
     fields = (_abi.Field * 17)()
     rc = lib.dsdx_forward_exchange(plan, fields, 17, 0, C.c_void_p(1))
     assert rc == _abi.EINVAL and b"16" in lib.dsdx_last_error()
+    # the allocation-free build checks its fixed limit before any device work
+    rc = lib.dsdx_halo_build_noalloc(C.byref(box), C.byref(p), None, 0, 262145, 10 ** 6, None,
+                                     C.c_void_p(1), None)
+    assert rc == _abi.EINVAL and b"262144" in lib.dsdx_last_error()
+    rc = lib.dsdx_halo_build_noalloc(C.byref(box), C.byref(p), None, 0, 10, 10, None, None, None)
+    assert rc == _abi.EINVAL and b"stream" in lib.dsdx_last_error()
     # no device halo yet: reverse and forward are no-ops, refresh is a state error
     assert lib.dsdx_reverse_halo(plan, None, None, None, 0, C.c_void_p(1)) == 0
     rc = lib.dsdx_halo_refresh(plan, None, C.byref(p), None, 0, 0, 0, C.c_void_p(1))
