@@ -37,13 +37,14 @@ WORKLOADS = {
 }
 
 
-def algorithmic_bytes(n, g, u, periodic_axes):
-    """Bytes each kernel must move by the path's definition: migrate reads
-    and writes x,y,z on every periodic axis (16 N per axis); the rebuild
-    reads x,y,z (24 N) in the classify pass and, per ghost, reads the owner's
-    72-byte record and writes the 96-byte ghost record in the copy pass (the
-    scan and the route pass move plan bookkeeping only); reverse_halo reads
-    24 B per ghost and reads + writes 48 B per owner."""
+def model_bytes(n, g, u, periodic_axes):
+    """The byte model the metric's roofline target is quoted on: migrate reads
+    and writes x, y, z on every periodic axis (16 N per axis); the rebuild
+    reads x, y, z (24 N) and, per ghost, the owner's 72-byte record and the
+    96-byte ghost record (168 G); reverse_halo reads 24 B per ghost and reads
+    + writes 48 B per owner.  It counts migrate's writes of every coordinate,
+    which the path does not make (only changed coordinates are stored), so
+    it is reported as the model's figure, never as a per-kernel fraction."""
     return {
         "migrate_wrap": 16.0 * n * periodic_axes,
         "halo_classify": 24.0 * n,
@@ -52,6 +53,32 @@ def algorithmic_bytes(n, g, u, periodic_axes):
         "halo_copy": 168.0 * g,
         "reverse_halo": 24.0 * g + 48.0 * u,
     }
+
+
+def required_bytes(n, g, u, periodic_axes, wrapped):
+    """Bytes each kernel must move: migrate reads the coordinates of every
+    periodic axis (8 N per axis) and writes the `wrapped` coordinates that
+    change (8 B each); the classification reads x, y, z (24 N); the ghost copy
+    reads the owner's 72-byte record and writes the 96-byte ghost record per
+    ghost (168 G); reverse_halo reads 24 B per ghost and reads + writes 48 B
+    per boundary owner; the scan and route passes move plan bookkeeping only.
+    The fused migrate + classification reads x, y, z once (24 N) and writes
+    the changed coordinates."""
+    return {
+        "migrate_wrap": 8.0 * n * periodic_axes + 8.0 * wrapped,
+        "halo_classify": 24.0 * n,
+        "halo_scan": 0.0,
+        "halo_scatter": 0.0,
+        "halo_copy": 168.0 * g,
+        "reverse_halo": 24.0 * g + 48.0 * u,
+        "migrate_classify": 24.0 * n + 8.0 * wrapped,
+    }
+
+
+def step_required(bytes_, fused):
+    """A step's required bytes: the split form's kernels, or the fused form's."""
+    skip = ("migrate_wrap", "halo_classify") if fused else ("migrate_classify",)
+    return sum(v for k, v in bytes_.items() if k not in skip)
 
 
 def cpu_baseline(name, n, L, w, sigma, periodic, steps, seed=7):
@@ -317,19 +344,23 @@ def measure_fields(args):
     registered fields (add_field) -- density (f64) and vec (3 x f64), F = 32 B
     per particle -- that travel into the ghosts with exchange_halo; per step
     migrate, exchange_halo, reverse_halo, reverse_exchange(Sum) of both
-    fields.  Algorithmic bytes per step: migrate 48 N, halo 24 N + (168 + 2F) G,
-    reverse 24 G + 48 U, reverse_exchange F G + 2F U."""
+    fields.  Required bytes per step: migrate 24 N + 8 B per rewritten
+    coordinate, halo 24 N + (168 + 2F) G, reverse 24 G + 48 U,
+    reverse_exchange F G + 2F U."""
     n, L, w, sigma = 1048576, 101.6, 2.8, 0.5
     res = run_comm_bench(args, n, L, w, sigma, (True, True, True), dict(os.environ), mode="fields")
     ms, G, U = res["hot_ms_per_step"], res["ghosts_per_step"], res["owners"]
+    C = res.get("wrapped_coords_per_step", 0.0)
     F = 32.0
-    step_bytes = 48.0 * n + 24.0 * n + (168.0 + 2 * F) * G + 24.0 * G + 48.0 * U + F * G + 2 * F * U
+    step_bytes = (24.0 * n + 8.0 * C + 24.0 * n + (168.0 + 2 * F) * G + 24.0 * G + 48.0 * U
+                  + F * G + 2 * F * U)
     return {
         "workload": f"{n} particles in 64 Gaussian clusters (sigma 4) in a periodic box L={L}, "
                     f"halo width {w}, displacement N(0,{sigma}) per axis per step, 2 registered "
                     f"fields (f64 density, 3 x f64 vector), reverse_exchange(Sum) of both",
         "value": n / (ms * 1e3), "unit": "Mparticles/s", "ms_per_step": ms,
-        "ghosts_per_step": G, "owners": U, "step_algorithmic_bytes": step_bytes,
+        "ghosts_per_step": G, "owners": U, "wrapped_coords_per_step": C,
+        "step_required_bytes": step_bytes,
         "step_hbm_frac_of_peak": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "kernels": {k: {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"]}
                     for k, v in res["kernels"].items() if v["launches"]},
@@ -751,17 +782,23 @@ def run(args):
     ms = res["hot_ms_per_step"]
     G = res["ghosts_per_step"]
     U = res["owners"]
+    C = res.get("wrapped_coords_per_step", 0.0)
     axes = sum(1 for a in periodic if a)
-    ab = algorithmic_bytes(n, G, U, axes)
-    step_bytes = sum(ab.values())
+    mb = model_bytes(n, G, U, axes)
+    step_bytes = sum(mb.values())
+    ab = required_bytes(n, G, U, axes, C)
+    step_req = step_required(ab, fused=False)
 
     per_kernel = {}
     for k, v in res["kernels"].items():
         if v["launches"]:
             per_kernel[k] = {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"]}
             if ab.get(k):
-                per_kernel[k]["algorithmic_GBps"] = ab[k] / (per_kernel[k]["avg_ms"] * 1e-3) / 1e9
-    dominant = max((k for k in per_kernel if k in ab), key=lambda k: per_kernel[k]["avg_ms"])
+                gbs = ab[k] / (per_kernel[k]["avg_ms"] * 1e-3) / 1e9
+                per_kernel[k]["required_bytes"] = ab[k]
+                per_kernel[k]["required_GBps"] = gbs
+                per_kernel[k]["frac_of_peak"] = gbs / HBM_PEAK_GBS
+    dominant = max((k for k in per_kernel if ab.get(k)), key=lambda k: per_kernel[k]["avg_ms"])
     dom_gbs = ab[dominant] / (per_kernel[dominant]["avg_ms"] * 1e-3) / 1e9
 
     out = {
@@ -791,12 +828,22 @@ def run(args):
             "algorithmic_bytes_per_launch": ab[dominant],
         },
         "step_hbm": {
-            "algorithmic_bytes": step_bytes,
+            "required_bytes": step_req,
+            "achieved_GBps": step_req / (ms * 1e-3) / 1e9,
+            "frac_of_peak": step_req / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "basis": "bytes each kernel must move (required_bytes in bench.py)",
+        },
+        "step_hbm_model": {
+            "model_bytes": step_bytes,
             "achieved_GBps": step_bytes / (ms * 1e-3) / 1e9,
             "frac_of_peak": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "target_ms_per_step_at_60pct": step_bytes / (0.6 * HBM_PEAK_GBS * 1e9) * 1e3,
+            "basis": "the metric's byte model (migrate 16 N per periodic axis, halo 24 N + 168 G, "
+                     "reverse 24 G + 48 U); it counts writes of unchanged coordinates",
         },
         "ghosts_per_step": G,
         "owners_per_step": U,
+        "wrapped_coords_per_step": C,
         "wall_ms_per_step": res["wall_ms_per_step"],
         "kernel_pass_ms_per_step": res.get("kernel_pass_hot_ms_per_step"),
         "kernels": per_kernel,
@@ -824,25 +871,37 @@ def run(args):
     fr = res.get("fused_run")
     if fr:
         fms = fr["hot_ms_per_step"]
-        fab = dict(ab)
-        fab["migrate_classify"] = fab.pop("migrate_wrap") + fab.pop("halo_classify")
+        fab = required_bytes(n, fr["ghosts_per_step"], fr["owners"], axes,
+                             fr.get("wrapped_coords_per_step", 0.0))
+        fkern = {}
+        for k, v in fr["kernels"].items():
+            if v["launches"]:
+                avg = v["total_ms"] / v["launches"]
+                fkern[k] = {"avg_ms": avg, "launches": v["launches"]}
+                if fab.get(k):
+                    fkern[k]["required_GBps"] = fab[k] / (avg * 1e-3) / 1e9
+                    fkern[k]["frac_of_peak"] = fkern[k]["required_GBps"] / HBM_PEAK_GBS
         out["fused"] = {
             "path": "C++ API extension Decomp::migrate_and_exchange_halo, then reverse_halo "
                     "(same results as the separate calls; one pass over the positions)",
             "value": n / (fms * 1e3), "unit": "Mparticles/s", "ms_per_step": fms,
-            "step_hbm_frac_of_peak": sum(fab.values()) / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "kernels": {k: {"avg_ms": v["total_ms"] / v["launches"], "launches": v["launches"]}
-                        for k, v in fr["kernels"].items() if v["launches"]},
+            "step_hbm_frac_of_peak": step_required(fab, fused=True) / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "step_required_bytes": step_required(fab, fused=True),
+            "step_hbm_model_frac_of_peak": step_bytes / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "kernels": fkern,
         }
     zr = res.get("zwalls_run")
     if zr:
         zms = zr["hot_ms_per_step"]
-        zab = algorithmic_bytes(n, zr["ghosts_per_step"], zr["owners"], 2)
+        zmb = model_bytes(n, zr["ghosts_per_step"], zr["owners"], 2)
+        zab = required_bytes(n, zr["ghosts_per_step"], zr["owners"], 2,
+                             zr.get("wrapped_coords_per_step", 0.0))
         out["z_walls"] = {
             "workload": f"{name} with z non-periodic (reflective walls at 0 and L applied by comm_bench)",
             "value": n / (zms * 1e3), "unit": "Mparticles/s", "ms_per_step": zms,
             "ghosts_per_step": zr["ghosts_per_step"], "owners_per_step": zr["owners"],
-            "step_hbm_frac_of_peak": sum(zab.values()) / (zms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "step_hbm_frac_of_peak": step_required(zab, fused=False) / (zms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "step_hbm_model_frac_of_peak": sum(zmb.values()) / (zms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         }
     rr = res.get("refresh_runs")
     if rr:

@@ -28,6 +28,7 @@
 // reverse_exchange(Sum) of both fields after reverse_halo.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -151,6 +152,21 @@ __global__ void count_boundary(const double* x, const double* y, const double* z
     if ((threadIdx.x & 63) == 0 && ballot) atomicAdd(out, static_cast<unsigned long long>(__popcll(ballot)));
 }
 
+// Coordinates that migrate's wrap rewrites: those outside [0, L) on a
+// periodic axis (the box starts at 0).
+__global__ void count_outside(const double* x, const double* y, const double* z, uint64_t n, double L,
+                              int zper, unsigned long long* out)
+{
+    const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    unsigned c = 0;
+    if (i < n) {
+        c = (x[i] < 0.0 || x[i] >= L) + (y[i] < 0.0 || y[i] >= L);
+        if (zper) c += (z[i] < 0.0 || z[i] >= L);
+    }
+    for (int d = 32; d > 0; d >>= 1) c += __shfl_xor(c, d, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, static_cast<unsigned long long>(c));
+}
+
 // A plain copy of known traffic (n doubles read, n written): rocprofv3's
 // FETCH_SIZE / WRITE_SIZE of this dispatch scale the counters of the halo
 // kernels in the same run to bytes.
@@ -224,11 +240,19 @@ int main(int argc, char** argv)
     uint64_t last_ghosts = 0;
     // One pass of `count` steps; returns the hot-path milliseconds summed over
     // the steps and adds each step's ghost count to *ghosts.
-    auto pass = [&](int count, bool timed, double* ghosts) {
+    // `wraps`: counts the coordinates each step's migrate rewrites (an extra
+    // kernel, so only in a pass of its own).
+    unsigned long long* d_wraps = nullptr;
+    ok(hipMalloc(&d_wraps, sizeof(*d_wraps)), "malloc");
+    ok(hipMemsetAsync(d_wraps, 0, sizeof(*d_wraps), s), "memset");
+    auto pass = [&](int count, bool timed, double* ghosts, bool wraps = false) {
         double hot = 0.0;
         for (int k = 0; k < count; ++k, ++step) {
             hipLaunchKernelGGL(move, dim3(grid(n)), dim3(256), 0, s, p.x_data(), p.y_data(), p.z_data(),
                                n, sigma, seed, step, zper ? 1 : 0, L);
+            if (wraps && !(refresh && step > 0))
+                hipLaunchKernelGGL(count_outside, dim3(grid(n)), dim3(256), 0, s, p.x_data(), p.y_data(),
+                                   p.z_data(), n, L, zper ? 1 : 0, d_wraps);
             if (timed) ok(hipEventRecord(ev[0], s), "record");
             HaloResult r;
             if (mode == "fused") {
@@ -280,6 +304,14 @@ int main(int argc, char** argv)
         dsdx_timing_enable(0);
     }
 
+    // Count pass: coordinates rewritten by migrate per step.
+    const int count_steps = std::min(steps, 20);
+    pass(count_steps, false, nullptr, true);
+    unsigned long long wraps = 0;
+    ok(hipMemcpyAsync(&wraps, d_wraps, sizeof(wraps), hipMemcpyDeviceToHost, s), "copy");
+    ok(hipStreamSynchronize(s), "sync");
+    ok(hipFree(d_wraps), "free");
+
     unsigned long long* d_u = nullptr;
     ok(hipMalloc(&d_u, sizeof(*d_u)), "malloc");
     ok(hipMemsetAsync(d_u, 0, sizeof(*d_u), s), "memset");
@@ -304,10 +336,11 @@ int main(int argc, char** argv)
     std::printf("{\"particles\": %llu, \"steps\": %d, \"warmup\": %d, \"hot_ms_per_step\": %.6f, "
                 "\"wall_ms_per_step\": %.6f, \"ghosts_per_step\": %.1f, \"last_ghosts\": %llu, "
                 "\"owners\": %llu, \"kernel_steps\": %d, \"kernel_pass_hot_ms_per_step\": %.6f, "
-                "\"kernels\": {",
+                "\"wrapped_coords_per_step\": %.1f, \"kernels\": {",
                 static_cast<unsigned long long>(n), steps, warmup, hot_ms / steps, wall_ms / steps,
                 ghosts_sum / steps, static_cast<unsigned long long>(last_ghosts), owners, kernel_steps,
-                kernel_steps > 0 ? kernel_pass_hot_ms / kernel_steps : 0.0);
+                kernel_steps > 0 ? kernel_pass_hot_ms / kernel_steps : 0.0,
+                count_steps > 0 ? static_cast<double>(wraps) / count_steps : 0.0);
     for (int k = 0; k < 9; ++k) {
         double ms = 0.0;
         uint64_t launches = 0;
