@@ -400,7 +400,7 @@ HaloResult Decomp::exchange_halo(ParticleInfo& particles, HaloExchangeOptions op
 bool Decomp::can_refresh_halo(const ParticleInfo& particles) const
 {
     const std::size_t owned = particles.owned_count();
-    ConstParticleView input = particles.read_view(owned);
+    ConstParticleView input = particles.const_view(owned);
     input.count = owned;
     if (input.memory_space == MemorySpace::Host) return false;
     return impl_->plan_valid && impl_->plan_owned == owned &&
@@ -420,7 +420,7 @@ void Decomp::begin_halo_exchange(ParticleInfo& particles, HaloExchangeOptions op
     if (m.operation_active) throw std::logic_error("dsDecompX: an operation is already in flight");
     if (options.rebuild) invalidate_halo();
     const std::size_t owned = particles.owned_count();
-    ConstParticleView input = particles.read_view(owned);
+    ConstParticleView input = particles.const_view(owned);
     input.count = owned;
     require_device(input.memory_space, "exchange_halo");
     if (!options.stream)
@@ -440,7 +440,7 @@ void Decomp::begin_halo_exchange(ParticleInfo& particles, HaloExchangeOptions op
     pending.compact = compact;
     pending.schema = schema;
     if (reuse) {
-        StorageResizeResult storage = particles.resize_storage(owned + m.plan_ghosts, owned);
+        StorageResizeResult storage = particles.ensure_capacity(owned + m.plan_ghosts, owned);
         if (storage.reallocated && compact)
             throw std::logic_error(
                 "dsDecompX: a positions-only halo refresh needed a larger particle storage; "
@@ -485,7 +485,7 @@ HaloResult Decomp::end_halo_exchange(ParticleInfo& particles)
         check(dsdx_plan_query(m.plan, nullptr, nullptr, &g, nullptr, &overflow, pending.options.stream));
         ghosts = static_cast<std::size_t>(g);
         if (overflow) {
-            StorageResizeResult storage = particles.resize_storage(owned + ghosts, owned);
+            StorageResizeResult storage = particles.ensure_capacity(owned + ghosts, owned);
             reallocated = storage.reallocated;
             ConstParticleView view = as_const_view(storage.view);
             std::vector<dsdx_field> fields = all_fields(view, owned + ghosts);
@@ -503,11 +503,11 @@ HaloResult Decomp::end_halo_exchange(ParticleInfo& particles)
     m.plan_reused = pending.reuse;
     const bool layout_changed = pending.options.rebuild || particles.ghost_count() != ghosts ||
                                 (ghosts != 0 && !pending.reuse);
-    particles.commit_counts(owned, ghosts);
+    particles.set_counts(owned, ghosts);
     if (m.perf_enabled) {
         ++m.stats.halo_calls;
         std::size_t record = pending.compact ? 3 * sizeof(double) : sizeof(ParticleRecord);
-        const ConstParticleView v = particles.read_view(owned + ghosts);
+        const ConstParticleView v = particles.const_view(owned + ghosts);
         if (!pending.compact)
             for (std::size_t f = 0; f < v.field_count; ++f) record += v.fields[f].record_bytes();
         m.stats.bytes_sent += ghosts * record;
@@ -537,7 +537,7 @@ HaloResult Decomp::migrate_and_exchange_halo(ParticleInfo& particles, HaloExchan
     if (m.operation_active)
         throw std::logic_error("dsDecompX: migration cannot overlap an active halo operation");
     const std::size_t owned = particles.owned_count();
-    ConstParticleView input = particles.read_view(owned);
+    ConstParticleView input = particles.const_view(owned);
     input.count = owned;
     require_device(input.memory_space, "migrate");
     check_migration_view(input, owned, options.stream);
@@ -545,8 +545,8 @@ HaloResult Decomp::migrate_and_exchange_halo(ParticleInfo& particles, HaloExchan
     // migrate()'s bookkeeping, then a rebuild whose classification pass wraps
     // the positions first (dsdx_migrate_halo_build).
     m.drop_device_halo();
-    particles.resize_storage(owned, 0);
-    particles.commit_counts(owned, 0);
+    particles.ensure_capacity(owned, 0);
+    particles.set_counts(owned, 0);
     if (m.perf_enabled) ++m.stats.migration_calls;
     options.rebuild = true;
     m.wrap_in_build = true;
@@ -565,7 +565,7 @@ MigrationResult Decomp::migrate(ParticleInfo& particles, ExecutionStream stream)
     if (m.operation_active)
         throw std::logic_error("dsDecompX: migration cannot overlap an active halo operation");
     const std::size_t owned = particles.owned_count();
-    ConstParticleView input = particles.read_view(owned);
+    ConstParticleView input = particles.const_view(owned);
     input.count = owned;
     require_device(input.memory_space, "migrate");
     check_migration_view(input, owned, stream);
@@ -576,8 +576,8 @@ MigrationResult Decomp::migrate(ParticleInfo& particles, ExecutionStream stream)
     check(dsdx_migrate_wrap(&m.box, const_cast<double*>(input.x), const_cast<double*>(input.y),
                             const_cast<double*>(input.z), owned, stream));
     m.drop_device_halo();
-    particles.resize_storage(owned, 0);
-    particles.commit_counts(owned, 0);
+    particles.ensure_capacity(owned, 0);
+    particles.set_counts(owned, 0);
     if (m.perf_enabled) ++m.stats.migration_calls;
     MigrationResult result;
     result.previous_owned_count = owned;
@@ -600,7 +600,7 @@ void Decomp::reverse_halo(ParticleInfo& particles, ExecutionStream stream)
     Impl& m = *impl_;
     if (m.operation_active)
         throw std::logic_error("dsDecompX: reverse halo cannot overlap an active operation");
-    StorageResizeResult storage = particles.resize_storage(particles.size(), particles.size());
+    StorageResizeResult storage = particles.ensure_capacity(particles.size(), particles.size());
     require_device(storage.view.memory_space, "reverse_halo");
     if (!stream) throw std::invalid_argument("dsDecompX: device reverse halo requires an explicit stream");
     if (!m.device_halo) return;
@@ -618,7 +618,7 @@ void Decomp::reverse_exchange(ParticleInfo& particles,
     Impl& m = *impl_;
     if (m.operation_active)
         throw std::logic_error("dsDecompX: reverse exchange cannot overlap an active operation");
-    StorageResizeResult storage = particles.resize_storage(particles.size(), particles.size());
+    StorageResizeResult storage = particles.ensure_capacity(particles.size(), particles.size());
     require_device(storage.view.memory_space, "reverse_exchange");
     if (!stream)
         throw std::invalid_argument("dsDecompX: device reverse exchange requires an explicit stream");
@@ -646,7 +646,7 @@ void Decomp::forward_exchange(ParticleInfo& particles, const std::vector<std::st
     Impl& m = *impl_;
     if (m.operation_active)
         throw std::logic_error("dsDecompX: forward exchange cannot overlap an active operation");
-    StorageResizeResult storage = particles.resize_storage(particles.size(), particles.size());
+    StorageResizeResult storage = particles.ensure_capacity(particles.size(), particles.size());
     require_device(storage.view.memory_space, "forward_exchange");
     if (!stream)
         throw std::invalid_argument("dsDecompX: device forward exchange requires an explicit stream");

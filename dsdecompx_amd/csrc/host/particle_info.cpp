@@ -1,363 +1,455 @@
-// particle_info.cpp -- ParticleInfo storage (host vectors, library-owned HIP
-// device/managed arrays, borrowed views and providers).  Semantics of the
-// reference's src/runtime.cpp:451-755: capacity, growth by at least 1.5x
-// (minimum 64) preserving the first `preserve` particles, borrowed storage
-// that is too small is a length_error unless a Provider grows it.
+// particle_info.cpp -- where a ParticleInfo keeps its 13 particle arrays.
+//
+// Three kinds of backing:
+//   host vectors  the public std::vector members (host storage; the GPU
+//                 operations reject it);
+//   borrowed      arrays the caller owns: a view, external() plus the pointer
+//                 setters, or a Provider that can hand out larger ones;
+//   block         library-owned device (or managed) memory, one allocation per
+//                 capacity holding all 13 arrays back to back, each starting
+//                 on a 256-byte boundary.
+// A block grows to the required particle count plus a quarter, rounded up to
+// whole pages of 1024 particles, and keeps the first `preserve` particles of
+// every array.  Growth waits for the device first: work queued on any stream
+// may still read the old arrays.  Registered fields are borrowed in every
+// case; views handed out carry them.  The exception types are the
+// reference's (invalid_argument for bad sizes and views, length_error for
+// storage that cannot hold the particles, logic_error for resizing borrowed
+// arrays, runtime_error for device failures).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <new>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 #include "dsdecompx.hpp"
 
 namespace dsdecompx {
+
 namespace {
 
-void hip_require(hipError_t status, const char* what)
+enum class Backing { HostVectors, Borrowed, Block };
+
+constexpr std::size_t kPage = 1024;   // particles
+constexpr std::size_t kAlign = 256;   // bytes
+
+void hip_check(hipError_t status, const char* what)
 {
     if (status != hipSuccess)
-        throw std::runtime_error(std::string("dsDecompX HIP: ") + what + ": " +
-                                 hipGetErrorString(status));
+        throw std::runtime_error(std::string("dsDecompX HIP: ") + what + ": " + hipGetErrorString(status));
 }
 
-template <class T> void device_alloc(MemorySpace space, T** p, std::size_t count)
+// The arrays of a view in a fixed order: 11 of doubles, then 2 of ints.
+constexpr double* ParticleView::*kDoubles[11] = {
+    &ParticleView::x, &ParticleView::y, &ParticleView::z, &ParticleView::vx, &ParticleView::vy,
+    &ParticleView::vz, &ParticleView::fx, &ParticleView::fy, &ParticleView::fz, &ParticleView::mass,
+    &ParticleView::charge};
+constexpr int* ParticleView::*kInts[2] = {&ParticleView::type, &ParticleView::global_id};
+
+std::size_t round_up(std::size_t v, std::size_t to) { return (v + to - 1) / to * to; }
+
+std::size_t block_bytes(std::size_t capacity)
 {
-    const std::size_t bytes = count * sizeof(T);
-    if (space == MemorySpace::CudaManaged)
-        hip_require(hipMallocManaged(reinterpret_cast<void**>(p), bytes), "hipMallocManaged");
-    else
-        hip_require(hipMalloc(reinterpret_cast<void**>(p), bytes), "hipMalloc");
+    return 11 * round_up(capacity * sizeof(double), kAlign) + 2 * round_up(capacity * sizeof(int), kAlign);
 }
 
-template <class T> void device_free(T*& p) noexcept
-{
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
-void free_device_view(ParticleView& v) noexcept
-{
-    device_free(v.x); device_free(v.y); device_free(v.z);
-    device_free(v.vx); device_free(v.vy); device_free(v.vz);
-    device_free(v.fx); device_free(v.fy); device_free(v.fz);
-    device_free(v.mass); device_free(v.charge);
-    device_free(v.type); device_free(v.global_id);
-    v = ParticleView{};
-}
-
-ParticleView allocate_device_view(MemorySpace space, std::size_t capacity)
+// The arrays of a block of `capacity` particles at `base`.
+ParticleView carve(void* base, std::size_t capacity, MemorySpace space)
 {
     ParticleView v;
     v.memory_space = space;
     v.count = capacity;
-    if (capacity == 0) return v;
-    try {
-        device_alloc(space, &v.x, capacity); device_alloc(space, &v.y, capacity);
-        device_alloc(space, &v.z, capacity); device_alloc(space, &v.vx, capacity);
-        device_alloc(space, &v.vy, capacity); device_alloc(space, &v.vz, capacity);
-        device_alloc(space, &v.fx, capacity); device_alloc(space, &v.fy, capacity);
-        device_alloc(space, &v.fz, capacity); device_alloc(space, &v.mass, capacity);
-        device_alloc(space, &v.charge, capacity); device_alloc(space, &v.type, capacity);
-        device_alloc(space, &v.global_id, capacity);
-    } catch (...) {
-        free_device_view(v);
-        throw;
+    auto* at = static_cast<unsigned char*>(base);
+    for (auto member : kDoubles) {
+        v.*member = reinterpret_cast<double*>(at);
+        at += round_up(capacity * sizeof(double), kAlign);
+    }
+    for (auto member : kInts) {
+        v.*member = reinterpret_cast<int*>(at);
+        at += round_up(capacity * sizeof(int), kAlign);
     }
     return v;
 }
 
-void copy_device_view(const ParticleView& from, ParticleView& to, std::size_t count)
+void* allocate_block(MemorySpace space, std::size_t capacity)
 {
-    if (!count) return;
-    auto copy = [count](auto* dst, const auto* src) {
-        using T = std::remove_pointer_t<decltype(dst)>;
-        hip_require(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyDefault),
-                    "hipMemcpy while growing ParticleInfo");
-    };
-    copy(to.x, from.x); copy(to.y, from.y); copy(to.z, from.z);
-    copy(to.vx, from.vx); copy(to.vy, from.vy); copy(to.vz, from.vz);
-    copy(to.fx, from.fx); copy(to.fy, from.fy); copy(to.fz, from.fz);
-    copy(to.mass, from.mass); copy(to.charge, from.charge);
-    copy(to.type, from.type); copy(to.global_id, from.global_id);
+    void* p = nullptr;
+    if (space == MemorySpace::CudaManaged)
+        hip_check(hipMallocManaged(&p, block_bytes(capacity)), "allocate managed particle storage");
+    else
+        hip_check(hipMalloc(&p, block_bytes(capacity)), "allocate device particle storage");
+    return p;
 }
 
-void attach_fields(ParticleView& view, std::vector<ParticleFieldView>& fields)
+// Capacity of a block that has to hold `required` particles.
+std::size_t block_capacity(std::size_t required)
 {
-    view.fields = fields.empty() ? nullptr : fields.data();
-    view.field_count = fields.size();
+    return std::max(kPage, round_up(required + required / 4, kPage));
 }
 
 }  // namespace
 
-ParticleInfo::~ParticleInfo() { release_device_memory(); }
+struct ParticleInfo::State {
+    Backing backing = Backing::HostVectors;
+    ParticleView arrays;              // borrowed or block: pointers, capacity, space (no fields)
+    void* block = nullptr;            // block: the allocation
+    Provider* provider = nullptr;     // borrowed: can replace the arrays with larger ones
+    std::vector<ParticleFieldView> fields;
+    std::size_t owned = 0;
+    std::size_t ghosts = 0;
 
-ParticleInfo::ParticleInfo(ParticleInfo&& other) noexcept { *this = std::move(other); }
+    State() = default;
+    State(const State&) = delete;
+    State& operator=(const State&) = delete;
+    ~State() { drop_block(); }
+
+    void drop_block() noexcept
+    {
+        if (block) (void)hipFree(block);
+        block = nullptr;
+    }
+
+    ParticleView view() const
+    {
+        ParticleView v = arrays;
+        v.fields = fields.empty() ? nullptr : const_cast<ParticleFieldView*>(fields.data());
+        v.field_count = fields.size();
+        return v;
+    }
+
+    void adopt(const ParticleView& v)
+    {
+        arrays = v;
+        arrays.fields = nullptr;
+        arrays.field_count = 0;
+        fields.clear();
+        if (v.fields && v.field_count) fields.assign(v.fields, v.fields + v.field_count);
+    }
+
+    void require_field_capacity(std::size_t required) const
+    {
+        for (const ParticleFieldView& f : fields)
+            if (f.capacity < required)
+                throw std::length_error("dsDecompX: registered field '" + f.name + "' holds fewer than " +
+                                        std::to_string(required) + " particles");
+    }
+
+    // A new block of room for `required` particles with the first `preserve`
+    // of every array copied over.
+    void grow_block(std::size_t required, std::size_t preserve)
+    {
+        const std::size_t capacity = block_capacity(required);
+        const MemorySpace space = arrays.memory_space;
+        hip_check(hipDeviceSynchronize(), "wait before moving particle storage");
+        void* fresh = allocate_block(space, capacity);
+        const ParticleView next = carve(fresh, capacity, space);
+        const std::size_t keep = block ? std::min(preserve, arrays.count) : 0;
+        if (keep) {
+            for (auto member : kDoubles) {
+                const hipError_t e = hipMemcpy(next.*member, arrays.*member, keep * sizeof(double),
+                                               hipMemcpyDeviceToDevice);
+                if (e != hipSuccess) {
+                    (void)hipFree(fresh);
+                    hip_check(e, "copy particle storage");
+                }
+            }
+            for (auto member : kInts) {
+                const hipError_t e = hipMemcpy(next.*member, arrays.*member, keep * sizeof(int),
+                                               hipMemcpyDeviceToDevice);
+                if (e != hipSuccess) {
+                    (void)hipFree(fresh);
+                    hip_check(e, "copy particle storage");
+                }
+            }
+        }
+        drop_block();
+        block = fresh;
+        arrays = next;
+    }
+};
+
+ParticleInfo::State& ParticleInfo::state()
+{
+    if (!state_) state_ = std::make_unique<State>();
+    return *state_;
+}
+
+ParticleInfo::ParticleInfo() : state_(std::make_unique<State>()) {}
+
+ParticleInfo::ParticleInfo(std::size_t owned) : state_(std::make_unique<State>()) { resize_owned(owned); }
+
+ParticleInfo::~ParticleInfo() = default;
+
+ParticleInfo::ParticleInfo(ParticleInfo&& other) noexcept
+    : x(std::move(other.x)), y(std::move(other.y)), z(std::move(other.z)), vx(std::move(other.vx)),
+      vy(std::move(other.vy)), vz(std::move(other.vz)), fx(std::move(other.fx)), fy(std::move(other.fy)),
+      fz(std::move(other.fz)), mass(std::move(other.mass)), charge(std::move(other.charge)),
+      type(std::move(other.type)), global_id(std::move(other.global_id)),
+      num_particles(std::exchange(other.num_particles, 0)), state_(std::move(other.state_))
+{
+}
 
 ParticleInfo& ParticleInfo::operator=(ParticleInfo&& other) noexcept
 {
-    if (this == &other) return *this;
-    release_device_memory();
-    x = std::move(other.x); y = std::move(other.y); z = std::move(other.z);
-    vx = std::move(other.vx); vy = std::move(other.vy); vz = std::move(other.vz);
-    fx = std::move(other.fx); fy = std::move(other.fy); fz = std::move(other.fz);
-    mass = std::move(other.mass); charge = std::move(other.charge);
-    type = std::move(other.type); global_id = std::move(other.global_id);
-    num_particles = other.num_particles;
-    external_ = other.external_;
-    owns_device_memory_ = other.owns_device_memory_;
-    owned_count_ = other.owned_count_;
-    ghost_count_ = other.ghost_count_;
-    fields_ = std::move(other.fields_);
-    storage_ = other.storage_;
-    provider_ = other.provider_;
-    attach_fields(storage_, fields_);
-    other.num_particles = 0;
-    other.external_ = other.owns_device_memory_ = false;
-    other.owned_count_ = other.ghost_count_ = 0;
-    other.fields_.clear();
-    other.storage_ = ParticleView{};
-    other.provider_ = nullptr;
+    if (this != &other) {
+        x = std::move(other.x); y = std::move(other.y); z = std::move(other.z);
+        vx = std::move(other.vx); vy = std::move(other.vy); vz = std::move(other.vz);
+        fx = std::move(other.fx); fy = std::move(other.fy); fz = std::move(other.fz);
+        mass = std::move(other.mass); charge = std::move(other.charge);
+        type = std::move(other.type); global_id = std::move(other.global_id);
+        num_particles = std::exchange(other.num_particles, 0);
+        state_ = std::move(other.state_);   // frees this object's block, if any
+    }
     return *this;
 }
 
-void ParticleInfo::release_device_memory() noexcept
-{
-    if (!owns_device_memory_) return;
-    free_device_view(storage_);
-    owns_device_memory_ = false;
-}
-
-ParticleInfo ParticleInfo::allocate(MemorySpace memory_space, std::size_t owned)
-{
-    if (memory_space == MemorySpace::Host) return ParticleInfo(owned);
-    ParticleInfo result;
-    result.external_ = true;
-    result.owns_device_memory_ = true;
-    result.owned_count_ = owned;
-    result.storage_ = allocate_device_view(memory_space, owned);
-    return result;
-}
-
 ParticleInfo::ParticleInfo(std::size_t owned, std::size_t ghosts, ParticleView storage)
-    : external_(true), owned_count_(owned), ghost_count_(ghosts), storage_(storage)
+    : state_(std::make_unique<State>())
 {
-    if (storage_.count < owned + ghosts)
-        throw std::invalid_argument(
-            "dsDecompX: external particle view is smaller than its particle counts");
-    if (storage.fields && storage.field_count)
-        fields_.assign(storage.fields, storage.fields + storage.field_count);
-    attach_fields(storage_, fields_);
+    if (storage.count < owned + ghosts)
+        throw std::invalid_argument("dsDecompX: a particle view of " + std::to_string(storage.count) +
+                                    " slots cannot hold " + std::to_string(owned + ghosts) + " particles");
+    state_->backing = Backing::Borrowed;
+    state_->adopt(storage);
+    state_->owned = owned;
+    state_->ghosts = ghosts;
 }
 
 ParticleInfo::ParticleInfo(std::size_t owned, std::size_t ghosts, Provider& provider)
     : ParticleInfo(owned, ghosts, provider.particle_info_view(owned + ghosts))
 {
-    provider_ = &provider;
+    state_->provider = &provider;
 }
 
-ParticleInfo ParticleInfo::external(MemorySpace memory_space, std::size_t capacity,
-                                    std::size_t owned, std::size_t ghosts)
+ParticleInfo ParticleInfo::allocate(MemorySpace memory_space, std::size_t owned)
 {
-    ParticleView view;
-    view.memory_space = memory_space;
-    view.count = capacity;
-    return ParticleInfo(owned, ghosts, view);
+    if (memory_space == MemorySpace::Host) return ParticleInfo(owned);
+    ParticleInfo p;
+    State& s = *p.state_;
+    s.backing = Backing::Block;
+    s.arrays.memory_space = memory_space;
+    s.grow_block(owned, 0);
+    s.owned = owned;
+    return p;
+}
+
+ParticleInfo ParticleInfo::external(MemorySpace memory_space, std::size_t capacity, std::size_t owned,
+                                    std::size_t ghosts)
+{
+    ParticleView v;
+    v.memory_space = memory_space;
+    v.count = capacity;
+    return ParticleInfo(owned, ghosts, v);
 }
 
 ParticleInfo& ParticleInfo::positions(double* px, double* py, double* pz) noexcept
 {
-    storage_.x = px; storage_.y = py; storage_.z = pz;
+    if (!state_) state_.reset(new (std::nothrow) State());
+    if (state_) {
+        state_->arrays.x = px; state_->arrays.y = py; state_->arrays.z = pz;
+    }
     return *this;
 }
 
 ParticleInfo& ParticleInfo::velocities(double* pvx, double* pvy, double* pvz) noexcept
 {
-    storage_.vx = pvx; storage_.vy = pvy; storage_.vz = pvz;
+    if (!state_) state_.reset(new (std::nothrow) State());
+    if (state_) {
+        state_->arrays.vx = pvx; state_->arrays.vy = pvy; state_->arrays.vz = pvz;
+    }
     return *this;
 }
 
 ParticleInfo& ParticleInfo::forces(double* pfx, double* pfy, double* pfz) noexcept
 {
-    storage_.fx = pfx; storage_.fy = pfy; storage_.fz = pfz;
+    if (!state_) state_.reset(new (std::nothrow) State());
+    if (state_) {
+        state_->arrays.fx = pfx; state_->arrays.fy = pfy; state_->arrays.fz = pfz;
+    }
     return *this;
 }
 
 ParticleInfo& ParticleInfo::properties(double* pm, double* pq, int* pt, int* pid) noexcept
 {
-    storage_.mass = pm; storage_.charge = pq; storage_.type = pt; storage_.global_id = pid;
+    if (!state_) state_.reset(new (std::nothrow) State());
+    if (state_) {
+        state_->arrays.mass = pm; state_->arrays.charge = pq;
+        state_->arrays.type = pt; state_->arrays.global_id = pid;
+    }
     return *this;
 }
 
-std::size_t ParticleInfo::capacity() const noexcept { return external_ ? storage_.count : x.size(); }
+std::size_t ParticleInfo::owned_count() const noexcept { return state_ ? state_->owned : 0; }
+std::size_t ParticleInfo::ghost_count() const noexcept { return state_ ? state_->ghosts : 0; }
+std::size_t ParticleInfo::size() const noexcept { return owned_count() + ghost_count(); }
+
+std::size_t ParticleInfo::capacity() const noexcept
+{
+    return state_ && state_->backing != Backing::HostVectors ? state_->arrays.count : x.size();
+}
 
 MemorySpace ParticleInfo::memory_space() const noexcept
 {
-    return external_ ? storage_.memory_space : MemorySpace::Host;
+    return state_ && state_->backing != Backing::HostVectors ? state_->arrays.memory_space
+                                                              : MemorySpace::Host;
 }
 
-double* ParticleInfo::x_data() noexcept { return external_ ? storage_.x : x.data(); }
-double* ParticleInfo::y_data() noexcept { return external_ ? storage_.y : y.data(); }
-double* ParticleInfo::z_data() noexcept { return external_ ? storage_.z : z.data(); }
-double* ParticleInfo::vx_data() noexcept { return external_ ? storage_.vx : vx.data(); }
-double* ParticleInfo::vy_data() noexcept { return external_ ? storage_.vy : vy.data(); }
-double* ParticleInfo::vz_data() noexcept { return external_ ? storage_.vz : vz.data(); }
-double* ParticleInfo::fx_data() noexcept { return external_ ? storage_.fx : fx.data(); }
-double* ParticleInfo::fy_data() noexcept { return external_ ? storage_.fy : fy.data(); }
-double* ParticleInfo::fz_data() noexcept { return external_ ? storage_.fz : fz.data(); }
-double* ParticleInfo::mass_data() noexcept { return external_ ? storage_.mass : mass.data(); }
-double* ParticleInfo::charge_data() noexcept { return external_ ? storage_.charge : charge.data(); }
-int* ParticleInfo::type_data() noexcept { return external_ ? storage_.type : type.data(); }
-int* ParticleInfo::global_id_data() noexcept
+bool ParticleInfo::uses_external_memory() const noexcept
 {
-    return external_ ? storage_.global_id : global_id.data();
+    return state_ && state_->backing == Backing::Borrowed;
 }
 
-void ParticleInfo::commit_counts(std::size_t owned, std::size_t ghosts) noexcept
+// Host storage hands out the vectors; every other backing its arrays.
+#define DSDX_ARRAY_ACCESSOR(T, name)                                                     \
+    T* ParticleInfo::name##_data() noexcept                                             \
+    {                                                                                    \
+        return state_ && state_->backing != Backing::HostVectors ? state_->arrays.name : name.data(); \
+    }
+DSDX_ARRAY_ACCESSOR(double, x)
+DSDX_ARRAY_ACCESSOR(double, y)
+DSDX_ARRAY_ACCESSOR(double, z)
+DSDX_ARRAY_ACCESSOR(double, vx)
+DSDX_ARRAY_ACCESSOR(double, vy)
+DSDX_ARRAY_ACCESSOR(double, vz)
+DSDX_ARRAY_ACCESSOR(double, fx)
+DSDX_ARRAY_ACCESSOR(double, fy)
+DSDX_ARRAY_ACCESSOR(double, fz)
+DSDX_ARRAY_ACCESSOR(double, mass)
+DSDX_ARRAY_ACCESSOR(double, charge)
+DSDX_ARRAY_ACCESSOR(int, type)
+DSDX_ARRAY_ACCESSOR(int, global_id)
+#undef DSDX_ARRAY_ACCESSOR
+
+void ParticleInfo::resize(std::size_t count)
 {
-    owned_count_ = owned;
-    ghost_count_ = ghosts;
+    for (std::vector<double>* v : {&x, &y, &z, &vx, &vy, &vz, &fx, &fy, &fz, &mass, &charge})
+        v->resize(count);
+    type.resize(count);
+    global_id.resize(count);
+    num_particles = count;
+}
+
+void ParticleInfo::set_counts(std::size_t owned, std::size_t ghosts) noexcept
+{
+    if (!state_) state_.reset(new (std::nothrow) State());
+    if (state_) {
+        state_->owned = owned;
+        state_->ghosts = ghosts;
+    }
 }
 
 void ParticleInfo::reset_counts(std::size_t owned, std::size_t ghosts)
 {
-    const std::size_t total = owned + ghosts;
-    if (read_view(total).count < total)
-        throw std::invalid_argument("dsDecompX: particle storage is smaller than the supplied counts");
-    commit_counts(owned, ghosts);
-}
-
-void ParticleInfo::resize(std::size_t count)
-{
-    x.resize(count); y.resize(count); z.resize(count);
-    vx.resize(count); vy.resize(count); vz.resize(count);
-    fx.resize(count); fy.resize(count); fz.resize(count);
-    mass.resize(count); charge.resize(count);
-    type.resize(count); global_id.resize(count);
-    num_particles = count;
-}
-
-ConstParticleView ParticleInfo::host_const_view(std::size_t count) const
-{
-    ConstParticleView v;
-    v.memory_space = MemorySpace::Host;
-    v.count = count;
-    v.x = x.data(); v.y = y.data(); v.z = z.data();
-    v.vx = vx.data(); v.vy = vy.data(); v.vz = vz.data();
-    v.fx = fx.data(); v.fy = fy.data(); v.fz = fz.data();
-    v.mass = mass.data(); v.charge = charge.data();
-    v.type = type.data(); v.global_id = global_id.data();
-    v.fields = fields_.empty() ? nullptr : fields_.data();
-    v.field_count = fields_.size();
-    return v;
-}
-
-ParticleView ParticleInfo::host_view(std::size_t count)
-{
-    ParticleView v;
-    v.memory_space = MemorySpace::Host;
-    v.count = count;
-    v.x = x.data(); v.y = y.data(); v.z = z.data();
-    v.vx = vx.data(); v.vy = vy.data(); v.vz = vz.data();
-    v.fx = fx.data(); v.fy = fy.data(); v.fz = fz.data();
-    v.mass = mass.data(); v.charge = charge.data();
-    v.type = type.data(); v.global_id = global_id.data();
-    attach_fields(v, fields_);
-    return v;
+    if (capacity() < owned + ghosts)
+        throw std::invalid_argument("dsDecompX: " + std::to_string(owned + ghosts) +
+                                    " particles do not fit storage of " + std::to_string(capacity()));
+    State& s = state();
+    s.owned = owned;
+    s.ghosts = ghosts;
 }
 
 void ParticleInfo::resize_owned(std::size_t count)
 {
-    if (external_) {
-        if (!owns_device_memory_)
-            throw std::logic_error("dsDecompX: resize_owned cannot resize borrowed particle memory");
-        resize_storage(count, std::min(count, owned_count_));
-        commit_counts(count, 0);
-        return;
+    State& s = state();
+    switch (s.backing) {
+    case Backing::Borrowed:
+        throw std::logic_error("dsDecompX: resize_owned cannot resize arrays the caller owns");
+    case Backing::Block:
+        ensure_capacity(count, std::min(count, s.owned));
+        break;
+    case Backing::HostVectors:
+        s.require_field_capacity(count);
+        resize(count);
+        break;
     }
-    for (const auto& f : fields_)
-        if (f.capacity < count)
-            throw std::length_error("dsDecompX: custom particle field '" + f.name +
-                                    "' has insufficient capacity");
-    resize(count);
-    commit_counts(count, 0);
+    s.owned = count;
+    s.ghosts = 0;
 }
 
-ConstParticleView ParticleInfo::read_view(std::size_t count) const
+ConstParticleView ParticleInfo::const_view(std::size_t need) const
 {
-    if (external_) {
-        if (storage_.count < count)
-            throw std::invalid_argument(
-                "dsDecompX: external particle memory has insufficient capacity");
-        return as_const_view(storage_);
+    if (capacity() < need)
+        throw std::invalid_argument("dsDecompX: particle storage of " + std::to_string(capacity()) +
+                                    " slots is smaller than " + std::to_string(need));
+    if (state_ && state_->backing != Backing::HostVectors) return as_const_view(state_->view());
+    ConstParticleView v;
+    v.memory_space = MemorySpace::Host;
+    v.count = need;
+    v.x = x.data(); v.y = y.data(); v.z = z.data();
+    v.vx = vx.data(); v.vy = vy.data(); v.vz = vz.data();
+    v.fx = fx.data(); v.fy = fy.data(); v.fz = fz.data();
+    v.mass = mass.data(); v.charge = charge.data();
+    v.type = type.data(); v.global_id = global_id.data();
+    if (state_ && !state_->fields.empty()) {
+        v.fields = state_->fields.data();
+        v.field_count = state_->fields.size();
     }
-    return host_const_view(count);
+    return v;
 }
 
-ParticleInfo& ParticleInfo::add_field_view(ParticleFieldView field)
+StorageResizeResult ParticleInfo::ensure_capacity(std::size_t required, std::size_t preserve)
 {
-    if (field.name.empty() || !field.data || field.components == 0 || field.capacity < size())
-        throw std::invalid_argument("dsDecompX: invalid or undersized custom particle field");
-    for (const auto& f : fields_)
-        if (f.name == field.name)
-            throw std::invalid_argument("dsDecompX: duplicate custom particle field '" +
-                                        field.name + "'");
-    fields_.push_back(std::move(field));
-    attach_fields(storage_, fields_);
-    return *this;
-}
-
-StorageResizeResult ParticleInfo::resize_storage(std::size_t required, std::size_t preserve)
-{
-    for (const auto& f : fields_)
-        if (f.capacity < required)
-            throw std::length_error("dsDecompX: custom particle field '" + f.name +
-                                    "' has insufficient capacity");
-    if (!external_) {
+    State& s = state();
+    s.require_field_capacity(required);
+    switch (s.backing) {
+    case Backing::HostVectors: {
         const double* before = x.data();
         resize(required);
-        return {host_view(required), before != x.data()};
+        ParticleView v;
+        v.memory_space = MemorySpace::Host;
+        v.count = required;
+        v.x = x.data(); v.y = y.data(); v.z = z.data();
+        v.vx = vx.data(); v.vy = vy.data(); v.vz = vz.data();
+        v.fx = fx.data(); v.fy = fy.data(); v.fz = fz.data();
+        v.mass = mass.data(); v.charge = charge.data();
+        v.type = type.data(); v.global_id = global_id.data();
+        v.fields = s.fields.empty() ? nullptr : s.fields.data();
+        v.field_count = s.fields.size();
+        return {v, before != x.data()};
     }
-    if (required <= storage_.count) return {storage_, false};
-    if (owns_device_memory_) {
-        const std::size_t grown =
-            std::max(required, std::max<std::size_t>(64, storage_.count + storage_.count / 2));
-        // Stream-ordered work may still read the old arrays.
-        hip_require(hipDeviceSynchronize(), "synchronize before growing ParticleInfo");
-        ParticleView replacement = allocate_device_view(storage_.memory_space, grown);
-        try {
-            copy_device_view(storage_, replacement, std::min(preserve, storage_.count));
-        } catch (...) {
-            free_device_view(replacement);
-            throw;
-        }
-        free_device_view(storage_);
-        storage_ = replacement;
-        attach_fields(storage_, fields_);
-        return {storage_, true};
+    case Backing::Block:
+        if (required <= s.arrays.count) return {s.view(), false};
+        s.grow_block(required, preserve);
+        return {s.view(), true};
+    case Backing::Borrowed:
+        break;
     }
-    if (!provider_) throw std::length_error("dsDecompX: fixed external particle storage is too small");
-    StorageResizeResult result = provider_->resize_particle_info(required, preserve);
-    storage_ = result.view;
-    attach_fields(storage_, fields_);
-    result.view = storage_;
-    return result;
+    if (required <= s.arrays.count) return {s.view(), false};
+    if (!s.provider)
+        throw std::length_error("dsDecompX: borrowed particle arrays of " + std::to_string(s.arrays.count) +
+                                " slots cannot hold " + std::to_string(required) + " particles");
+    const StorageResizeResult grown = s.provider->resize_particle_info(required, preserve);
+    s.arrays = grown.view;   // the registered fields stay the ones registered here
+    s.arrays.fields = nullptr;
+    s.arrays.field_count = 0;
+    return {s.view(), grown.reallocated};
+}
+
+ParticleInfo& ParticleInfo::register_field(ParticleFieldView field)
+{
+    State& s = state();
+    if (field.name.empty() || !field.data || field.components == 0 || field.capacity < size())
+        throw std::invalid_argument("dsDecompX: registered field '" + field.name +
+                                    "' needs a name, data, components and room for every particle");
+    for (const ParticleFieldView& f : s.fields)
+        if (f.name == field.name)
+            throw std::invalid_argument("dsDecompX: a field named '" + field.name + "' is already registered");
+    s.fields.push_back(std::move(field));
+    return *this;
 }
 
 void ParticleInfo::reset_view(ParticleView storage)
 {
     if (storage.count < size())
-        throw std::invalid_argument(
-            "dsDecompX: replacement particle view is smaller than current particles");
-    release_device_memory();
-    external_ = true;
-    owns_device_memory_ = false;
-    provider_ = nullptr;
-    storage_ = storage;
-    fields_.clear();
-    if (storage.fields && storage.field_count)
-        fields_.assign(storage.fields, storage.fields + storage.field_count);
-    attach_fields(storage_, fields_);
+        throw std::invalid_argument("dsDecompX: the new particle view has " + std::to_string(storage.count) +
+                                    " slots for " + std::to_string(size()) + " particles");
+    State& s = state();
+    s.drop_block();
+    s.backing = Backing::Borrowed;
+    s.provider = nullptr;
+    s.adopt(storage);
 }
 
 }  // namespace dsdecompx

@@ -61,7 +61,10 @@ struct RebalanceResult {
 
 // Particle storage of every Decomp operation.  allocate(CudaDevice, n) gives
 // library-owned device arrays that grow as ghosts arrive; external(...) plus
-// the setters borrow caller arrays; a Provider grows borrowed storage.
+// the setters borrow caller arrays; a Provider grows borrowed storage.  The
+// public members and functions are the reference's (include/dsdecompx.hpp:
+// 70-168); where the arrays live is the library's own business
+// (dsdecompx_amd/csrc/host/particle_info.cpp).
 class ParticleInfo {
 public:
     class Provider {
@@ -72,8 +75,8 @@ public:
                                                          std::size_t preserve) = 0;
     };
 
-    ParticleInfo() = default;
-    explicit ParticleInfo(std::size_t owned) { resize_owned(owned); }
+    ParticleInfo();
+    explicit ParticleInfo(std::size_t owned);
     ~ParticleInfo();
     ParticleInfo(ParticleInfo&& other) noexcept;
     ParticleInfo& operator=(ParticleInfo&& other) noexcept;
@@ -95,9 +98,8 @@ public:
     ParticleInfo& add_field(std::string name, T* data, std::size_t capacity = 0,
                             std::size_t components = 1)
     {
-        ParticleFieldView field{std::move(name), field_type_v<T>, components,
-                                capacity ? capacity : this->capacity(), data};
-        return add_field_view(std::move(field));
+        return register_field(ParticleFieldView{std::move(name), field_type_v<T>, components,
+                                                capacity ? capacity : this->capacity(), data});
     }
 
     std::vector<double> x, y, z;
@@ -109,12 +111,12 @@ public:
 
     void resize(std::size_t count);
 
-    std::size_t owned_count() const noexcept { return owned_count_; }
-    std::size_t ghost_count() const noexcept { return ghost_count_; }
-    std::size_t size() const noexcept { return owned_count_ + ghost_count_; }
+    std::size_t owned_count() const noexcept;
+    std::size_t ghost_count() const noexcept;
+    std::size_t size() const noexcept;
     std::size_t capacity() const noexcept;
     MemorySpace memory_space() const noexcept;
-    bool uses_external_memory() const noexcept { return external_ && !owns_device_memory_; }
+    bool uses_external_memory() const noexcept;
 
     double* x_data() noexcept;
     double* y_data() noexcept;
@@ -136,21 +138,18 @@ public:
 
 private:
     friend class Decomp;
-    ConstParticleView read_view(std::size_t count) const;
-    ConstParticleView host_const_view(std::size_t count) const;
-    ParticleView host_view(std::size_t count);
-    StorageResizeResult resize_storage(std::size_t required, std::size_t preserve);
-    void commit_counts(std::size_t owned, std::size_t ghosts) noexcept;
-    void release_device_memory() noexcept;
-    ParticleInfo& add_field_view(ParticleFieldView field);
+    // The arrays with room for at least `need` particles, fields attached.
+    ConstParticleView const_view(std::size_t need) const;
+    // Room for `required` particles, keeping the first `preserve` of every
+    // array when the storage moves.
+    StorageResizeResult ensure_capacity(std::size_t required, std::size_t preserve);
+    void set_counts(std::size_t owned, std::size_t ghosts) noexcept;
+    ParticleInfo& register_field(ParticleFieldView field);
 
-    bool external_ = false;
-    bool owns_device_memory_ = false;
-    std::size_t owned_count_ = 0;
-    std::size_t ghost_count_ = 0;
-    std::vector<ParticleFieldView> fields_;
-    ParticleView storage_;
-    Provider* provider_ = nullptr;
+    // Null after a move: the object then reads as empty host storage.
+    struct State;
+    std::unique_ptr<State> state_;
+    State& state();
 };
 
 // The user entry point.  Every operation runs on the caller's HIP stream; a
