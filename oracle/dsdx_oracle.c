@@ -102,13 +102,8 @@ int dsdxo_halo_build(const dsdxo_box* box, const dsdxo_neighbors* nb,
 {
     for (int n = 0; n < DSDXO_MAX_NEIGHBORS; ++n) plan->send_count[n] = 0;
     const double width = box->width;
-    /* local domain: the topology's cut planes (src/topology.cpp:66-72) */
     double lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-        const double extent = box->upper[a] - box->lower[a];
-        lo[a] = box->local_given ? box->local_lower[a] : box->lower[a] + extent * 0.0 / 1.0;
-        hi[a] = box->local_given ? box->local_upper[a] : box->lower[a] + extent * 1.0 / 1.0;
-    }
+    dsdxo_local_domain(box, lo, hi);
     for (int64_t i = 0; i < owned; ++i) {
         const double px = x[i], py = y[i], pz = z[i];
         const int in_xlo = px < lo[0] + width;
@@ -378,6 +373,33 @@ void dsdxo_migrate_wrap(const dsdxo_box* box, int64_t owned, double* x,
             const double m = q * extent;
             value = value - m;
             v[i] = value + lower;
+        }
+    }
+}
+
+/* src/topology.cpp:66-72: the cut planes of a fresh 1x1x1 topology,
+ * global_lower + extent * i / dims for i = 0, 1, unless given. */
+void dsdxo_local_domain(const dsdxo_box* box, double lower[3], double upper[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        const double extent = box->upper[a] - box->lower[a];
+        lower[a] = box->local_given ? box->local_lower[a] : box->lower[a] + extent * 0.0 / 1.0;
+        upper[a] = box->local_given ? box->local_upper[a] : box->lower[a] + extent * 1.0 / 1.0;
+    }
+}
+
+/* src/decomp.cpp:117-124 */
+void dsdxo_rescale_cuts(const double old_lower[3], const double old_upper[3],
+                        const double new_lower[3], const double new_upper[3],
+                        double cut_lo[3], double cut_hi[3])
+{
+    for (int a = 0; a < 3; ++a) {
+        const double old_extent = old_upper[a] - old_lower[a];
+        const double new_extent = new_upper[a] - new_lower[a];
+        double* cuts[2] = {&cut_lo[a], &cut_hi[a]};
+        for (int c = 0; c < 2; ++c) {
+            const double fraction = (*cuts[c] - old_lower[a]) / old_extent;
+            *cuts[c] = new_lower[a] + fraction * new_extent;
         }
     }
 }

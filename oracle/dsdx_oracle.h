@@ -134,6 +134,17 @@ int dsdxo_forward_exchange(const dsdxo_neighbors* nb, const dsdxo_plan* plan,
 void dsdxo_migrate_wrap(const dsdxo_box* box, int64_t owned, double* x,
                         double* y, double* z);
 
+/* The local domain the slab tests use (src/topology.cpp:66-72): the given
+ * one, or the fresh 1x1x1 cuts lower + extent * i / 1. */
+void dsdxo_local_domain(const dsdxo_box* box, double lower[3], double upper[3]);
+
+/* Decomposition::rescale_domain (src/decomp.cpp:117-124): every cut keeps its
+ * fraction of the extent, fraction = (cut - old_lower) / old_extent,
+ * cut = new_lower + fraction * new_extent, in place on cut_lo / cut_hi. */
+void dsdxo_rescale_cuts(const double old_lower[3], const double old_upper[3],
+                        const double new_lower[3], const double new_upper[3],
+                        double cut_lo[3], double cut_hi[3]);
+
 /* ---- Wire format (dsdx_oracle_wire.c) ------------------------------------
  * ParticleRecord (include/field.hpp:150-157), the particle and exchange
  * record layouts and the device path's pack / unpack / reduce steps on the

@@ -116,6 +116,8 @@ def lib():
         L.dsdxo_forward_exchange.argtypes = [C.POINTER(_Neighbors), C.POINTER(_Plan),
                                              C.c_int64, C.POINTER(_Field), C.c_int32]
         L.dsdxo_migrate_wrap.argtypes = [C.POINTER(_Box), C.c_int64, _DP, _DP, _DP]
+        L.dsdxo_local_domain.argtypes = [C.POINTER(_Box), _DP, _DP]
+        L.dsdxo_rescale_cuts.argtypes = [_DP, _DP, _DP, _DP, _DP, _DP]
         L.dsdxo_grid_build.argtypes = [C.POINTER(_Box), C.POINTER(C.c_int32), C.c_int32,
                                        C.POINTER(_Grid)]
         L.dsdxo_grid_build.restype = C.c_int
@@ -204,6 +206,22 @@ class Box:
                 b.local_upper[a] = float(self.local[1][a])
             b.local_given = 1
         return b
+
+
+def local_domain(box: "Box"):
+    """The local domain the slab tests of `box` use (dsdxo_local_domain)."""
+    lo, hi = np.zeros(3), np.zeros(3)
+    lib().dsdxo_local_domain(C.byref(box.c()), _dp(lo), _dp(hi))
+    return tuple(lo), tuple(hi)
+
+
+def rescale_cuts(old: "Box", new: "Box", cuts):
+    """rescale_domain's in-place cut update (dsdxo_rescale_cuts): `cuts` =
+    (lower cuts, upper cuts) of the old domain -> those of the new one."""
+    arrs = [np.array(v, np.float64) for v in (old.lower, old.upper, new.lower, new.upper)]
+    lo, hi = np.array(cuts[0], np.float64), np.array(cuts[1], np.float64)
+    lib().dsdxo_rescale_cuts(*[_dp(a) for a in arrs], _dp(lo), _dp(hi))
+    return tuple(lo), tuple(hi)
 
 
 @dataclass

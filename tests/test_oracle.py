@@ -126,6 +126,41 @@ def test_upper_cut_is_lower_plus_extent(oracle_lib):
     assert po.Oracle(given).exchange_halo(_make(given, [(52.5, mid, mid)]), 1) == 0
 
 
+def test_rescale_cuts_keep_their_fraction(oracle_lib):
+    """rescale_domain moves every cut to new_lower + ((cut - old_lower) /
+    old_extent) * new_extent (src/decomp.cpp:117-124), one rounding per
+    operation.  Expected values from exact rationals; closed form: doubling a
+    box at lower 0 doubles every cut.  The inputs separate this order from
+    scaling by the extents' ratio, new_lower + (cut - old_lower) *
+    (new_extent / old_extent)."""
+    rng = np.random.default_rng(41)
+    alt_differs = False
+    for trial in range(200):
+        lo_o = tuple(np.round(rng.uniform(-20, 20, 3), 2))
+        up_o = tuple(np.round(np.array(lo_o) + rng.uniform(5, 90, 3), 2))
+        lo_n = tuple(np.round(np.array(lo_o) + rng.uniform(-3, 3, 3), 2))
+        up_n = tuple(np.round(np.array(lo_n) + rng.uniform(5, 90, 3), 2))
+        old, new = po.Box(lower=lo_o, upper=up_o), po.Box(lower=lo_n, upper=up_n)
+        cuts = po.local_domain(old)
+        got = po.rescale_cuts(old, new, cuts)
+        for a in range(3):
+            oe = _rn(Fraction(up_o[a]) - Fraction(lo_o[a]))
+            ne = _rn(Fraction(up_n[a]) - Fraction(lo_n[a]))
+            assert cuts[0][a] == _rn(Fraction(lo_o[a]) + Fraction(_rn(Fraction(oe) * 0) / 1))
+            assert cuts[1][a] == _rn(Fraction(lo_o[a]) + Fraction(_rn(Fraction(oe) * 1) / 1))
+            for c in range(2):
+                d = _rn(Fraction(cuts[c][a]) - Fraction(lo_o[a]))
+                frac = _rn(Fraction(d) / Fraction(oe))
+                want = _rn(Fraction(lo_n[a]) + Fraction(_rn(Fraction(frac) * Fraction(ne))))
+                assert got[c][a] == want, (trial, a, c)
+                ratio = _rn(Fraction(ne) / Fraction(oe))
+                alt = _rn(Fraction(lo_n[a]) + Fraction(_rn(Fraction(d) * Fraction(ratio))))
+                alt_differs |= alt != want
+    assert alt_differs
+    o2, n2 = po.Box(upper=(7.0, 9.5, 11.25)), po.Box(upper=(14.0, 19.0, 22.5))
+    assert po.rescale_cuts(o2, n2, po.local_domain(o2)) == ((0.0,) * 3, (14.0, 19.0, 22.5))
+
+
 def test_wide_halo_particle_in_both_slabs(oracle_lib):
     """w > L/2: a particle can be in the low and the high slab of an axis and
     then has images on both sides (26 for a particle in all six slabs)."""
