@@ -128,6 +128,75 @@ def test_batch_matches_oracle_and_loop(gpu, count, seed, nmax):
                               f"loop r{r} reverse {k}")
 
 
+def test_batch_512_replicas_full_size(gpu):
+    """The batched workload at its configured size: 512 periodic replicas,
+    particle counts log-uniform in [2048, 262144], density U[0.6, 1.2],
+    w = 2.8, one N(0, 0.1) step of migrate -> exchange_halo -> reverse_halo
+    (examples/communication_test/main.cu:222-275).  Every replica is checked
+    bit for bit against the oracle run on that replica alone, and the plain
+    per-box loop against the batch."""
+    torch = gpu
+    from dsdecompx_amd.batch import LoopedReplicas, ReplicaBatch
+    specs = _specs(512, 2024, nmin=2048, nmax=262144)
+    say("batch 512", sum(s.owned for s in specs), "particles")
+    b = ReplicaBatch(specs)
+    _fill(torch, b, 7)
+    start = {k: host(getattr(b, k)).copy() for k in F64 + I32}
+    rng = np.random.default_rng(8)
+    disp = rng.normal(0, 0.1, (3, b.total))
+    forces = rng.uniform(-1, 1, (3, b.total))
+    for a, k in enumerate(("x", "y", "z")):
+        getattr(b, k).add_(torch.from_numpy(disp[a]).cuda())
+    b.migrate()
+    b.exchange_halo()
+    ghosts, owners, status = b.counts()
+    assert not any(status)
+    after_halo = _snapshot(b)
+    own_all = host(b.ghost_owners())
+    for a, k in enumerate(("fx", "fy", "fz")):
+        getattr(b, k).copy_(torch.from_numpy(forces[a]).cuda())
+    b.reverse_halo()
+    after_rev = {k: host(getattr(b, k)).copy() for k in ("fx", "fy", "fz")}
+    say("batch 512 oracle", sum(ghosts), "ghosts")
+    for r, s in enumerate(specs):
+        n, g, off = s.owned, ghosts[r], b.offsets[r]
+        o = po.Oracle(po.Box(upper=s.box.upper, width=s.box.width))
+        p = po.Particles(b.capacities[r])
+        for k in F64 + I32:
+            getattr(p, k)[:n] = start[k][off:off + n]
+        for a, k in enumerate(("x", "y", "z")):
+            getattr(p, k)[:n] += disp[a][off:off + n]
+        o.migrate(p, n)
+        assert o.exchange_halo(p, n) == g, f"replica {r} ghost count"
+        for k in F64 + I32:
+            assert_bits_equal(after_halo[k][off:off + n + g], getattr(p, k)[:n + g], f"r{r} halo {k}")
+        assert_bits_equal(own_all[off:off + g], o.ghost_owners(), f"r{r} ghost owners")
+        for a, k in enumerate(("fx", "fy", "fz")):
+            getattr(p, k)[:n + g] = forces[a][off:off + n + g]
+        o.reverse_halo(p, n)
+        for k in ("fx", "fy", "fz"):
+            assert_bits_equal(after_rev[k][off:off + n], getattr(p, k)[:n], f"r{r} reverse {k}")
+    say("batch 512 loop")
+    for k in F64 + I32:
+        getattr(b, k).copy_(torch.from_numpy(start[k]).cuda())
+    loop = LoopedReplicas(b)
+    for a, k in enumerate(("x", "y", "z")):
+        getattr(b, k).add_(torch.from_numpy(disp[a]).cuda())
+    loop.migrate()
+    loop.exchange_halo()
+    assert loop.ghosts == ghosts
+    for a, k in enumerate(("fx", "fy", "fz")):
+        getattr(b, k).copy_(torch.from_numpy(forces[a]).cuda())
+    loop.reverse_halo()
+    got = _snapshot(b)
+    for r, s in enumerate(specs):
+        off, n, g = b.offsets[r], s.owned, ghosts[r]
+        for k in ("x", "y", "z", "vx", "mass", "type", "global_id"):
+            assert_bits_equal(got[k][off:off + n + g], after_halo[k][off:off + n + g], f"loop r{r} {k}")
+        for k in ("fx", "fy", "fz"):
+            assert_bits_equal(got[k][off:off + n], after_rev[k][off:off + n], f"loop r{r} reverse {k}")
+
+
 @pytest.mark.parametrize("shape,fused", [((8, 4, 1024), False), ((4, 2, 256), True),
                                          ((16, 8, 512), False), ((8, 4, 1024), True)])
 def test_graph_replays_match_eager(gpu, shape, fused):
