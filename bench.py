@@ -404,120 +404,46 @@ def measure_grid(args, n, L, w, sigma, dims=(2, 2, 2)):
 
 def measure_batched(args):
     """512 replica boxes, particle counts log-uniform in [2048, 262144],
-    density U[0.6, 1.2], w = 2.8, displacement N(0, 0.1).  The same steps
-    (application displacement + migrate + exchange_halo + forces +
-    reverse_halo) run as the plain per-box loop and as the batched form
-    (eager, and captured in one hipGraph); replica-steps/s of each, and the
-    final states of loop and graph compared bit for bit."""
-    import numpy as np
-    import torch
-    from dsdecompx_amd import _abi
-    from dsdecompx_amd.batch import LoopedReplicas, ReplicaBatch, ReplicaSpec, capture_step
-    from dsdecompx_amd.halo import BoxConfig
-
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
-    torch.cuda.set_stream(torch.cuda.Stream(dev))
-    _abi.timing_enable(False)
-    rng = np.random.default_rng(args.seed + 5)
+    density U[0.6, 1.2], w = 2.8, displacement N(0, 0.1), through the C ABI
+    from C++ (dsdecompx_amd/bin/batch_bench): the plain per-box loop, the
+    batched form eager, and the batched step captured once with
+    hipStreamBeginCapture and replayed with hipGraphLaunch, each unfused
+    (migrate then exchange_halo) and fused (migrate_and_exchange_halo), with
+    the same application work (displacement, forces) in every form.  A check
+    run of the driver first confirms that every form ends bit-identical to the
+    unfused loop."""
+    import subprocess
+    exe = os.path.join(ROOT, "dsdecompx_amd", "bin", "batch_bench")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} is missing: run `python -c 'import __graft_entry__ as g; g.build()'`")
     R = args.replicas
-    n = np.exp(rng.uniform(np.log(2048), np.log(262144), R)).astype(int)
-    rho = rng.uniform(0.6, 1.2, R)
-    specs = []
-    for k in range(R):
-        L = float((n[k] / rho[k]) ** (1.0 / 3.0))
-        specs.append(ReplicaSpec(int(n[k]), BoxConfig(upper=(L, L, L), width=2.8)))
-    b = ReplicaBatch(specs, dev)
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(args.seed + 6)
-    lengths = torch.zeros(b.total, dtype=torch.float64, device=dev)
-    owned_mask = torch.zeros(b.total, dtype=torch.bool, device=dev)
-    for r, sp in enumerate(specs):
-        lengths[b.owned_slice(r)] = sp.box.upper[0]
-        owned_mask[b.owned_slice(r)] = True
-    for k in ("x", "y", "z"):
-        getattr(b, k).uniform_(0.0, 1.0, generator=gen).mul_(lengths)
-    b.mass.fill_(1.0)
-    start = {k: getattr(b, k).clone() for k in ("x", "y", "z", "vx", "vy", "vz", "fx", "fy",
-                                               "fz", "mass", "charge", "type", "global_id")}
-    pool = [torch.randn((3, b.total), dtype=torch.float64, device=dev, generator=gen)
-            .mul_(0.1).mul_(owned_mask) for _ in range(4)]
-    frc = torch.rand((3, b.total), dtype=torch.float64, device=dev, generator=gen).mul_(2).sub_(1)
-    step_disp = torch.zeros((3, b.total), dtype=torch.float64, device=dev)
-
-    def restore():
-        for k, v in start.items():
-            getattr(b, k).copy_(v)
-
-    def app_halo():
-        b.x.add_(step_disp[0]); b.y.add_(step_disp[1]); b.z.add_(step_disp[2])
-
-    def app_rev():
-        b.fx.copy_(frc[0]); b.fy.copy_(frc[1]); b.fz.copy_(frc[2])
-
-    steps = args.batch_steps
-
-    def timed(run_step, count):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(count):
-            step_disp.copy_(pool[k % 4])
-            run_step()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
-    loop = LoopedReplicas(b)
-
-    def loop_step():
-        app_halo(); loop.migrate(); loop.exchange_halo(); app_rev(); loop.reverse_halo()
-
-    def eager_step():
-        app_halo(); b.migrate(); b.exchange_halo(); app_rev(); b.reverse_halo()
-
-    restore()
-    timed(loop_step, 1)
-    restore()
-    t_loop = timed(loop_step, steps)
-    loop_state = {k: getattr(b, k).clone() for k in start}
-    tuned = None
-    if args.batch_tune:
-        from dsdecompx_amd import autotune
-        restore()
-        tuned = autotune.tune_batch(b)
-    restore()
-    timed(eager_step, 1)
-    restore()
-    t_eager = timed(eager_step, steps)
-    restore()
-    graph = capture_step(b, app_halo, app_rev, warmup=1, fused=True)
-    restore()
-    t_graph = timed(graph.replay, steps)
-    equal = all(torch.equal(getattr(b, k).view(torch.int64) if getattr(b, k).dtype == torch.float64
-                            else getattr(b, k),
-                            loop_state[k].view(torch.int64) if loop_state[k].dtype == torch.float64
-                            else loop_state[k]) for k in start)
-    ghosts, owners, status = b.counts()
-    total = int(n.sum())
+    chk = subprocess.run([exe, str(R), "3", "0", str(args.seed + 5), "check"], stdout=subprocess.PIPE,
+                         stderr=subprocess.PIPE, text=True)
+    if chk.returncode not in (0, 1):
+        raise SystemExit(f"batch_bench check failed ({chk.returncode}): {chk.stderr[-2000:]}")
+    check = json.loads(chk.stdout.strip().splitlines()[-1])
+    r = subprocess.run([exe, str(R), str(args.batch_steps), "2", str(args.seed + 5), "bench"],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        raise SystemExit(f"batch_bench failed ({r.returncode}): {r.stderr[-2000:]}")
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    ms, rs = res["ms_per_step"], res["replica_steps_per_s"]
     return {
-        "workload": f"{R} periodic replica boxes, {total} particles (log-uniform "
+        "workload": f"{R} periodic replica boxes, {res['particles']} particles (log-uniform "
                     f"2048..262144 per box, density U[0.6,1.2]), halo width 2.8, "
                     f"displacement N(0,0.1)",
-        "replicas": R, "particles": total, "steps": steps,
-        "ghosts": int(sum(ghosts)), "overflowed_replicas": int(sum(1 for s_ in status if s_)),
-        "looped_replica_steps_per_s": R * steps / t_loop,
-        "batched_eager_replica_steps_per_s": R * steps / t_eager,
-        "batched_graph_replica_steps_per_s": R * steps / t_graph,
-        "graph_vs_looped": t_loop / t_graph,
-        "eager_vs_looped": t_loop / t_eager,
-        "graph_ms_per_step": 1e3 * t_graph / steps,
-        "looped_ms_per_step": 1e3 * t_loop / steps,
-        "bit_exact_graph_vs_looped": bool(equal),
-        "shape": list(b.shape()),
-        "autotune": tuned,
-        "graph_form": "one hipGraph per step: displacement, fused migrate+classify, scan, "
-                      "scatter, force fill, reverse_halo",
-        "timing": "wall clock over the steps, application displacement and force fill included "
-                  "in every form",
+        "replicas": R, "particles": res["particles"], "steps": res["steps"],
+        "ms_per_step": ms, "replica_steps_per_s": rs,
+        "graph_vs_looped": ms["loop"] / ms["graph"],
+        "graph_vs_looped_fused": ms["loop_fused"] / ms["graph_fused"],
+        "eager_vs_looped": ms["loop"] / ms["batch"],
+        "graph_vs_eager": ms["batch"] / ms["graph"],
+        "batch_shape": res.get("batch_shape"),
+        "bit_exact_all_forms": check["bit_exact"],
+        "check": check,
+        "driver": "dsdecompx_amd/bin/batch_bench (C++, C ABI, hipStreamBeginCapture / hipGraphLaunch)",
+        "timing": "wall clock over the steps, application displacement and force fill included in "
+                  "every form; like for like: unfused against unfused, fused against fused",
     }
 
 
@@ -979,8 +905,6 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="autotune the launch shape of the single box before measuring "
                          "(default: the library's default shape)")
-    ap.add_argument("--no-batch-tune", dest="batch_tune", action="store_false",
-                    help="keep the default launch shape for the batched replicas")
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-traffic", dest="traffic", action="store_false",
