@@ -23,6 +23,11 @@ def test_oracle_reproduces_fixture(oracle_lib, name):
         assert a.tobytes() == b.tobytes(), k
 
 
+# Written by the reference itself (tests/golden/make_ref_fixtures.py) and
+# checked by tests/test_oracle_grid.py and tests/test_gpu_grid.py.
+REFERENCE_FIXTURES = ["ref_grid_2x2x2"]
+
+
 def test_fixture_set_is_complete():
     files = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "golden", "*.npz")))
-    assert files == sorted(CASES)
+    assert files == sorted(list(CASES) + REFERENCE_FIXTURES)

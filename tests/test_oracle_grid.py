@@ -307,3 +307,79 @@ def test_reverse_conservation_and_forward_adjoint():
     g.reverse_exchange(owned, ry)
     rhs = sum(float(xs[r][:owned[r]] @ ry[r][0].data[:owned[r]]) for r in range(g.ranks))
     assert abs(lhs - rhs) <= 1e-10 * (1 + abs(lhs))
+
+
+def test_reference_run_on_eight_ranks_bit_exact():
+    """The fixture tests/golden/ref_grid_2x2x2.npz holds what the reference
+    itself (its unmodified Host library on 8 MPI ranks, a 2x2x2 periodic
+    grid; tests/golden/make_ref_fixtures.py) held on every rank after each
+    operation.  The box does not start at 0, and its extents are not powers of
+    two.  3000 particles start on the wrong ranks, and 10 % lie past each face.
+    The oracle, on the same inputs, must reproduce it bit for bit over two
+    steps:
+    - after migrate: owned ids in order, positions (wrapped), velocities and
+      the registered field;
+    - after exchange_halo: ghost ids in order, positions, velocities, masses,
+      types and the field;
+    - after reverse_halo: the owned forces;
+    - after reverse_exchange(Sum) of the field: the owned values."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_grid_2x2x2.npz")
+    with np.load(path) as z:
+        ref = {k: z[k] for k in z.files}
+    box = po.Box(lower=tuple(ref["box_lower"]), upper=tuple(ref["box_upper"]), width=float(ref["box_width"][0]))
+    g = po.Grid(box, tuple(int(v) for v in ref["dims"]))
+    cap = 4 * 3000
+    parts = [po.Particles(cap) for _ in range(g.ranks)]
+    aux = [np.zeros((cap, 3)) for _ in range(g.ranks)]
+    owned = []
+    for r, p in enumerate(parts):
+        n = len(ref[f"r{r}_s0_start_global_id"])
+        for k in ("global_id", "type", "x", "y", "z", "vx", "vy", "vz", "mass", "charge"):
+            getattr(p, k)[:n] = ref[f"r{r}_s0_start_{k}"]
+        aux[r][:n] = ref[f"r{r}_s0_start_aux"].reshape(n, 3)
+        owned.append(n)
+
+    def same(a, b, what):
+        a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+        assert a.shape == b.shape, what
+        if a.dtype == np.float64:
+            a, b = a.view(np.uint64), b.view(np.uint64)
+        assert np.array_equal(a, b), what
+
+    for s in range(2):
+        fields = [[po.Field(aux[r])] for r in range(g.ranks)]
+        g.migrate(parts, owned, fields)
+        for r, p in enumerate(parts):
+            n = owned[r]
+            pre = f"r{r}_s{s}_"
+            same(p.global_id[:n], ref[pre + "migrate_global_id"], f"step {s} rank {r} owned ids")
+            for k in ("x", "y", "z", "vx"):
+                same(getattr(p, k)[:n], ref[pre + "migrate_" + k], f"step {s} rank {r} migrate {k}")
+            same(aux[r][:n].ravel(), ref[pre + "migrate_aux"], f"step {s} rank {r} migrate aux")
+            for k in ("x", "y", "z"):
+                getattr(p, k)[:n] = ref[pre + "ties_" + k]
+        ghosts = g.exchange_halo(parts, owned, fields)
+        for r, p in enumerate(parts):
+            n, gh = owned[r], ghosts[r]
+            pre = f"r{r}_s{s}_"
+            same(p.global_id[n:n + gh], ref[pre + "halo_global_id"], f"step {s} rank {r} ghost ids")
+            for k in ("x", "y", "z", "vz", "mass", "type"):
+                same(getattr(p, k)[n:n + gh], ref[pre + "halo_" + k], f"step {s} rank {r} ghost {k}")
+            same(aux[r][n:n + gh].ravel(), ref[pre + "halo_aux"], f"step {s} rank {r} ghost aux")
+            for k in ("fx", "fy", "fz"):
+                getattr(p, k)[:n + gh] = ref[pre + "forces_" + k]
+        g.reverse_halo(parts, owned)
+        for r, p in enumerate(parts):
+            n, gh = owned[r], ghosts[r]
+            pre = f"r{r}_s{s}_"
+            for k in ("fx", "fy", "fz"):
+                same(getattr(p, k)[:n], ref[pre + "reverse_" + k], f"step {s} rank {r} reverse {k}")
+            aux[r][n:n + gh] = ref[pre + "aux_ghosts"].reshape(gh, 3)
+        g.reverse_exchange(owned, [[po.Field(aux[r])] for r in range(g.ranks)])
+        for r, p in enumerate(parts):
+            n = owned[r]
+            pre = f"r{r}_s{s}_"
+            same(aux[r][:n].ravel(), ref[pre + "aux_reduced"], f"step {s} rank {r} reduced aux")
+            for k in ("x", "y", "z"):
+                getattr(p, k)[:n] = ref[pre + "displaced_" + k]
