@@ -447,10 +447,12 @@ def measure_batched(args):
     }
 
 
-# Library kernel name -> device kernel (as rocprofv3 names it).
-DEVICE_KERNELS = {"migrate_wrap": "migrate_wrap_kernel", "halo_classify": "classify_kernel",
-                  "halo_scan": "scan_kernel", "halo_scatter": "route_kernel",
-                  "halo_copy": "copy_kernel", "reverse_halo": "reverse_forces_kernel"}
+# Library kernel name -> device kernels (as rocprofv3 names them; Decomp::migrate
+# launches wrap_box_kernel, the C ABI's per-array form migrate_wrap_kernel).
+DEVICE_KERNELS = {"migrate_wrap": ("wrap_box_kernel", "migrate_wrap_kernel"),
+                  "halo_classify": ("classify_kernel",), "halo_scan": ("scan_kernel",),
+                  "halo_scatter": ("route_kernel",), "halo_copy": ("copy_kernel",),
+                  "reverse_halo": ("reverse_forces_kernel",)}
 CALIBRATION_BYTES = 1 << 30
 
 
@@ -502,11 +504,14 @@ def measure_traffic(n, L, w, sigma, periodic, env, seed):
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     per_kernel = {}
-    for lib_name, dev in DEVICE_KERNELS.items():
-        if dev in means["FETCH_SIZE"] and dev in means["WRITE_SIZE"]:
+    for lib_name, devs in DEVICE_KERNELS.items():
+        for dev in devs:
+            if dev not in means["FETCH_SIZE"] or dev not in means["WRITE_SIZE"]:
+                continue
             rd = means["FETCH_SIZE"][dev] * 1024.0 / scale["FETCH_SIZE"]
             wr = means["WRITE_SIZE"][dev] * 1024.0 / scale["WRITE_SIZE"]
             per_kernel[lib_name] = {"read_bytes": rd, "write_bytes": wr, "bytes": rd + wr}
+            break
     return {"kernels": per_kernel,
             "scale": {"FETCH_SIZE": scale["FETCH_SIZE"], "WRITE_SIZE": scale["WRITE_SIZE"]},
             "method": f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, on comm_bench (5 steps after "

@@ -346,6 +346,191 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
     });
 }
 
+// The ghost copy of one large self-loopback box, staged through LDS.  A
+// workgroup takes a unit of kRunTiles owner tiles (kRunOwners consecutive
+// boundary owners).  The unit's ghosts through send slot s are consecutive
+// ghosts of receive block 25 - s (their ranks among the slot's owners are
+// consecutive), so the unit writes one run per slot:
+//   1. one lane per owner gathers the owner's record (velocity, mass,
+//      charge, type, id, and its position from the classify pass) into LDS,
+//      and counts the unit's images per tile and slot; the first owner of
+//      each slot reads where the slot's run starts (its ghost-list entry);
+//   2. the image counts become, for every image of the unit, its index in
+//      the unit's runs laid end to end, and map[index] = its owner;
+//   3. lane k of a round writes ghost k of those runs, so a store
+//      instruction covers whole cache lines of at most a few runs.
+// The per-owner form (copy_kernel) writes each tile's short runs lane by
+// lane: with 13 arrays and 26 receive blocks open at once, the partly
+// written lines leave the L2 before their neighbours arrive.  A unit whose
+// images exceed the map is written by one lane per owner from the same LDS
+// records.  Registered fields are copied per owner in step 1.
+constexpr uint32_t kRunTiles = 8;
+constexpr uint32_t kRunWaves = 8;
+constexpr uint32_t kRunOwners = kRunTiles * kOwnerTile;
+constexpr uint32_t kRunMap = 2 * kRunOwners;
+
+struct RunStage {
+    double px[kRunOwners], py[kRunOwners], pz[kRunOwners];
+    double vx[kRunOwners], vy[kRunOwners], vz[kRunOwners], m[kRunOwners], q[kRunOwners];
+    int32_t ty[kRunOwners], id[kRunOwners];
+    uint16_t map[kRunMap];      // run index -> owner (index in the unit)
+    uint8_t slot_of[kRunMap];   // run index -> send slot
+    uint8_t code[kRunOwners];   // slab code of each owner (0: no owner)
+    uint32_t tile_rank[kRunTiles][kSlots];   // images of slot s in the unit's earlier tiles
+    uint32_t first[kSlots];     // ghost-list entry of the unit's first image through slot s
+    uint32_t run_start[kSlots]; // first run index of slot s
+    uint32_t images;
+    SlotTable table;
+};
+
+template <bool STREAM>
+__global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_particles p, FieldParams fp,
+                                                                      const ReplicaDesc* __restrict__ desc,
+                                                                      Layout L, PlanBuffers B)
+{
+    __shared__ RunStage sh;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = uniform(threadIdx.x / kWave);
+    const ReplicaDesc* d = desc;
+    const uint32_t owners = uniform(B.totals[kStatusWord]) ? 0u : uniform(B.totals[kColOwners]);
+    const uint32_t units = (owners + kRunOwners - 1) / kRunOwners;
+    if (wave == 0) fill_slot_table(sh.table, desc, d);
+    const uint32_t lut_l = d->bp.lut[lane];
+    const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * L.chunk_size;
+    const uint64_t ghost0 = d->offset + d->owned;
+    const uint32_t* glist = B.ghost_list + d->ghost_list_offset;
+    for (uint32_t c = xcd_block(L.xcd_run); c < units; c += gridDim.x) {
+        const uint32_t e0 = (L.descending ? units - 1u - c : c) * kRunOwners;
+        for (uint32_t k = threadIdx.x; k < kRunTiles * kSlots; k += kRunWaves * kWave)
+            (&sh.tile_rank[0][0])[k] = 0u;
+        if (threadIdx.x < kSlots) sh.first[threadIdx.x] = 0xFFFFFFFFu;
+        __syncthreads();
+        // 1. records into LDS; per tile and slot the image count
+        for (uint32_t t = wave; t < kRunTiles; t += kRunWaves) {
+            const uint32_t ol = t * kWave + lane;
+            const uint32_t e = e0 + ol;
+            const bool live = e < owners;
+            const uint64_t u = slot0 + e;
+            uint32_t oi = 0, code = 0, E = 0, start = 0;
+            if (live) {
+                oi = B.owner_index[u];
+                code = B.owner_code[u];
+                E = B.owner_entry[u];
+                start = B.image_start[u];
+            }
+            const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(code), kWave));
+            if (live) {
+                const uint64_t i = d->offset + oi;
+                sh.px[ol] = B.bpos[E];
+                sh.py[ol] = B.bpos[B.bpos_stride + E];
+                sh.pz[ol] = B.bpos[2 * B.bpos_stride + E];
+                sh.vx[ol] = read_once<STREAM>(p.vx + i);
+                sh.vy[ol] = read_once<STREAM>(p.vy + i);
+                sh.vz[ol] = read_once<STREAM>(p.vz + i);
+                sh.m[ol] = read_once<STREAM>(p.mass + i);
+                sh.q[ol] = read_once<STREAM>(p.charge + i);
+                sh.ty[ol] = read_once<STREAM>(p.type + i);
+                sh.id[ol] = read_once<STREAM>(p.global_id + i);
+                if (fp.count) {
+                    uint32_t k = 0;
+                    for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
+                        copy_field_words(fp, i, ghost0 + glist[start + k]);
+                }
+            }
+            sh.code[ol] = static_cast<uint8_t>(code);
+            uint32_t present = uniform(wave_or(slots));
+            uint32_t leads = 0;   // slots whose first image in this tile is this lane's
+            while (present) {
+                const uint32_t s = static_cast<uint32_t>(__builtin_ctz(present));
+                present &= present - 1u;
+                const uint64_t b = __ballot((slots >> s) & 1u);
+                if (lane == static_cast<uint32_t>(__builtin_ctzll(b))) leads |= 1u << s;
+                if (lane == 0) sh.tile_rank[t][s] = static_cast<uint32_t>(__popcll(b));
+            }
+            for (; leads; leads &= leads - 1u) {
+                const uint32_t s = static_cast<uint32_t>(__builtin_ctz(leads));
+                const uint32_t k = static_cast<uint32_t>(__popc(slots & ((1u << s) - 1u)));
+                atomicMin(&sh.first[s], glist[start + k]);
+            }
+        }
+        __syncthreads();
+        // 2a. lane s of wave 0: its tile counts -> exclusive ranks, the slot's run start
+        if (wave == 0) {
+            uint32_t n = 0;
+            if (lane < kSlots)
+                for (uint32_t t = 0; t < kRunTiles; ++t) {
+                    const uint32_t v = sh.tile_rank[t][lane];
+                    sh.tile_rank[t][lane] = n;
+                    n += v;
+                }
+            const uint32_t end = wave_inclusive_sum(n);
+            if (lane < kSlots) sh.run_start[lane] = end - n;
+            if (lane == kSlots - 1) sh.images = end;
+        }
+        __syncthreads();
+        const uint32_t images = sh.images;
+        if (images <= kRunMap) {
+            // 2b. run index of every image -> its owner and slot
+            for (uint32_t t = wave; t < kRunTiles; t += kRunWaves) {
+                const uint32_t ol = t * kWave + lane;
+                const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(sh.code[ol]), kWave));
+                uint32_t present = uniform(wave_or(slots));
+                while (present) {
+                    const uint32_t s = static_cast<uint32_t>(__builtin_ctz(present));
+                    present &= present - 1u;
+                    const uint64_t b = __ballot((slots >> s) & 1u);
+                    if ((slots >> s) & 1u) {
+                        const uint32_t r = sh.run_start[s] + sh.tile_rank[t][s] + lanes_below(b);
+                        sh.map[r] = static_cast<uint16_t>(ol);
+                        sh.slot_of[r] = static_cast<uint8_t>(s);
+                    }
+                }
+            }
+            __syncthreads();
+            // 3. lane k of a round writes run index k
+            for (uint32_t r = wave * kWave + lane; r < ((images + kWave - 1) / kWave) * kWave; r += kRunWaves * kWave) {
+                if (r >= images) continue;
+                const uint32_t s = sh.slot_of[r];
+                const uint32_t ol = sh.map[r];
+                const uint64_t j = ghost0 + sh.first[s] + (r - sh.run_start[s]);
+                p.x[j] = sh.px[ol] + sh.table.shift[s][0];
+                p.y[j] = sh.py[ol] + sh.table.shift[s][1];
+                p.z[j] = sh.pz[ol] + sh.table.shift[s][2];
+                p.vx[j] = sh.vx[ol]; p.vy[j] = sh.vy[ol]; p.vz[j] = sh.vz[ol];
+                if (p.fx) p.fx[j] = 0.0;
+                if (p.fy) p.fy[j] = 0.0;
+                if (p.fz) p.fz[j] = 0.0;
+                p.mass[j] = sh.m[ol]; p.charge[j] = sh.q[ol];
+                p.type[j] = sh.ty[ol]; p.global_id[j] = sh.id[ol];
+            }
+        } else {
+            // more images than the map holds: one lane per owner
+            for (uint32_t t = wave; t < kRunTiles; t += kRunWaves) {
+                const uint32_t ol = t * kWave + lane;
+                const uint32_t e = e0 + ol;
+                const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(sh.code[ol]), kWave));
+                if (e >= owners) continue;
+                const uint32_t* list = glist + B.image_start[slot0 + e];
+                uint32_t k = 0;
+                for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k) {
+                    const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
+                    const uint64_t j = ghost0 + list[k];
+                    p.x[j] = sh.px[ol] + sh.table.shift[s][0];
+                    p.y[j] = sh.py[ol] + sh.table.shift[s][1];
+                    p.z[j] = sh.pz[ol] + sh.table.shift[s][2];
+                    p.vx[j] = sh.vx[ol]; p.vy[j] = sh.vy[ol]; p.vz[j] = sh.vz[ol];
+                    if (p.fx) p.fx[j] = 0.0;
+                    if (p.fy) p.fy[j] = 0.0;
+                    if (p.fz) p.fz[j] = 0.0;
+                    p.mass[j] = sh.m[ol]; p.charge[j] = sh.q[ol];
+                    p.type[j] = sh.ty[ol]; p.global_id[j] = sh.id[ol];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // reverse_halo: f[i] = ((f[i] + g_s1) + g_s2) + ... over the owner's send
 // slots s1 < s2 < ..., the order of the Host reduction (src/halo.cpp:1357-1370
 // for n ascending).  No atomics: one lane sums one owner.  The forces of the
@@ -726,6 +911,14 @@ hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Lay
 {
     const Layout O = ordered(L, s, kOrderCopy);
     if (L.tiles == 0) return hipSuccess;
+    // One box that sends to itself and whose records exceed the memory-side
+    // cache: ghost runs staged through LDS (one workgroup per unit, each
+    // walking units g, g + grid, ... up to the device's owner count).
+    if (L.replicas == 1 && B.recv == nullptr && L.streaming) {
+        const uint32_t tiles = L.tile_waves ? L.tile_waves : L.tiles;
+        const dim3 grid(std::max<uint32_t>(1, (tiles + kRunTiles - 1) / kRunTiles));
+        return launch(copy_runs_kernel<true>, grid, dim3(kRunWaves * kWave), stream, p, fp, L.desc, O, B);
+    }
     return dispatch_waves(s, [&](auto waves) {
         return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr int W = decltype(waves)::value;

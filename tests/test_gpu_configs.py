@@ -8,6 +8,11 @@ oracle on identical inputs:
   reverse_exchange(Sum); three steps.
 - uniform, 16,777,216 particles in L = 256, periodic and with z non-periodic
   (reflecting walls applied by the application); one full step.
+- 3,000,000 particles (owned records larger than the 256 MB memory-side
+  cache, so the ghost copy stages its runs through LDS): with an f64 and an
+  f32 3-vector field, walking its units first to last and last to first, and
+  with a halo so wide (w = 4 in L = 20) that most units hold more images than
+  the staging map and are written one lane per owner.
 """
 import numpy as np
 import pytest
@@ -124,3 +129,50 @@ def test_uniform_16m_full_size_step(gpu, zper):
     halo.reverse_halo(dev, n)
     for k in ("fx", "fy", "fz"):
         assert_bits_equal(host(getattr(dev, k)[:n]), getattr(op, k)[:n], f"uniform 16M reverse {k}")
+
+
+@pytest.mark.parametrize("L,w,descending", [(160.0, 2.8, 0), (160.0, 2.8, 8), (20.0, 4.0, 0)],
+                         ids=["runs", "runs_last_to_first", "wide_halo_per_owner"])
+def test_large_box_ghost_runs(gpu, monkeypatch, L, w, descending):
+    torch = gpu
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    n = 3_000_000
+    monkeypatch.setenv("DSDX_DESCENDING", str(1 | 16 | descending))
+    say("large box ghost runs", n, L, w, descending)
+    d = synth(n, L, 4040 + int(L) + descending)
+    rng = np.random.default_rng(4041)
+    images = ((1 + 2 * w / L) ** 3 - 1) * n
+    cap = int(n + 1.1 * images) + 4096
+    dens = np.zeros(cap)
+    dens[:n] = rng.uniform(0, 2, n)
+    vec = np.zeros((cap, 3), np.float32)
+    vec[:n] = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    ora = po.Oracle(po.Box(upper=(L,) * 3, width=w))
+    op = to_oracle(d, cap)
+    dev = to_device(d, cap)
+    td = torch.from_numpy(dens.copy()).cuda()
+    tv = torch.from_numpy(vec.copy()).cuda()
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
+    for step in range(2):
+        say("large box step", step)
+        disp = rng.normal(0, 0.1, (3, n))
+        for a, k in enumerate(("x", "y", "z")):
+            getattr(op, k)[:n] += disp[a]
+            getattr(dev, k)[:n] += torch.from_numpy(disp[a]).cuda()
+        ora.migrate(op, n)
+        halo.migrate(dev, n)
+        g = ora.exchange_halo(op, n, fields=[po.Field(dens), po.Field(vec)])
+        assert halo.exchange_halo(dev, n, fields=[td, tv]) == g
+        assert abs(g - images) < 0.02 * images, (g, images)
+        compare_particles(dev, op, n + g, f"large box step {step}")
+        assert_bits_equal(host(td[:n + g]), dens[:n + g], "density")
+        assert_bits_equal(host(tv[:n + g]).reshape(-1).view(np.int32), vec[:n + g].reshape(-1).view(np.int32),
+                          "vec f32")
+        f = rng.uniform(-1, 1, (3, n + g))
+        for a, k in enumerate(("fx", "fy", "fz")):
+            getattr(op, k)[:n + g] = f[a]
+            getattr(dev, k)[:n + g] = torch.from_numpy(f[a]).cuda()
+        ora.reverse_halo(op, n)
+        halo.reverse_halo(dev, n)
+        for k in ("fx", "fy", "fz"):
+            assert_bits_equal(host(getattr(dev, k)[:n]), getattr(op, k)[:n], f"large box reverse {k}")
