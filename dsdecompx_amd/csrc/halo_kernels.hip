@@ -346,28 +346,35 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
     });
 }
 
-// The ghost copy of one large self-loopback box, staged through LDS.  A
+// The ghost writes of one large self-loopback box, staged through LDS.  A
 // workgroup takes a unit of kRunTiles owner tiles (kRunOwners consecutive
 // boundary owners).  The unit's ghosts through send slot s are consecutive
 // ghosts of receive block 25 - s (their ranks among the slot's owners are
 // consecutive), so the unit writes one run per slot:
-//   1. one lane per owner gathers the owner's record (velocity, mass,
-//      charge, type, id, and its position from the classify pass) into LDS,
-//      and counts the unit's images per tile and slot; the first owner of
-//      each slot reads where the slot's run starts (its ghost-list entry);
+//   1. one lane per owner gathers the owner's record into LDS and counts the
+//      unit's images per tile and slot; the first owner of each slot reads
+//      where the slot's run starts (its ghost-list entry);
 //   2. the image counts become, for every image of the unit, its index in
 //      the unit's runs laid end to end, and map[index] = its owner;
-//   3. lane k of a round writes ghost k of those runs, so a store
-//      instruction covers whole cache lines of at most a few runs.
-// The per-owner form (copy_kernel) writes each tile's short runs lane by
-// lane: with 13 arrays and 26 receive blocks open at once, the partly
-// written lines leave the L2 before their neighbours arrive.  A unit whose
-// images exceed the map is written by one lane per owner from the same LDS
-// records.  Registered fields are copied per owner in step 1.
+//   3. lane k of a round writes run index k, so a store instruction covers
+//      whole cache lines of at most a few runs.
+// The per-owner kernels write each tile's short runs lane by lane: with 13
+// arrays and 26 receive blocks open at once, the partly written lines leave
+// the L2 before their neighbours arrive.  A unit whose images exceed the map
+// is written by one lane per owner from the same LDS records.  Registered
+// fields are copied per owner in step 1.
+//
+// MODE kRunsBuild: the rebuild's copy, positions from the classify pass
+// (bpos).  kRunsRefresh: the reused-plan refresh of the whole record, the
+// owner's current position.  kRunsPositions: the positions-only refresh
+// (x, y, z and zeroed forces).
 constexpr uint32_t kRunTiles = 8;
 constexpr uint32_t kRunWaves = 8;
 constexpr uint32_t kRunOwners = kRunTiles * kOwnerTile;
 constexpr uint32_t kRunMap = 2 * kRunOwners;
+constexpr int kRunsBuild = 0;
+constexpr int kRunsRefresh = 1;
+constexpr int kRunsPositions = 2;
 
 struct RunStage {
     double px[kRunOwners], py[kRunOwners], pz[kRunOwners];
@@ -383,7 +390,24 @@ struct RunStage {
     SlotTable table;
 };
 
-template <bool STREAM>
+template <int MODE>
+__device__ __forceinline__ void write_run_ghost(const dsdx_particles& p, const RunStage& sh, uint32_t ol,
+                                                uint32_t s, uint64_t j)
+{
+    p.x[j] = sh.px[ol] + sh.table.shift[s][0];
+    p.y[j] = sh.py[ol] + sh.table.shift[s][1];
+    p.z[j] = sh.pz[ol] + sh.table.shift[s][2];
+    if (p.fx) p.fx[j] = 0.0;
+    if (p.fy) p.fy[j] = 0.0;
+    if (p.fz) p.fz[j] = 0.0;
+    if (MODE != kRunsPositions) {
+        p.vx[j] = sh.vx[ol]; p.vy[j] = sh.vy[ol]; p.vz[j] = sh.vz[ol];
+        p.mass[j] = sh.m[ol]; p.charge[j] = sh.q[ol];
+        p.type[j] = sh.ty[ol]; p.global_id[j] = sh.id[ol];
+    }
+}
+
+template <int MODE, bool STREAM>
 __global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_particles p, FieldParams fp,
                                                                       const ReplicaDesc* __restrict__ desc,
                                                                       Layout L, PlanBuffers B)
@@ -411,30 +435,38 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_parti
             const uint32_t e = e0 + ol;
             const bool live = e < owners;
             const uint64_t u = slot0 + e;
-            uint32_t oi = 0, code = 0, E = 0, start = 0;
+            uint32_t oi = 0, code = 0, start = 0;
             if (live) {
                 oi = B.owner_index[u];
                 code = B.owner_code[u];
-                E = B.owner_entry[u];
                 start = B.image_start[u];
             }
             const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(code), kWave));
             if (live) {
                 const uint64_t i = d->offset + oi;
-                sh.px[ol] = B.bpos[E];
-                sh.py[ol] = B.bpos[B.bpos_stride + E];
-                sh.pz[ol] = B.bpos[2 * B.bpos_stride + E];
-                sh.vx[ol] = read_once<STREAM>(p.vx + i);
-                sh.vy[ol] = read_once<STREAM>(p.vy + i);
-                sh.vz[ol] = read_once<STREAM>(p.vz + i);
-                sh.m[ol] = read_once<STREAM>(p.mass + i);
-                sh.q[ol] = read_once<STREAM>(p.charge + i);
-                sh.ty[ol] = read_once<STREAM>(p.type + i);
-                sh.id[ol] = read_once<STREAM>(p.global_id + i);
-                if (fp.count) {
-                    uint32_t k = 0;
-                    for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
-                        copy_field_words(fp, i, ghost0 + glist[start + k]);
+                if (MODE == kRunsBuild) {
+                    const uint32_t E = B.owner_entry[u];
+                    sh.px[ol] = B.bpos[E];
+                    sh.py[ol] = B.bpos[B.bpos_stride + E];
+                    sh.pz[ol] = B.bpos[2 * B.bpos_stride + E];
+                } else {
+                    sh.px[ol] = read_once<STREAM>(p.x + i);
+                    sh.py[ol] = read_once<STREAM>(p.y + i);
+                    sh.pz[ol] = read_once<STREAM>(p.z + i);
+                }
+                if (MODE != kRunsPositions) {
+                    sh.vx[ol] = read_once<STREAM>(p.vx + i);
+                    sh.vy[ol] = read_once<STREAM>(p.vy + i);
+                    sh.vz[ol] = read_once<STREAM>(p.vz + i);
+                    sh.m[ol] = read_once<STREAM>(p.mass + i);
+                    sh.q[ol] = read_once<STREAM>(p.charge + i);
+                    sh.ty[ol] = read_once<STREAM>(p.type + i);
+                    sh.id[ol] = read_once<STREAM>(p.global_id + i);
+                    if (fp.count) {
+                        uint32_t k = 0;
+                        for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
+                            copy_field_words(fp, i, ghost0 + glist[start + k]);
+                    }
                 }
             }
             sh.code[ol] = static_cast<uint8_t>(code);
@@ -488,20 +520,9 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_parti
             }
             __syncthreads();
             // 3. lane k of a round writes run index k
-            for (uint32_t r = wave * kWave + lane; r < ((images + kWave - 1) / kWave) * kWave; r += kRunWaves * kWave) {
-                if (r >= images) continue;
+            for (uint32_t r = wave * kWave + lane; r < images; r += kRunWaves * kWave) {
                 const uint32_t s = sh.slot_of[r];
-                const uint32_t ol = sh.map[r];
-                const uint64_t j = ghost0 + sh.first[s] + (r - sh.run_start[s]);
-                p.x[j] = sh.px[ol] + sh.table.shift[s][0];
-                p.y[j] = sh.py[ol] + sh.table.shift[s][1];
-                p.z[j] = sh.pz[ol] + sh.table.shift[s][2];
-                p.vx[j] = sh.vx[ol]; p.vy[j] = sh.vy[ol]; p.vz[j] = sh.vz[ol];
-                if (p.fx) p.fx[j] = 0.0;
-                if (p.fy) p.fy[j] = 0.0;
-                if (p.fz) p.fz[j] = 0.0;
-                p.mass[j] = sh.m[ol]; p.charge[j] = sh.q[ol];
-                p.type[j] = sh.ty[ol]; p.global_id[j] = sh.id[ol];
+                write_run_ghost<MODE>(p, sh, sh.map[r], s, ghost0 + sh.first[s] + (r - sh.run_start[s]));
             }
         } else {
             // more images than the map holds: one lane per owner
@@ -512,23 +533,25 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_parti
                 if (e >= owners) continue;
                 const uint32_t* list = glist + B.image_start[slot0 + e];
                 uint32_t k = 0;
-                for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k) {
-                    const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
-                    const uint64_t j = ghost0 + list[k];
-                    p.x[j] = sh.px[ol] + sh.table.shift[s][0];
-                    p.y[j] = sh.py[ol] + sh.table.shift[s][1];
-                    p.z[j] = sh.pz[ol] + sh.table.shift[s][2];
-                    p.vx[j] = sh.vx[ol]; p.vy[j] = sh.vy[ol]; p.vz[j] = sh.vz[ol];
-                    if (p.fx) p.fx[j] = 0.0;
-                    if (p.fy) p.fy[j] = 0.0;
-                    if (p.fz) p.fz[j] = 0.0;
-                    p.mass[j] = sh.m[ol]; p.charge[j] = sh.q[ol];
-                    p.type[j] = sh.ty[ol]; p.global_id[j] = sh.id[ol];
-                }
+                for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
+                    write_run_ghost<MODE>(p, sh, ol, static_cast<uint32_t>(__builtin_ctz(rest)), ghost0 + list[k]);
             }
         }
         __syncthreads();
     }
+}
+
+// One self-loopback box whose owned records exceed the memory-side cache:
+// its ghost writes go through copy_runs_kernel.
+inline bool ghost_runs(const Layout& L, const PlanBuffers& B)
+{
+    return L.replicas == 1 && B.recv == nullptr && L.streaming;
+}
+
+inline dim3 run_grid(const Layout& L)
+{
+    const uint32_t tiles = L.tile_waves ? L.tile_waves : L.tiles;
+    return dim3(std::max<uint32_t>(1, (tiles + kRunTiles - 1) / kRunTiles));
 }
 
 // reverse_halo: f[i] = ((f[i] + g_s1) + g_s2) + ... over the owner's send
@@ -911,14 +934,10 @@ hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Lay
 {
     const Layout O = ordered(L, s, kOrderCopy);
     if (L.tiles == 0) return hipSuccess;
-    // One box that sends to itself and whose records exceed the memory-side
-    // cache: ghost runs staged through LDS (one workgroup per unit, each
-    // walking units g, g + grid, ... up to the device's owner count).
-    if (L.replicas == 1 && B.recv == nullptr && L.streaming) {
-        const uint32_t tiles = L.tile_waves ? L.tile_waves : L.tiles;
-        const dim3 grid(std::max<uint32_t>(1, (tiles + kRunTiles - 1) / kRunTiles));
-        return launch(copy_runs_kernel<true>, grid, dim3(kRunWaves * kWave), stream, p, fp, L.desc, O, B);
-    }
+    // Each workgroup walks units g, g + grid, ... up to the device's owner count.
+    if (ghost_runs(L, B))
+        return launch(copy_runs_kernel<kRunsBuild, true>, run_grid(L), dim3(kRunWaves * kWave), stream, p, fp,
+                      L.desc, O, B);
     return dispatch_waves(s, [&](auto waves) {
         return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr int W = decltype(waves)::value;
@@ -970,6 +989,13 @@ hipError_t launch_refresh(const dsdx_particles& p, const FieldParams& fp, int32_
 {
     const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
+    if (ghost_runs(L, B)) {
+        if (positions_only)
+            return launch(copy_runs_kernel<kRunsPositions, true>, run_grid(L), dim3(kRunWaves * kWave), stream, p,
+                          fp, L.desc, O, B);
+        return launch(copy_runs_kernel<kRunsRefresh, true>, run_grid(L), dim3(kRunWaves * kWave), stream, p, fp,
+                      L.desc, O, B);
+    }
     return dispatch_waves(s, [&](auto waves) {
         return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr int W = decltype(waves)::value;

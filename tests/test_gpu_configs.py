@@ -12,7 +12,8 @@ oracle on identical inputs:
   cache, so the ghost copy stages its runs through LDS): with an f64 and an
   f32 3-vector field, walking its units first to last and last to first, and
   with a halo so wide (w = 4 in L = 20) that most units hold more images than
-  the staging map and are written one lane per owner.
+  the staging map and are written one lane per owner; then the reused-plan
+  refresh, positions only and the whole record with the fields.
 """
 import numpy as np
 import pytest
@@ -176,3 +177,28 @@ def test_large_box_ghost_runs(gpu, monkeypatch, L, w, descending):
         halo.reverse_halo(dev, n)
         for k in ("fx", "fy", "fz"):
             assert_bits_equal(host(getattr(dev, k)[:n]), getattr(op, k)[:n], f"large box reverse {k}")
+    # the last plan again: owners move a little, the ghosts re-read them
+    for full in (False, True):
+        say("large box refresh full", full)
+        dx = rng.normal(0, 0.05, (3, n))
+        for a, k in enumerate(("x", "y", "z")):
+            getattr(op, k)[:n] += dx[a]
+            getattr(dev, k)[:n] = torch.from_numpy(getattr(op, k)[:n].copy()).cuda()
+        if full:
+            op.vx[:n] += 1.0
+            dev.vx[:n] = torch.from_numpy(op.vx[:n].copy()).cuda()
+            dens[:n] += 1.0
+            td[:n] = torch.from_numpy(dens[:n].copy()).cuda()
+        before = {k: host(getattr(dev, k)[:n + g]).copy() for k in F64 + I32}
+        dens_ref, vec_ref = host(td).copy(), host(tv).copy()
+        halo.refresh(dev, n, fields=[td, tv] if full else (), positions_only=not full)
+        ref = to_oracle({k: getattr(op, k)[:n].copy() for k in F64 + I32}, n + g)
+        for k in F64 + I32:
+            getattr(ref, k)[n:n + g] = before[k][n:]
+        ora.write_ghosts(ref, n, fields=[po.Field(dens_ref), po.Field(vec_ref)] if full else ())
+        if not full:   # positions_only: only x, y, z and zero forces change
+            for k in ("vx", "vy", "vz", "mass", "charge", "type", "global_id"):
+                getattr(ref, k)[n:n + g] = before[k][n:]
+        compare_particles(dev, ref, n + g, f"large box refresh full={full}")
+        assert_bits_equal(host(td[:n + g]), dens_ref[:n + g], "refresh density")
+        assert_bits_equal(host(tv[:n + g]), vec_ref[:n + g], "refresh vec f32")
