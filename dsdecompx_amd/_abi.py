@@ -112,6 +112,10 @@ def lib():
         "dsdx_batch_ghost_owners": ([P, P, P], C.c_int),
         "dsdx_batch_counts": ([P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32), P], C.c_int),
         "dsdx_batch_device_totals": ([P], C.c_void_p),
+        "dsdx_graph_begin": ([P], C.c_int),
+        "dsdx_graph_end": ([P, C.POINTER(P)], C.c_int),
+        "dsdx_graph_launch": ([P, P], C.c_int),
+        "dsdx_graph_destroy": ([P], C.c_int),
         "dsdx_grid_create": ([C.POINTER(Box), C.POINTER(i32), u64, P, C.POINTER(P)], C.c_int),
         "dsdx_grid_destroy": ([P], C.c_int),
         "dsdx_grid_set_owned": ([P, C.POINTER(u64), P], C.c_int),
@@ -161,7 +165,8 @@ def exported_symbols():
             "dsdx_batch_migrate_wrap", "dsdx_batch_halo_build", "dsdx_batch_refresh",
             "dsdx_batch_reverse_halo", "dsdx_batch_reverse_exchange",
             "dsdx_batch_forward_exchange", "dsdx_batch_ghost_owners", "dsdx_batch_counts",
-            "dsdx_batch_device_totals", "dsdx_migrate_halo_build",
+            "dsdx_batch_device_totals", "dsdx_graph_begin", "dsdx_graph_end",
+            "dsdx_graph_launch", "dsdx_graph_destroy", "dsdx_migrate_halo_build",
             "dsdx_batch_migrate_halo_build", "dsdx_grid_create", "dsdx_grid_destroy",
             "dsdx_grid_set_owned", "dsdx_grid_counts", "dsdx_grid_migrate",
             "dsdx_grid_halo_build", "dsdx_grid_reverse_halo", "dsdx_grid_reverse_exchange",

@@ -1,7 +1,8 @@
 """Batched replicas: many independent periodic boxes in one set of device SoA
 arrays, each phase of the step (migrate, exchange_halo, reverse_halo, field
 exchanges) one kernel launch over all of them, and the whole step capturable
-in a hipGraph (torch.cuda.CUDAGraph is hipGraph on ROCm).
+in a HIP graph through the library (dsdx_graph_begin / dsdx_graph_end,
+``capture_step``).  torch tensors here are only the device arrays and streams.
 
 ``ReplicaBatch`` lays the replicas out back to back: replica r owns particle
 slots [offset_r, offset_r + capacity_r), owned first.  The per-replica
@@ -222,11 +223,36 @@ class LoopedReplicas:
                                                     spec.owned, s))
 
 
+class CapturedStep:
+    """A step recorded by dsdx_graph_begin / dsdx_graph_end (a HIP graph the
+    library instantiates); replay() launches it on the current stream."""
+
+    def __init__(self, graph):
+        self._graph = graph
+
+    def replay(self, stream=None) -> None:
+        _abi.check(_abi.lib().dsdx_graph_launch(self._graph, ReplicaBatch._stream(stream)),
+                   "graph launch")
+
+    def close(self) -> None:
+        if self._graph:
+            _abi.check(_abi.lib().dsdx_graph_destroy(self._graph), "graph destroy")
+            self._graph = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def capture_step(batch: ReplicaBatch, app_before_halo=None, app_before_reverse=None,
-                 warmup: int = 2, fused: bool = False):
+                 warmup: int = 2, fused: bool = False) -> CapturedStep:
     """Captures migrate -> exchange_halo -> [app] -> reverse_halo of every
-    replica in one hipGraph and returns the torch.cuda.CUDAGraph.  The
-    application callbacks (displacement, forces) are captured in place.  With
+    replica in one HIP graph through the library (dsdx_graph_begin /
+    dsdx_graph_end) and returns it.  The application callbacks
+    (displacement, forces) are recorded in place; they run on the capturing
+    stream and must not allocate.  `warmup` steps run eagerly first.  With
     `fused` migrate and the rebuild share one pass over the positions."""
     _abi.timing_enable(False)
     s = torch.cuda.Stream()
@@ -247,8 +273,14 @@ def capture_step(batch: ReplicaBatch, app_before_halo=None, app_before_reverse=N
     with torch.cuda.stream(s):
         for _ in range(warmup):
             body()
+    s.synchronize()
+    graph = C.c_void_p()
+    with torch.cuda.stream(s):
+        _abi.check(_abi.lib().dsdx_graph_begin(C.c_void_p(s.cuda_stream)), "graph begin")
+        try:
+            body()
+        finally:
+            rc = _abi.lib().dsdx_graph_end(C.c_void_p(s.cuda_stream), C.byref(graph))
+        _abi.check(rc, "graph end")
     torch.cuda.current_stream().wait_stream(s)
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g, stream=s):
-        body()
-    return g
+    return CapturedStep(graph)

@@ -982,4 +982,61 @@ const uint32_t* dsdx_batch_device_totals(const dsdx_batch* batch)
     return batch ? batch->totals.as<uint32_t>() : nullptr;
 }
 
+// A step captured on a stream (thread-local capture mode: only this thread's
+// allocations and waits are barred while it records) and instantiated once.
+struct dsdx_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int dsdx_graph_begin(void* stream)
+{
+    if (!stream) return fail(DSDX_EINVAL, "graph capture needs a stream other than the null stream");
+    DSDX_HIP(hipStreamBeginCapture(static_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal),
+             "begin graph capture");
+    return DSDX_OK;
+}
+
+int dsdx_graph_end(void* stream, dsdx_graph** graph)
+{
+    if (!stream || !graph) return fail(DSDX_EINVAL, "null stream or graph pointer");
+    *graph = nullptr;
+    hipGraph_t g = nullptr;
+    DSDX_HIP(hipStreamEndCapture(static_cast<hipStream_t>(stream), &g), "end graph capture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t st = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (st != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(st, "instantiate captured graph");
+    }
+    auto* out = new (std::nothrow) dsdx_graph();
+    if (!out) {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(g);
+        return fail(DSDX_EDEVICE, "out of host memory");
+    }
+    out->graph = g;
+    out->exec = exec;
+    *graph = out;
+    return DSDX_OK;
+}
+
+int dsdx_graph_launch(const dsdx_graph* graph, void* stream)
+{
+    if (!graph) return fail(DSDX_EINVAL, "null graph");
+    DSDX_HIP(hipGraphLaunch(graph->exec, static_cast<hipStream_t>(stream)), "launch captured graph");
+    return DSDX_OK;
+}
+
+int dsdx_graph_destroy(dsdx_graph* graph)
+{
+    if (!graph) return DSDX_OK;
+    const hipError_t a = hipGraphExecDestroy(graph->exec);
+    const hipError_t b = hipGraphDestroy(graph->graph);
+    delete graph;
+    if (a != hipSuccess) return hip_fail(a, "destroy graph executable");
+    if (b != hipSuccess) return hip_fail(b, "destroy graph");
+    return DSDX_OK;
+}
+
 }  // extern "C"

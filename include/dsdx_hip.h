@@ -312,6 +312,20 @@ int dsdx_batch_counts(const dsdx_batch* batch, uint32_t* ghosts,
  * kernels that must know a replica's ghost count without a host wait. */
 const uint32_t* dsdx_batch_device_totals(const dsdx_batch* batch);
 
+/* A step captured into a HIP graph and replayed: dsdx_graph_begin starts a
+ * thread-local capture on `stream` (not the null stream); the calls that
+ * follow on that stream (dsdx_batch_*, and the application's own kernels)
+ * are recorded, not run; dsdx_graph_end ends the capture and instantiates
+ * the graph; dsdx_graph_launch replays it on any stream.  A call that
+ * allocates or waits on the host inside the capture ends it with an error
+ * (the batch calls do neither).  No counterpart in the reference (its steps
+ * are host-driven); for callers that replay a fixed step. */
+typedef struct dsdx_graph dsdx_graph;
+int dsdx_graph_begin(void* stream);
+int dsdx_graph_end(void* stream, dsdx_graph** graph);
+int dsdx_graph_launch(const dsdx_graph* graph, void* stream);
+int dsdx_graph_destroy(dsdx_graph* graph);
+
 /* ---- Communication records ------------------------------------------------
  * The reference's wire formats, for callers that carry records over a
  * transport of their own (MPI, RCCL) between the pack and unpack steps:

@@ -1,5 +1,5 @@
 """Writes tests/golden/ref_grid_2x2x2.npz from the reference itself (TEST
-INFRASTRUCTURE, run in the survey container only, where /root/reference and
+INFRASTRUCTURE, run in the build container only, where /root/reference and
 MPICH exist): builds the reference's Host library and this project's fixture
 writer tests/golden/ref_grid_fixture.cpp with oracle/ref.mk, runs it on 8 MPI
 ranks (a 2x2x2 periodic process grid, the reference unmodified) and stores
