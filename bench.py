@@ -457,12 +457,15 @@ DEVICE_KERNELS = {"migrate_wrap": ("wrap_box_kernel", "migrate_wrap_kernel"),
 CALIBRATION_BYTES = 1 << 30
 
 
-def _counter_means(path):
-    """Mean counter value per dispatch, by plain kernel name."""
+def _counter_means(path, counter=None):
+    """Mean value per dispatch of `counter` (of the file's only counter when
+    None), by plain kernel name."""
     import csv
     sums = {}
     with open(path) as f:
         for row in csv.DictReader(f):
+            if counter is not None and row["Counter_Name"] != counter:
+                continue
             name = row["Kernel_Name"].replace("void ", "").replace("(anonymous namespace)::", "")
             name = name.replace("dsdx::", "").split("(")[0].split("<")[0].strip()
             total, count = sums.get(name, (0.0, 0))
@@ -474,7 +477,10 @@ def measure_traffic(n, L, w, sigma, periodic, env, seed):
     """HBM bytes per dispatch of every kernel of the step, from two rocprofv3
     counter runs of the C++ driver on the same workload (FETCH_SIZE, then
     WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
-    for comm_bench's plain copy of CALIBRATION_BYTES in the same run.
+    for comm_bench's plain copy of CALIBRATION_BYTES in the same run; and
+    from three more runs the L2 (TCC) hit rate and the memory requests by
+    size (reads are whole 128-byte lines on gfx950; writes go out as 64- or
+    32-byte requests, the 32-byte ones being partly written lines).
     None when rocprofv3 is not on PATH."""
     import glob
     import shutil
@@ -489,19 +495,25 @@ def measure_traffic(n, L, w, sigma, periodic, env, seed):
     tmp = tempfile.mkdtemp(prefix="dsdx_pmc_", dir="/tmp")
     means, scale = {}, {}
     try:
-        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-            out_dir = os.path.join(tmp, counter)
-            cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv",
+        # one pass per group (the TCC block collects at most 4 counters at once)
+        for group in (("FETCH_SIZE",), ("WRITE_SIZE",), ("TCC_HIT_sum", "TCC_MISS_sum"),
+                      ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum"), ("TCC_EA0_RDREQ_sum",)):
+            out_dir = os.path.join(tmp, group[0])
+            cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", *group, "--output-format", "csv",
                    "-d", out_dir, "-o", "run", "--", exe, str(n), repr(L), repr(w), repr(sigma),
                    "1" if periodic[2] else "0", "5", "2", str(seed), str(CALIBRATION_BYTES), "0"]
             r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
                                env=env, cwd="/tmp")
             files = glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True)
             if r.returncode != 0 or not files:
-                return {"error": f"rocprofv3 --pmc {counter} exited {r.returncode}: "
-                                 f"{r.stderr.strip()[-300:]}"}
-            means[counter] = _counter_means(files[0])
-            scale[counter] = means[counter]["calibration_copy"] * 1024.0 / CALIBRATION_BYTES
+                if group[0] in ("FETCH_SIZE", "WRITE_SIZE"):
+                    return {"error": f"rocprofv3 --pmc {group[0]} exited {r.returncode}: "
+                                     f"{r.stderr.strip()[-300:]}"}
+                continue
+            for counter in group:
+                means[counter] = _counter_means(files[0], counter)
+            if group[0] in ("FETCH_SIZE", "WRITE_SIZE"):
+                scale[group[0]] = means[group[0]]["calibration_copy"] * 1024.0 / CALIBRATION_BYTES
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     per_kernel = {}
@@ -511,13 +523,26 @@ def measure_traffic(n, L, w, sigma, periodic, env, seed):
                 continue
             rd = means["FETCH_SIZE"][dev] * 1024.0 / scale["FETCH_SIZE"]
             wr = means["WRITE_SIZE"][dev] * 1024.0 / scale["WRITE_SIZE"]
-            per_kernel[lib_name] = {"read_bytes": rd, "write_bytes": wr, "bytes": rd + wr}
+            entry = {"device_kernel": dev, "read_bytes": rd, "write_bytes": wr, "bytes": rd + wr}
+            c = {k: means[k][dev] for k in ("TCC_HIT_sum", "TCC_MISS_sum", "TCC_EA0_WRREQ_sum",
+                                             "TCC_EA0_WRREQ_64B_sum", "TCC_EA0_RDREQ_sum")
+                 if k in means and dev in means[k]}
+            if "TCC_HIT_sum" in c and "TCC_MISS_sum" in c and c["TCC_HIT_sum"] + c["TCC_MISS_sum"] > 0:
+                entry["tcc_hit_rate"] = c["TCC_HIT_sum"] / (c["TCC_HIT_sum"] + c["TCC_MISS_sum"])
+            if "TCC_EA0_WRREQ_sum" in c and "TCC_EA0_WRREQ_64B_sum" in c:
+                entry["ea_write_requests_64B"] = c["TCC_EA0_WRREQ_64B_sum"]
+                entry["ea_write_requests_32B"] = c["TCC_EA0_WRREQ_sum"] - c["TCC_EA0_WRREQ_64B_sum"]
+            if "TCC_EA0_RDREQ_sum" in c:
+                entry["ea_read_requests"] = c["TCC_EA0_RDREQ_sum"]
+            per_kernel[lib_name] = entry
             break
     return {"kernels": per_kernel,
             "scale": {"FETCH_SIZE": scale["FETCH_SIZE"], "WRITE_SIZE": scale["WRITE_SIZE"]},
             "method": f"rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, on comm_bench (5 steps after "
                       f"2 warm-up steps); KiB per dispatch x 1024 / (counter of a "
-                      f"{CALIBRATION_BYTES}-byte plain copy x 1024 / {CALIBRATION_BYTES})"}
+                      f"{CALIBRATION_BYTES}-byte plain copy x 1024 / {CALIBRATION_BYTES}); then "
+                      f"TCC_HIT_sum + TCC_MISS_sum, TCC_EA0_WRREQ_sum + TCC_EA0_WRREQ_64B_sum and "
+                      f"TCC_EA0_RDREQ_sum, one run each, per dispatch"}
 
 
 def measure_wire(args, n, L, w):
@@ -789,6 +814,10 @@ def run(args):
                     per_kernel[k]["hbm_bytes"] = v["bytes"]
                     per_kernel[k]["hbm_read_bytes"] = v["read_bytes"]
                     per_kernel[k]["hbm_write_bytes"] = v["write_bytes"]
+                    for extra in ("device_kernel", "tcc_hit_rate", "ea_read_requests",
+                                  "ea_write_requests_64B", "ea_write_requests_32B"):
+                        if extra in v:
+                            per_kernel[k][extra] = v[extra]
                     if per_kernel[k].get("avg_ms"):
                         per_kernel[k]["hbm_GBps"] = v["bytes"] / (per_kernel[k]["avg_ms"] * 1e-3) / 1e9
             dom = t["kernels"].get(dominant)
