@@ -256,8 +256,9 @@ void number_chunks(Plan* plan)
 // at most min(capacity - owned, ghost list capacity) ghosts.
 void number_tiles(Plan* plan)
 {
-    uint32_t next = 0;
+    uint32_t next = 0, next_unit = 0;
     plan->tile_replica.clear();
+    plan->unit_replica.clear();
     for (size_t r = 0; r < plan->desc.size(); ++r) {
         ReplicaDesc& d = plan->desc[r];
         // a self-routed replica's ghosts are its own sends, so its storage
@@ -271,8 +272,14 @@ void number_tiles(Plan* plan)
         next += tiles;
         if (plan->desc.size() > 1)
             plan->tile_replica.insert(plan->tile_replica.end(), tiles, static_cast<uint32_t>(r));
+        const uint32_t units = (tiles + kRunTiles - 1) / kRunTiles;
+        d.first_unit = next_unit;
+        next_unit += units;
+        if (plan->desc.size() > static_cast<size_t>(kWave))
+            plan->unit_replica.insert(plan->unit_replica.end(), units, static_cast<uint32_t>(r));
     }
     plan->tiles = next;
+    plan->units = next_unit;
 }
 
 // Copies the table to the device.  A single box's descriptor goes down as the
@@ -303,6 +310,12 @@ int upload_table(Plan* plan, hipStream_t stream)
             DSDX_HIP(hipMemcpyAsync(plan->d_tile_replica.ptr, plan->tile_replica.data(),
                                     plan->tile_replica.size() * 4, hipMemcpyHostToDevice, stream),
                      "upload tile table");
+        DSDX_HIP(plan->d_unit_replica.reserve(std::max<size_t>(1, plan->unit_replica.size()) * 4),
+                 "hipMalloc");
+        if (!plan->unit_replica.empty())
+            DSDX_HIP(hipMemcpyAsync(plan->d_unit_replica.ptr, plan->unit_replica.data(),
+                                    plan->unit_replica.size() * 4, hipMemcpyHostToDevice, stream),
+                     "upload unit table");
     }
     DSDX_HIP(hipStreamSynchronize(stream), "wait for the replica table");
     plan->desc_on_device = true;
@@ -441,6 +454,7 @@ int shape_from_env(Shape* s)
     if (const char* v = std::getenv("DSDX_SCAN_THREADS")) s->scan_threads = std::atoi(v);
     if (const char* v = std::getenv("DSDX_XCD_RUN")) s->xcd_run = std::atoi(v);
     if (const char* v = std::getenv("DSDX_DESCENDING")) s->descending = std::atoi(v);
+    if (const char* v = std::getenv("DSDX_GHOST_RUNS")) s->ghost_runs = std::atoi(v);
     return shape_supported(*s) ? DSDX_OK : fail(DSDX_EINVAL, "unsupported launch shape");
 }
 

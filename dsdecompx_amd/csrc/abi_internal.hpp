@@ -45,11 +45,13 @@ struct Plan {
     std::vector<ReplicaDesc> desc;        // host copy of the table on the device
     std::vector<uint32_t> chunk_replica;  // host copy (R > 1 only)
     std::vector<uint32_t> tile_replica;   // host copy (R > 1 only)
-    DeviceBuffer d_desc, d_chunk_replica, d_tile_replica;
+    std::vector<uint32_t> unit_replica;   // host copy (R > kWave only)
+    DeviceBuffer d_desc, d_chunk_replica, d_tile_replica, d_unit_replica;
     bool desc_on_device = false;
     uint32_t chunks = 0;
     uint32_t max_replica_chunks = 0;
     uint32_t tiles = 0;
+    uint32_t units = 0;
     // Up to 64 replicas: live owner tiles of the last build whose counts the
     // host read (0 before the first).  Owner-major launches size their grid
     // from it; their waves walk however many tiles the device counts hold.
@@ -106,6 +108,8 @@ struct Plan {
         L.chunk_size = chunk_size();
         L.tiles = tiles;
         L.tile_waves = tiles;
+        L.unit_replica = desc.size() > static_cast<size_t>(kWave) ? d_unit_replica.as<uint32_t>() : nullptr;
+        L.units = units;
         if (desc.size() <= static_cast<size_t>(kWave) && live_tiles)
             L.tile_waves = std::min(tiles, live_tiles + live_tiles / 16 + 8);
         // Reads that a pass makes once stream past the caches when the owned
@@ -116,6 +120,7 @@ struct Plan {
         uint64_t owned_total = 0;
         for (const ReplicaDesc& d : desc) owned_total += d.owned;
         L.streaming = owned_total * kRecordBytes > kMallBytes ? 1u : 0u;
+        L.ghost_runs = shape.ghost_runs < 0 ? L.streaming : static_cast<uint32_t>(shape.ghost_runs != 0);
         L.xcd_run = static_cast<uint32_t>(shape.xcd_run);
         L.descending = 0;
         return L;

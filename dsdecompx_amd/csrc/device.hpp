@@ -75,7 +75,7 @@ struct ReplicaDesc {
     // on a process grid, kNoPeer where the slot has no neighbour.
     uint32_t dest[kSlots];
     uint32_t self_routed;          // every dest[s] is the replica itself
-    uint32_t pad;
+    uint32_t first_unit;           // first ghost-run unit (kRunTiles owner tiles each)
 };
 
 // The geometry migrate's wrap needs for one box, passed by value.
@@ -111,6 +111,9 @@ struct Shape {
     // the application has just written the forces (or positions) in index
     // order, and the highest owners' lines are the cached ones.
     int descending = kOrderMigrate | kOrderOwners;
+    // Ghost writes staged through LDS (copy_runs_kernel): -1 when the owned
+    // records exceed the memory-side cache, 0 never, 1 always.
+    int ghost_runs = -1;
 };
 
 // Owner tiles: the owner slots [0, bound_r) of replica r (relative to
@@ -118,6 +121,12 @@ struct Shape {
 // ghost_list_capacity_r) >= its boundary owners, cut into tiles of
 // kOwnerTile; one wave of the copy pass per tile.
 constexpr uint32_t kOwnerTile = kWave;
+
+// Ghost-run units of the staged copy: kRunTiles consecutive owner tiles of one
+// replica, one workgroup of kRunWaves waves each.
+constexpr uint32_t kRunTiles = 8;
+constexpr uint32_t kRunWaves = 8;
+constexpr uint32_t kRunOwners = kRunTiles * kOwnerTile;
 
 struct Layout {
     const ReplicaDesc* desc;        // device, one per replica
@@ -129,8 +138,11 @@ struct Layout {
     uint32_t chunk_size;            // kWave * items (a power of two)
     uint32_t tiles;                 // owner tiles over all replicas
     uint32_t tile_waves;            // waves of an owner-major launch (each walks tiles w, w + tile_waves, ...)
+    const uint32_t* unit_replica;   // device, replica of each ghost-run unit; null up to kWave replicas
+    uint32_t units;                 // ghost-run units over all replicas (ceil(tiles_r / kRunTiles) each)
     uint32_t streaming;             // the owned records exceed the memory-side cache: passes that
                                     // read an array once load it nontemporally (read_once, load_items)
+    uint32_t ghost_runs;            // ghost writes through copy_runs_kernel (Shape::ghost_runs)
     uint32_t xcd_run;               // block renumbering (see xcd_block)
     uint32_t descending;            // this launch walks its chunks / tiles last to first
 };

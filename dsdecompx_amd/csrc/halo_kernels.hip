@@ -346,10 +346,10 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
     });
 }
 
-// The ghost writes of one large self-loopback box, staged through LDS.  A
-// workgroup takes a unit of kRunTiles owner tiles (kRunOwners consecutive
+// The ghost writes of large boxes, staged through LDS.  A workgroup takes a
+// unit of kRunTiles owner tiles of one replica (kRunOwners consecutive
 // boundary owners).  The unit's ghosts through send slot s are consecutive
-// ghosts of receive block 25 - s (their ranks among the slot's owners are
+// ghosts of one receive block (their ranks among the slot's owners are
 // consecutive), so the unit writes one run per slot:
 //   1. one lane per owner gathers the owner's record into LDS and counts the
 //      unit's images per tile and slot; the first owner of each slot reads
@@ -362,15 +362,14 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
 // arrays and 26 receive blocks open at once, the partly written lines leave
 // the L2 before their neighbours arrive.  A unit whose images exceed the map
 // is written by one lane per owner from the same LDS records.  Registered
-// fields are copied per owner in step 1.
+// fields are copied per owner in step 1.  The receive block of slot s is the
+// replica's own (a loopback box or batch) or the neighbouring subdomain's (a
+// process grid), through the slot table.
 //
 // MODE kRunsBuild: the rebuild's copy, positions from the classify pass
 // (bpos).  kRunsRefresh: the reused-plan refresh of the whole record, the
 // owner's current position.  kRunsPositions: the positions-only refresh
 // (x, y, z and zeroed forces).
-constexpr uint32_t kRunTiles = 8;
-constexpr uint32_t kRunWaves = 8;
-constexpr uint32_t kRunOwners = kRunTiles * kOwnerTile;
 constexpr uint32_t kRunMap = 2 * kRunOwners;
 constexpr int kRunsBuild = 0;
 constexpr int kRunsRefresh = 1;
@@ -410,21 +409,17 @@ __device__ __forceinline__ void write_run_ghost(const dsdx_particles& p, const R
 template <int MODE, bool STREAM>
 __global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_particles p, FieldParams fp,
                                                                       const ReplicaDesc* __restrict__ desc,
+                                                                      const uint32_t* __restrict__ unit_replica,
                                                                       Layout L, PlanBuffers B)
 {
     __shared__ RunStage sh;
     const uint32_t lane = lane_id();
     const uint32_t wave = uniform(threadIdx.x / kWave);
-    const ReplicaDesc* d = desc;
-    const uint32_t owners = uniform(B.totals[kStatusWord]) ? 0u : uniform(B.totals[kColOwners]);
-    const uint32_t units = (owners + kRunOwners - 1) / kRunOwners;
-    if (wave == 0) fill_slot_table(sh.table, desc, d);
-    const uint32_t lut_l = d->bp.lut[lane];
-    const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * L.chunk_size;
-    const uint64_t ghost0 = d->offset + d->owned;
-    const uint32_t* glist = B.ghost_list + d->ghost_list_offset;
-    for (uint32_t c = xcd_block(L.xcd_run); c < units; c += gridDim.x) {
-        const uint32_t e0 = (L.descending ? units - 1u - c : c) * kRunOwners;
+    for_run_units(desc, unit_replica, L, B, [&](const ReplicaDesc* d, uint32_t e0, uint32_t owners) {
+        const uint32_t lut_l = d->bp.lut[lane];
+        const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * L.chunk_size;
+        const uint32_t* glist = B.ghost_list + d->ghost_list_offset;
+        if (wave == 0) fill_slot_table(sh.table, desc, d);
         for (uint32_t k = threadIdx.x; k < kRunTiles * kSlots; k += kRunWaves * kWave)
             (&sh.tile_rank[0][0])[k] = 0u;
         if (threadIdx.x < kSlots) sh.first[threadIdx.x] = 0xFFFFFFFFu;
@@ -465,7 +460,8 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_parti
                     if (fp.count) {
                         uint32_t k = 0;
                         for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
-                            copy_field_words(fp, i, ghost0 + glist[start + k]);
+                            copy_field_words(fp, i, ghost_slot(desc, d, static_cast<uint32_t>(__builtin_ctz(rest)),
+                                                               glist[start + k]));
                     }
                 }
             }
@@ -522,7 +518,7 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_parti
             // 3. lane k of a round writes run index k
             for (uint32_t r = wave * kWave + lane; r < images; r += kRunWaves * kWave) {
                 const uint32_t s = sh.slot_of[r];
-                write_run_ghost<MODE>(p, sh, sh.map[r], s, ghost0 + sh.first[s] + (r - sh.run_start[s]));
+                write_run_ghost<MODE>(p, sh, sh.map[r], s, sh.table.ghost0[s] + sh.first[s] + (r - sh.run_start[s]));
             }
         } else {
             // more images than the map holds: one lane per owner
@@ -533,25 +529,28 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_parti
                 if (e >= owners) continue;
                 const uint32_t* list = glist + B.image_start[slot0 + e];
                 uint32_t k = 0;
-                for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
-                    write_run_ghost<MODE>(p, sh, ol, static_cast<uint32_t>(__builtin_ctz(rest)), ghost0 + list[k]);
+                for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k) {
+                    const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
+                    write_run_ghost<MODE>(p, sh, ol, s, sh.table.ghost0[s] + list[k]);
+                }
             }
         }
         __syncthreads();
-    }
+    });
 }
 
-// One self-loopback box whose owned records exceed the memory-side cache:
-// its ghost writes go through copy_runs_kernel.
-inline bool ghost_runs(const Layout& L, const PlanBuffers& B)
-{
-    return L.replicas == 1 && B.recv == nullptr && L.streaming;
-}
+// By default the ghost writes go through copy_runs_kernel once the owned
+// records exceed the memory-side cache (below that size the per-owner
+// kernels measured faster: the arrays may be cache hits); DSDX_GHOST_RUNS
+// forces either kernel.
+inline bool ghost_runs(const Layout& L) { return L.ghost_runs != 0; }
 
 inline dim3 run_grid(const Layout& L)
 {
-    const uint32_t tiles = L.tile_waves ? L.tile_waves : L.tiles;
-    return dim3(std::max<uint32_t>(1, (tiles + kRunTiles - 1) / kRunTiles));
+    const uint32_t units = L.replicas <= static_cast<uint32_t>(kWave)
+                               ? ((L.tile_waves ? L.tile_waves : L.tiles) + kRunTiles - 1) / kRunTiles
+                               : L.units;
+    return dim3(std::max<uint32_t>(1, units));
 }
 
 // reverse_halo: f[i] = ((f[i] + g_s1) + g_s2) + ... over the owner's send
@@ -935,9 +934,12 @@ hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Lay
     const Layout O = ordered(L, s, kOrderCopy);
     if (L.tiles == 0) return hipSuccess;
     // Each workgroup walks units g, g + grid, ... up to the device's owner count.
-    if (ghost_runs(L, B))
-        return launch(copy_runs_kernel<kRunsBuild, true>, run_grid(L), dim3(kRunWaves * kWave), stream, p, fp,
-                      L.desc, O, B);
+    if (ghost_runs(L))
+        return dispatch_stream(L.streaming, [&](auto streaming) {
+            constexpr bool S = decltype(streaming)::value;
+            return launch(copy_runs_kernel<kRunsBuild, S>, run_grid(L), dim3(kRunWaves * kWave), stream, p, fp,
+                          L.desc, L.unit_replica, O, B);
+        });
     return dispatch_waves(s, [&](auto waves) {
         return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr int W = decltype(waves)::value;
@@ -989,13 +991,15 @@ hipError_t launch_refresh(const dsdx_particles& p, const FieldParams& fp, int32_
 {
     const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
-    if (ghost_runs(L, B)) {
-        if (positions_only)
-            return launch(copy_runs_kernel<kRunsPositions, true>, run_grid(L), dim3(kRunWaves * kWave), stream, p,
-                          fp, L.desc, O, B);
-        return launch(copy_runs_kernel<kRunsRefresh, true>, run_grid(L), dim3(kRunWaves * kWave), stream, p, fp,
-                      L.desc, O, B);
-    }
+    if (ghost_runs(L))
+        return dispatch_stream(L.streaming, [&](auto streaming) {
+            constexpr bool S = decltype(streaming)::value;
+            if (positions_only)
+                return launch(copy_runs_kernel<kRunsPositions, S>, run_grid(L), dim3(kRunWaves * kWave), stream,
+                              p, fp, L.desc, L.unit_replica, O, B);
+            return launch(copy_runs_kernel<kRunsRefresh, S>, run_grid(L), dim3(kRunWaves * kWave), stream, p, fp,
+                          L.desc, L.unit_replica, O, B);
+        });
     return dispatch_waves(s, [&](auto waves) {
         return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr int W = decltype(waves)::value;

@@ -128,6 +128,8 @@ extern "C" int dsdx_halo_build_noalloc(const dsdx_box* box, const dsdx_particles
     L.chunk_size = static_cast<uint32_t>(kChunk);
     L.tiles = tiles;
     L.tile_waves = tiles;
+    L.units = (tiles + kRunTiles - 1) / kRunTiles;
+    L.ghost_runs = shape.ghost_runs > 0 ? 1u : 0u;
     L.xcd_run = static_cast<uint32_t>(shape.xcd_run);
     PlanBuffers B{};
     B.entries = reinterpret_cast<uint32_t*>(base + kOffEntries);

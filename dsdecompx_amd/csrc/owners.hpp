@@ -115,6 +115,55 @@ __device__ __forceinline__ void for_owner_tiles(const ReplicaDesc* __restrict__ 
     }
 }
 
+// Ghost-run units: kRunTiles consecutive owner tiles of one replica, one
+// workgroup each.  Calls body(d, e0, owners) for every unit this workgroup
+// walks (units g, g + grid, ...): d the replica, e0 its first owner
+// (relative to the replica's owner slots), owners the replica's owner count.
+// Every wave of the block computes the same units, so the calls are
+// block-uniform and the body may hold block barriers.  Up to 64 replicas the
+// units walked are exactly the live ones, counted from the device totals as
+// for_owner_tiles counts tiles; more replicas: the units numbered on the
+// host (ceil(tiles_r / kRunTiles) per replica), one past its replica's
+// owners skipped.  A replica whose build overflowed has none.
+template <class Body>
+__device__ __forceinline__ void for_run_units(const ReplicaDesc* __restrict__ desc,
+                                              const uint32_t* __restrict__ unit_replica,
+                                              const Layout& L, const PlanBuffers& B, Body&& body)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t first = xcd_block(L.xcd_run);
+    if (L.replicas <= kWave) {
+        uint32_t owners_r = 0;
+        if (lane < L.replicas) {
+            const uint32_t* t = B.totals + static_cast<uint64_t>(lane) * kTotals;
+            owners_r = t[kStatusWord] ? 0u : t[kColOwners];
+        }
+        const uint32_t units_r = (owners_r + kRunOwners - 1) / kRunOwners;
+        const uint32_t end_r = wave_inclusive_sum(units_r);
+        const uint32_t begin_r = end_r - units_r;
+        const uint32_t total = uniform(__shfl(end_r, kWave - 1, kWave));
+        for (uint32_t c = first; c < total; c += gridDim.x) {
+            const uint32_t unit = L.descending ? total - 1u - c : c;
+            const uint64_t hit = __ballot(begin_r <= unit && unit < end_r);   // one lane
+            const uint32_t r = static_cast<uint32_t>(__builtin_ctzll(hit));
+            const uint32_t e0 = (unit - __builtin_amdgcn_readlane(begin_r, r)) * kRunOwners;
+            body(desc + r, e0, static_cast<uint32_t>(__builtin_amdgcn_readlane(owners_r, r)));
+        }
+        return;
+    }
+    for (uint32_t c = first; c < L.units; c += gridDim.x) {
+        const uint32_t unit = L.descending ? L.units - 1u - c : c;
+        const uint32_t r = uniform(unit_replica[unit]);
+        const ReplicaDesc* d = desc + r;
+        const uint32_t* totals = B.totals + static_cast<uint64_t>(r) * kTotals;
+        if (uniform(totals[kStatusWord])) continue;
+        const uint32_t owners = uniform(totals[kColOwners]);
+        const uint32_t e0 = (unit - d->first_unit) * kRunOwners;
+        if (e0 >= owners) continue;
+        body(d, e0, owners);
+    }
+}
+
 // The ghosts of one owner: its slot set and its ghost-list entries (one per
 // set bit, ascending slot).  at(k, s) is the particle slot of the k-th ghost,
 // sent through slot s.

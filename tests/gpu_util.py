@@ -5,6 +5,7 @@ from __future__ import annotations
 import sys
 
 import numpy as np
+import pytest
 
 from oracle import pyoracle as po
 
@@ -84,3 +85,13 @@ def assert_bits_equal(a, b, what):
 def compare_particles(dev, ora, count, what):
     for k in F64 + I32:
         assert_bits_equal(host(getattr(dev, k)[:count]), getattr(ora, k)[:count], f"{what}.{k}")
+
+
+@pytest.fixture(autouse=True, params=["per_owner", "runs"])
+def ghost_writes(request, monkeypatch):
+    """Runs a test module's tests twice: the ghost copy and refresh writing
+    one lane per owner (DSDX_GHOST_RUNS=0) and staging each unit's slot runs
+    through LDS (DSDX_GHOST_RUNS=1), whatever the sizes would pick.  The
+    variable is read when a plan, batch or grid is created."""
+    monkeypatch.setenv("DSDX_GHOST_RUNS", "1" if request.param == "runs" else "0")
+    return request.param
