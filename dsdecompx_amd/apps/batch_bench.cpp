@@ -19,7 +19,7 @@
 //   batch   one launch per phase over every replica (dsdx_batch_*), eager
 //   graph   the batch step captured once with hipStreamBeginCapture and
 //           replayed with hipGraphLaunch
-// usage: batch_bench replicas steps warmup seed [bench|check]
+// usage: batch_bench replicas steps warmup seed [bench|check|graph]
 //   bench: the batch's launch shape tuned first (items per lane, waves per
 //          block, scan threads: the fastest eager fused step), then
 //          replica-steps per second of every form, one JSON line.
@@ -31,6 +31,10 @@
 //          inside would fail the capture), and `steps` replays against the
 //          same steps run eagerly on a second plan.  Exit status 1 on any
 //          mismatch.
+//   graph: the captured batch step only, unfused then fused, at the launch
+//          shape the environment gives (DSDX_ITEMS, DSDX_WAVES,
+//          DSDX_SCAN_THREADS; no tuning), so that a kernel trace of the run
+//          holds the batch's kernels alone.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -400,7 +404,7 @@ uint64_t single_box_capture_check(int steps, uint64_t seed, hipStream_t stream)
 int main(int argc, char** argv)
 {
     if (argc < 5) {
-        std::fprintf(stderr, "usage: %s replicas steps warmup seed [bench|check]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s replicas steps warmup seed [bench|check|graph]\n", argv[0]);
         return 2;
     }
     const int count = std::atoi(argv[1]);
@@ -408,7 +412,7 @@ int main(int argc, char** argv)
     const int warmup = std::atoi(argv[3]);
     const uint64_t seed = std::strtoull(argv[4], nullptr, 10);
     const std::string mode = argc > 5 ? argv[5] : "bench";
-    if (count < 1 || steps < 1 || (mode != "bench" && mode != "check")) {
+    if (count < 1 || steps < 1 || (mode != "bench" && mode != "check" && mode != "graph")) {
         std::fprintf(stderr, "bad arguments\n");
         return 2;
     }
@@ -458,6 +462,29 @@ int main(int argc, char** argv)
                     static_cast<unsigned long long>(single), all ? "true" : "false");
         ok(hipStreamDestroy(stream), "stream");
         return all ? 0 : 1;
+    }
+
+    if (mode == "graph") {
+        int shape[3] = {0, 0, 0};
+        dsdx(dsdx_batch_get_shape(batch.b, &shape[0], &shape[1], &shape[2]), "shape");
+        double ms[2] = {0.0, 0.0};
+        for (int fused = 0; fused < 2; ++fused) {
+            w.reset(s, stream);
+            ok(hipStreamSynchronize(stream), "sync");
+            hipGraphExec_t exec = capture(stream, hipStreamCaptureModeGlobal,
+                                          [&] { batch.step(w, s, fused != 0, stream); });
+            for (int k = 0; k < warmup; ++k) ok(hipGraphLaunch(exec, stream), "graph launch");
+            ms[fused] = 1e3 * wall_seconds(stream, [&] {
+                for (int k = 0; k < steps; ++k) ok(hipGraphLaunch(exec, stream), "graph launch");
+            }) / steps;
+            ok(hipGraphExecDestroy(exec), "exec destroy");
+        }
+        std::printf("{\"replicas\": %d, \"particles\": %llu, \"steps\": %d, \"batch_shape\": [%d, %d, %d], "
+                    "\"ms_per_step\": {\"graph\": %.4f, \"graph_fused\": %.4f}}\n",
+                    count, static_cast<unsigned long long>(R.particles), steps, shape[0], shape[1], shape[2], ms[0],
+                    ms[1]);
+        ok(hipStreamDestroy(stream), "stream");
+        return 0;
     }
 
     // Launch shape of the batch: the fastest eager step over items per lane
