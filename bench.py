@@ -123,6 +123,41 @@ def cpu_baseline(name, n, L, w, sigma, periodic, steps, seed=7):
                       f"oracle (C, -O2, one thread) on {os.cpu_count()} visible host cores"}
 
 
+def cpu_reference(n, L, w, sigma, steps=1):
+    """The reference's own Host path (its unmodified sources built by
+    oracle/ref.mk into oracle/_ref/) on 8 MPI ranks: a 2x2x2 periodic grid of
+    the same particle count and box, the reference's communication_test step
+    (migrate, exchange_halo(rebuild), forces, reverse_halo), one warm-up and
+    `steps` timed steps, the slowest rank's busy time.  The reference cannot
+    run the single-box loopback itself (on one rank it has no ghosts), so its
+    figure is for its own decomposition.  None when the build or MPICH is not
+    here."""
+    import json as _json
+    import shutil
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_grid_fixture")
+    mpiexec = "/opt/conda/bin/mpiexec" if os.path.exists("/opt/conda/bin/mpiexec") else shutil.which("mpiexec")
+    if not os.path.exists(exe) or not mpiexec:
+        return None
+    env = dict(os.environ)
+    env["LD_LIBRARY_PATH"] = "/usr/lib/x86_64-linux-gnu:/opt/conda/lib:" + env.get("LD_LIBRARY_PATH", "")
+    try:
+        r = subprocess.run([mpiexec, "-n", "8", exe, "time", str(n), repr(L), repr(w), repr(sigma), str(steps)],
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, env=env, cwd="/tmp",
+                           timeout=240)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out after 240 s"}
+    if r.returncode != 0:
+        return {"error": f"exit {r.returncode}: {r.stderr.strip()[-300:]}"}
+    res = _json.loads(r.stdout.strip().splitlines()[-1])
+    ms = res["ms_per_step"]
+    return {"value": n / (ms * 1e3), "unit": "Mparticles/s", "ms_per_step": ms, "cores": 8,
+            "kind": "reference", "cpu_model": _cpu_model(),
+            "sample": f"the reference's Host library (unmodified, oracle/ref.mk) on 8 MPI ranks, a 2x2x2 "
+                      f"periodic grid of {n} particles in L={L}, w={w}, N(0,{sigma}) per step; "
+                      f"{res['steps']} timed step(s) after 1 warm-up, slowest rank"}
+
+
 def cpu_all_cores(name, n, L, w, sigma, steps, seed=11):
     """The oracle on every host core at once: P independent periodic
     sub-boxes of n / P particles each at the workload's density (the work of a
@@ -892,6 +927,8 @@ def run(args):
         out["wire"] = measure_wire(args, n, L, w)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(name, n, L, w, sigma, periodic,
                                                                  args.cpu_steps)
+    if out["cpu_baseline"] and all(periodic):
+        out["cpu_baseline"]["reference_8_ranks"] = cpu_reference(n, L, w, sigma)
     if not args.no_cpu and all(periodic):
         out["cpu_all_cores"] = cpu_all_cores(name, n, L, w, sigma, args.cpu_steps)
     if not args.no_batched:
