@@ -11,6 +11,7 @@
 #include "device.hpp"
 #include "kernels.hpp"
 #include "owners.hpp"
+#include "runs.hpp"
 #include "wave.hpp"
 
 namespace dsdx {
@@ -346,48 +347,17 @@ __global__ __launch_bounds__(WAVES * kWave) void copy_kernel(
     });
 }
 
-// The ghost writes of large boxes, staged through LDS.  A workgroup takes a
-// unit of kRunTiles owner tiles of one replica (kRunOwners consecutive
-// boundary owners).  The unit's ghosts through send slot s are consecutive
-// ghosts of one receive block (their ranks among the slot's owners are
-// consecutive), so the unit writes one run per slot:
-//   1. one lane per owner gathers the owner's record into LDS and counts the
-//      unit's images per tile and slot; the first owner of each slot reads
-//      where the slot's run starts (its ghost-list entry);
-//   2. the image counts become, for every image of the unit, its index in
-//      the unit's runs laid end to end, and map[index] = its owner;
-//   3. lane k of a round writes run index k, so a store instruction covers
-//      whole cache lines of at most a few runs.
-// The per-owner kernels write each tile's short runs lane by lane: with 13
-// arrays and 26 receive blocks open at once, the partly written lines leave
-// the L2 before their neighbours arrive.  A unit whose images exceed the map
-// is written by one lane per owner from the same LDS records.  Registered
-// fields are copied per owner in step 1.  The receive block of slot s is the
-// replica's own (a loopback box or batch) or the neighbouring subdomain's (a
-// process grid), through the slot table.
-//
-// MODE kRunsBuild: the rebuild's copy, positions from the classify pass
-// (bpos).  kRunsRefresh: the reused-plan refresh of the whole record, the
+// The ghost copy and the reused-plan refresh through the staged runs of
+// runs.hpp.  MODE kRunsBuild: the rebuild's copy, positions from the
+// classify pass (bpos).  kRunsRefresh: the refresh of the whole record, the
 // owner's current position.  kRunsPositions: the positions-only refresh
-// (x, y, z and zeroed forces).
-constexpr uint32_t kRunMap = 2 * kRunOwners;
+// (x, y, z and zeroed forces).  The receive block of slot s is the
+// replica's own (a loopback box or batch) or the neighbouring subdomain's (a
+// process grid), through the slot table.  Registered fields are copied per
+// owner while the records are staged.
 constexpr int kRunsBuild = 0;
 constexpr int kRunsRefresh = 1;
 constexpr int kRunsPositions = 2;
-
-struct RunStage {
-    double px[kRunOwners], py[kRunOwners], pz[kRunOwners];
-    double vx[kRunOwners], vy[kRunOwners], vz[kRunOwners], m[kRunOwners], q[kRunOwners];
-    int32_t ty[kRunOwners], id[kRunOwners];
-    uint16_t map[kRunMap];      // run index -> owner (index in the unit)
-    uint8_t slot_of[kRunMap];   // run index -> send slot
-    uint8_t code[kRunOwners];   // slab code of each owner (0: no owner)
-    uint32_t tile_rank[kRunTiles][kSlots];   // images of slot s in the unit's earlier tiles
-    uint32_t first[kSlots];     // ghost-list entry of the unit's first image through slot s
-    uint32_t run_start[kSlots]; // first run index of slot s
-    uint32_t images;
-    SlotTable table;
-};
 
 template <int MODE>
 __device__ __forceinline__ void write_run_ghost(const dsdx_particles& p, const RunStage& sh, uint32_t ol,
@@ -413,144 +383,17 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_parti
                                                                       Layout L, PlanBuffers B)
 {
     __shared__ RunStage sh;
-    const uint32_t lane = lane_id();
-    const uint32_t wave = uniform(threadIdx.x / kWave);
+    const RunSource src{MODE == kRunsBuild, MODE != kRunsPositions};
     for_run_units(desc, unit_replica, L, B, [&](const ReplicaDesc* d, uint32_t e0, uint32_t owners) {
-        const uint32_t lut_l = d->bp.lut[lane];
-        const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * L.chunk_size;
-        const uint32_t* glist = B.ghost_list + d->ghost_list_offset;
-        if (wave == 0) fill_slot_table(sh.table, desc, d);
-        for (uint32_t k = threadIdx.x; k < kRunTiles * kSlots; k += kRunWaves * kWave)
-            (&sh.tile_rank[0][0])[k] = 0u;
-        if (threadIdx.x < kSlots) sh.first[threadIdx.x] = 0xFFFFFFFFu;
-        __syncthreads();
-        // 1. records into LDS; per tile and slot the image count
-        for (uint32_t t = wave; t < kRunTiles; t += kRunWaves) {
-            const uint32_t ol = t * kWave + lane;
-            const uint32_t e = e0 + ol;
-            const bool live = e < owners;
-            const uint64_t u = slot0 + e;
-            uint32_t oi = 0, code = 0, start = 0;
-            if (live) {
-                oi = B.owner_index[u];
-                code = B.owner_code[u];
-                start = B.image_start[u];
-            }
-            const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(code), kWave));
-            if (live) {
-                const uint64_t i = d->offset + oi;
-                if (MODE == kRunsBuild) {
-                    const uint32_t E = B.owner_entry[u];
-                    sh.px[ol] = B.bpos[E];
-                    sh.py[ol] = B.bpos[B.bpos_stride + E];
-                    sh.pz[ol] = B.bpos[2 * B.bpos_stride + E];
-                } else {
-                    sh.px[ol] = read_once<STREAM>(p.x + i);
-                    sh.py[ol] = read_once<STREAM>(p.y + i);
-                    sh.pz[ol] = read_once<STREAM>(p.z + i);
-                }
-                if (MODE != kRunsPositions) {
-                    sh.vx[ol] = read_once<STREAM>(p.vx + i);
-                    sh.vy[ol] = read_once<STREAM>(p.vy + i);
-                    sh.vz[ol] = read_once<STREAM>(p.vz + i);
-                    sh.m[ol] = read_once<STREAM>(p.mass + i);
-                    sh.q[ol] = read_once<STREAM>(p.charge + i);
-                    sh.ty[ol] = read_once<STREAM>(p.type + i);
-                    sh.id[ol] = read_once<STREAM>(p.global_id + i);
-                    if (fp.count) {
-                        uint32_t k = 0;
-                        for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
-                            copy_field_words(fp, i, ghost_slot(desc, d, static_cast<uint32_t>(__builtin_ctz(rest)),
-                                                               glist[start + k]));
-                    }
-                }
-            }
-            sh.code[ol] = static_cast<uint8_t>(code);
-            uint32_t present = uniform(wave_or(slots));
-            uint32_t leads = 0;   // slots whose first image in this tile is this lane's
-            while (present) {
-                const uint32_t s = static_cast<uint32_t>(__builtin_ctz(present));
-                present &= present - 1u;
-                const uint64_t b = __ballot((slots >> s) & 1u);
-                if (lane == static_cast<uint32_t>(__builtin_ctzll(b))) leads |= 1u << s;
-                if (lane == 0) sh.tile_rank[t][s] = static_cast<uint32_t>(__popcll(b));
-            }
-            for (; leads; leads &= leads - 1u) {
-                const uint32_t s = static_cast<uint32_t>(__builtin_ctz(leads));
-                const uint32_t k = static_cast<uint32_t>(__popc(slots & ((1u << s) - 1u)));
-                atomicMin(&sh.first[s], glist[start + k]);
-            }
-        }
-        __syncthreads();
-        // 2a. lane s of wave 0: its tile counts -> exclusive ranks, the slot's run start
-        if (wave == 0) {
-            uint32_t n = 0;
-            if (lane < kSlots)
-                for (uint32_t t = 0; t < kRunTiles; ++t) {
-                    const uint32_t v = sh.tile_rank[t][lane];
-                    sh.tile_rank[t][lane] = n;
-                    n += v;
-                }
-            const uint32_t end = wave_inclusive_sum(n);
-            if (lane < kSlots) sh.run_start[lane] = end - n;
-            if (lane == kSlots - 1) sh.images = end;
-        }
-        __syncthreads();
-        const uint32_t images = sh.images;
-        if (images <= kRunMap) {
-            // 2b. run index of every image -> its owner and slot
-            for (uint32_t t = wave; t < kRunTiles; t += kRunWaves) {
-                const uint32_t ol = t * kWave + lane;
-                const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(sh.code[ol]), kWave));
-                uint32_t present = uniform(wave_or(slots));
-                while (present) {
-                    const uint32_t s = static_cast<uint32_t>(__builtin_ctz(present));
-                    present &= present - 1u;
-                    const uint64_t b = __ballot((slots >> s) & 1u);
-                    if ((slots >> s) & 1u) {
-                        const uint32_t r = sh.run_start[s] + sh.tile_rank[t][s] + lanes_below(b);
-                        sh.map[r] = static_cast<uint16_t>(ol);
-                        sh.slot_of[r] = static_cast<uint8_t>(s);
-                    }
-                }
-            }
-            __syncthreads();
-            // 3. lane k of a round writes run index k
-            for (uint32_t r = wave * kWave + lane; r < images; r += kRunWaves * kWave) {
-                const uint32_t s = sh.slot_of[r];
-                write_run_ghost<MODE>(p, sh, sh.map[r], s, sh.table.ghost0[s] + sh.first[s] + (r - sh.run_start[s]));
-            }
-        } else {
-            // more images than the map holds: one lane per owner
-            for (uint32_t t = wave; t < kRunTiles; t += kRunWaves) {
-                const uint32_t ol = t * kWave + lane;
-                const uint32_t e = e0 + ol;
-                const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(sh.code[ol]), kWave));
-                if (e >= owners) continue;
-                const uint32_t* list = glist + B.image_start[slot0 + e];
-                uint32_t k = 0;
-                for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k) {
-                    const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
-                    write_run_ghost<MODE>(p, sh, ol, s, sh.table.ghost0[s] + list[k]);
-                }
-            }
-        }
-        __syncthreads();
+        ghost_runs_unit<STREAM>(
+            sh, p, desc, d, e0, owners, L, B, src,
+            [&](uint64_t i, uint32_t s, uint32_t rel) {
+                if (fp.count) copy_field_words(fp, i, ghost_slot(desc, d, s, rel));
+            },
+            [&](uint32_t ol, uint32_t s, uint32_t rel) {
+                write_run_ghost<MODE>(p, sh, ol, s, sh.table.ghost0[s] + rel);
+            });
     });
-}
-
-// By default the ghost writes go through copy_runs_kernel once the owned
-// records exceed the memory-side cache (below that size the per-owner
-// kernels measured faster: the arrays may be cache hits); DSDX_GHOST_RUNS
-// forces either kernel.
-inline bool ghost_runs(const Layout& L) { return L.ghost_runs != 0; }
-
-inline dim3 run_grid(const Layout& L)
-{
-    const uint32_t units = L.replicas <= static_cast<uint32_t>(kWave)
-                               ? ((L.tile_waves ? L.tile_waves : L.tiles) + kRunTiles - 1) / kRunTiles
-                               : L.units;
-    return dim3(std::max<uint32_t>(1, units));
 }
 
 // reverse_halo: f[i] = ((f[i] + g_s1) + g_s2) + ... over the owner's send

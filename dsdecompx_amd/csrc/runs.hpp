@@ -1,0 +1,194 @@
+// runs.hpp -- ghost runs staged through LDS, shared by the ghost copy
+// (halo_kernels.hip) and the record pack (wire_kernels.hip).
+//
+// A workgroup of kRunWaves waves takes a unit of kRunTiles owner tiles of one
+// replica (kRunOwners consecutive boundary owners, for_run_units).  The
+// unit's ghosts through send slot s are consecutive ghosts of one receive
+// block (their ranks among the slot's owners are consecutive), so the unit
+// writes one run per slot:
+//   1. one lane per owner gathers the owner's record into LDS and counts the
+//      unit's images per tile and slot; the first owner of each slot reads
+//      where the slot's run starts (its ghost-list entry);
+//   2. the image counts become, for every image of the unit, its index in
+//      the unit's runs laid end to end, and map[index] = its owner;
+//   3. lane k of a round writes run index k, so a store instruction covers
+//      whole cache lines of at most a few runs.
+// The per-owner kernels write each tile's short runs lane by lane: with 13
+// arrays and 26 receive blocks open at once, the partly written lines leave
+// the L2 before their neighbours arrive.  A unit whose images exceed the map
+// is written by one lane per owner from the same LDS records.
+#pragma once
+
+#include "device.hpp"
+#include "owners.hpp"
+#include "wave.hpp"
+
+namespace dsdx::kern {
+
+constexpr uint32_t kRunMap = 2 * kRunOwners;
+
+struct RunStage {
+    double px[kRunOwners], py[kRunOwners], pz[kRunOwners];
+    double vx[kRunOwners], vy[kRunOwners], vz[kRunOwners], m[kRunOwners], q[kRunOwners];
+    int32_t ty[kRunOwners], id[kRunOwners];
+    uint16_t map[kRunMap];      // run index -> owner (index in the unit)
+    uint8_t slot_of[kRunMap];   // run index -> send slot
+    uint8_t code[kRunOwners];   // slab code of each owner (0: no owner)
+    uint32_t tile_rank[kRunTiles][kSlots];   // images of slot s in the unit's earlier tiles
+    uint32_t first[kSlots];     // ghost-list entry of the unit's first image through slot s
+    uint32_t run_start[kSlots]; // first run index of slot s
+    uint32_t images;
+    SlotTable table;
+};
+
+// What a unit stages per owner: its position from the classification pass
+// (bpos) or from the particle arrays, and the rest of its record or not.
+struct RunSource {
+    bool positions_from_entries;
+    bool records;
+};
+
+// One unit (e0: its first owner, owners: the replica's owner count), all
+// waves of the block.  fields(i, s, rel) runs on the owner's lane for each
+// image (i: the owner's particle slot, rel: the image's ghost-list entry),
+// write(ol, s, rel) once per image from the LDS records.
+template <bool STREAM, class Fields, class Write>
+__device__ __forceinline__ void ghost_runs_unit(RunStage& sh, const dsdx_particles& p,
+                                                const ReplicaDesc* __restrict__ desc, const ReplicaDesc* d,
+                                                uint32_t e0, uint32_t owners, const Layout& L,
+                                                const PlanBuffers& B, RunSource src, Fields&& fields,
+                                                Write&& write)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wave = uniform(threadIdx.x / kWave);
+    const uint32_t lut_l = d->bp.lut[lane];
+    const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * L.chunk_size;
+    const uint32_t* glist = B.ghost_list + d->ghost_list_offset;
+    if (wave == 0) fill_slot_table(sh.table, desc, d);
+    for (uint32_t k = threadIdx.x; k < kRunTiles * kSlots; k += kRunWaves * kWave)
+        (&sh.tile_rank[0][0])[k] = 0u;
+    if (threadIdx.x < kSlots) sh.first[threadIdx.x] = 0xFFFFFFFFu;
+    __syncthreads();
+    // 1. records into LDS; per tile and slot the image count
+    for (uint32_t t = wave; t < kRunTiles; t += kRunWaves) {
+        const uint32_t ol = t * kWave + lane;
+        const uint32_t e = e0 + ol;
+        const bool live = e < owners;
+        const uint64_t u = slot0 + e;
+        uint32_t oi = 0, code = 0, start = 0;
+        if (live) {
+            oi = B.owner_index[u];
+            code = B.owner_code[u];
+            start = B.image_start[u];
+        }
+        const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(code), kWave));
+        if (live) {
+            const uint64_t i = d->offset + oi;
+            if (src.positions_from_entries) {
+                const uint32_t E = B.owner_entry[u];
+                sh.px[ol] = B.bpos[E];
+                sh.py[ol] = B.bpos[B.bpos_stride + E];
+                sh.pz[ol] = B.bpos[2 * B.bpos_stride + E];
+            } else {
+                sh.px[ol] = read_once<STREAM>(p.x + i);
+                sh.py[ol] = read_once<STREAM>(p.y + i);
+                sh.pz[ol] = read_once<STREAM>(p.z + i);
+            }
+            if (src.records) {
+                sh.vx[ol] = read_once<STREAM>(p.vx + i);
+                sh.vy[ol] = read_once<STREAM>(p.vy + i);
+                sh.vz[ol] = read_once<STREAM>(p.vz + i);
+                sh.m[ol] = read_once<STREAM>(p.mass + i);
+                sh.q[ol] = read_once<STREAM>(p.charge + i);
+                sh.ty[ol] = read_once<STREAM>(p.type + i);
+                sh.id[ol] = read_once<STREAM>(p.global_id + i);
+                uint32_t k = 0;
+                for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
+                    fields(i, static_cast<uint32_t>(__builtin_ctz(rest)), glist[start + k]);
+            }
+        }
+        sh.code[ol] = static_cast<uint8_t>(code);
+        uint32_t present = uniform(wave_or(slots));
+        uint32_t leads = 0;   // slots whose first image in this tile is this lane's
+        while (present) {
+            const uint32_t s = static_cast<uint32_t>(__builtin_ctz(present));
+            present &= present - 1u;
+            const uint64_t b = __ballot((slots >> s) & 1u);
+            if (lane == static_cast<uint32_t>(__builtin_ctzll(b))) leads |= 1u << s;
+            if (lane == 0) sh.tile_rank[t][s] = static_cast<uint32_t>(__popcll(b));
+        }
+        for (; leads; leads &= leads - 1u) {
+            const uint32_t s = static_cast<uint32_t>(__builtin_ctz(leads));
+            const uint32_t k = static_cast<uint32_t>(__popc(slots & ((1u << s) - 1u)));
+            atomicMin(&sh.first[s], glist[start + k]);
+        }
+    }
+    __syncthreads();
+    // 2a. lane s of wave 0: its tile counts -> exclusive ranks, the slot's run start
+    if (wave == 0) {
+        uint32_t n = 0;
+        if (lane < kSlots)
+            for (uint32_t t = 0; t < kRunTiles; ++t) {
+                const uint32_t v = sh.tile_rank[t][lane];
+                sh.tile_rank[t][lane] = n;
+                n += v;
+            }
+        const uint32_t end = wave_inclusive_sum(n);
+        if (lane < kSlots) sh.run_start[lane] = end - n;
+        if (lane == kSlots - 1) sh.images = end;
+    }
+    __syncthreads();
+    const uint32_t images = sh.images;
+    if (images <= kRunMap) {
+        // 2b. run index of every image -> its owner and slot
+        for (uint32_t t = wave; t < kRunTiles; t += kRunWaves) {
+            const uint32_t ol = t * kWave + lane;
+            const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(sh.code[ol]), kWave));
+            uint32_t present = uniform(wave_or(slots));
+            while (present) {
+                const uint32_t s = static_cast<uint32_t>(__builtin_ctz(present));
+                present &= present - 1u;
+                const uint64_t b = __ballot((slots >> s) & 1u);
+                if ((slots >> s) & 1u) {
+                    const uint32_t r = sh.run_start[s] + sh.tile_rank[t][s] + lanes_below(b);
+                    sh.map[r] = static_cast<uint16_t>(ol);
+                    sh.slot_of[r] = static_cast<uint8_t>(s);
+                }
+            }
+        }
+        __syncthreads();
+        // 3. lane k of a round writes run index k
+        for (uint32_t r = wave * kWave + lane; r < images; r += kRunWaves * kWave) {
+            const uint32_t s = sh.slot_of[r];
+            write(static_cast<uint32_t>(sh.map[r]), s, sh.first[s] + (r - sh.run_start[s]));
+        }
+    } else {
+        // more images than the map holds: one lane per owner
+        for (uint32_t t = wave; t < kRunTiles; t += kRunWaves) {
+            const uint32_t ol = t * kWave + lane;
+            const uint32_t e = e0 + ol;
+            const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(sh.code[ol]), kWave));
+            if (e >= owners) continue;
+            const uint32_t* list = glist + B.image_start[slot0 + e];
+            uint32_t k = 0;
+            for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
+                write(ol, static_cast<uint32_t>(__builtin_ctz(rest)), list[k]);
+        }
+    }
+    __syncthreads();
+}
+
+// Owned records larger than the memory-side cache take the runs by default
+// (below that size the per-owner kernels measured faster: the arrays may be
+// cache hits); DSDX_GHOST_RUNS forces either kernel.
+inline bool ghost_runs(const Layout& L) { return L.ghost_runs != 0; }
+
+inline dim3 run_grid(const Layout& L)
+{
+    const uint32_t units = L.replicas <= static_cast<uint32_t>(kWave)
+                               ? ((L.tile_waves ? L.tile_waves : L.tiles) + kRunTiles - 1) / kRunTiles
+                               : L.units;
+    return dim3(std::max<uint32_t>(1, units));
+}
+
+}  // namespace dsdx::kern
