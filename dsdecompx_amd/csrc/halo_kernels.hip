@@ -137,9 +137,27 @@ __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Pass 2: exclusive scan of one count column over one replica's chunks; grid
-// (kCols, replicas).  totals[r][col] receives the column total.
-// One block per (count column, replica).
+// Pass 2: exclusive scan of one count column over one replica's chunks.
+// totals[r][col] receives the column total (and the scan clears the status
+// word, sets a loopback replica's received count, copies both to the mapped
+// host totals of a single box, and clears a process grid's status).
+__device__ __forceinline__ void column_done(const PlanBuffers& B, const Layout& L, uint32_t col, uint32_t r,
+                                            uint32_t carry)
+{
+    uint32_t* tot = B.totals + static_cast<uint64_t>(r) * kTotals;
+    tot[col] = carry;
+    if (col == 0) tot[kStatusWord] = 0u;
+    // a replica that sends to itself receives what it sends; a process
+    // grid's exchange pass overwrites this with the neighbours' sends
+    if (col == kColImages) tot[kColReceived] = carry;
+    if (B.host_totals) {
+        B.host_totals[static_cast<uint64_t>(r) * kTotals + col] = carry;
+        if (col == kColImages) B.host_totals[static_cast<uint64_t>(r) * kTotals + kColReceived] = carry;
+    }
+    if (B.recv && col == 0 && r == 0) B.recv[static_cast<uint64_t>(L.replicas) * kTotals] = 0u;
+}
+
+// One block per (count column, replica): grid (kCols, replicas).
 __global__ __launch_bounds__(kMaxScanThreads) void scan_kernel(const ReplicaDesc* __restrict__ desc,
                                                                Layout L, PlanBuffers B)
 {
@@ -150,19 +168,50 @@ __global__ __launch_bounds__(kMaxScanThreads) void scan_kernel(const ReplicaDesc
     const ReplicaDesc* d = desc + r;
     const uint64_t base = static_cast<uint64_t>(col) * L.chunks + d->first_chunk;
     const uint32_t carry = scan_column(B.counts + base, B.prefix + base, d->chunks, s_tile, s_wave);
-    if (threadIdx.x == 0) {
-        uint32_t* tot = B.totals + static_cast<uint64_t>(r) * kTotals;
-        tot[col] = carry;
-        if (col == 0) tot[kStatusWord] = 0u;
-        // a replica that sends to itself receives what it sends; a process
-        // grid's exchange pass overwrites this with the neighbours' sends
-        if (col == kColImages) tot[kColReceived] = carry;
-        if (B.host_totals) {
-            B.host_totals[static_cast<uint64_t>(r) * kTotals + col] = carry;
-            if (col == kColImages) B.host_totals[static_cast<uint64_t>(r) * kTotals + kColReceived] = carry;
+    if (threadIdx.x == 0) column_done(B, L, col, r, carry);
+}
+
+// Short columns (every replica at most kShortScanChunks chunks, as in a batch
+// of small boxes): one wave per (column, replica), kShortScanPer consecutive
+// values per lane and a wave scan, no block barrier; a block holds
+// kShortScanWaves such columns.
+constexpr uint32_t kShortScanPer = 8;
+constexpr uint32_t kShortScanWaves = 8;
+constexpr uint32_t kShortScanChunks = 2 * kWave * kShortScanPer;
+
+__global__ __launch_bounds__(kShortScanWaves * kWave) void scan_short_kernel(const ReplicaDesc* __restrict__ desc,
+                                                                            Layout L, PlanBuffers B)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t pair = blockIdx.x * kShortScanWaves + uniform(threadIdx.x / kWave);
+    if (pair >= static_cast<uint32_t>(kCols) * L.replicas) return;
+    const uint32_t col = pair % kCols;
+    const uint32_t r = pair / kCols;
+    const ReplicaDesc* d = desc + r;
+    const uint64_t base = static_cast<uint64_t>(col) * L.chunks + d->first_chunk;
+    const uint32_t* src = B.counts + base;
+    uint32_t* dst = B.prefix + base;
+    const uint32_t n = d->chunks;
+    uint32_t carry = 0;
+    for (uint32_t k0 = 0; k0 < n; k0 += kWave * kShortScanPer) {
+        const uint32_t at = k0 + lane * kShortScanPer;
+        uint32_t v[kShortScanPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kShortScanPer; ++j) {
+            v[j] = at + j < n ? src[at + j] : 0u;
+            sum += v[j];
         }
-        if (B.recv && col == 0 && r == 0) B.recv[static_cast<uint64_t>(L.replicas) * kTotals] = 0u;
+        const uint32_t incl = wave_inclusive_sum(sum);
+        uint32_t run = carry + incl - sum;
+#pragma unroll
+        for (uint32_t j = 0; j < kShortScanPer; ++j) {
+            if (at + j < n) dst[at + j] = run;
+            run += v[j];
+        }
+        carry += uniform(__shfl(incl, kWave - 1, kWave));
     }
+    if (lane == 0) column_done(B, L, col, r, carry);
 }
 
 // ---------------------------------------------------------------------------
@@ -757,6 +806,11 @@ hipError_t launch_classify(double* x, double* y, double* z, bool wrap_first, con
 hipError_t launch_scan(const Layout& L, const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
     if (L.replicas == 0) return hipSuccess;
+    if (L.max_replica_chunks <= kShortScanChunks) {
+        const uint32_t pairs = static_cast<uint32_t>(kCols) * L.replicas;
+        return launch(scan_short_kernel, dim3((pairs + kShortScanWaves - 1) / kShortScanWaves),
+                      dim3(kShortScanWaves * kWave), stream, L.desc, L, B);
+    }
     return launch(scan_kernel, dim3(kCols, L.replicas), dim3(s.scan_threads), stream, L.desc, L, B);
 }
 
