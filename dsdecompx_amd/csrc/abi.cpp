@@ -455,6 +455,7 @@ int shape_from_env(Shape* s)
     if (const char* v = std::getenv("DSDX_XCD_RUN")) s->xcd_run = std::atoi(v);
     if (const char* v = std::getenv("DSDX_DESCENDING")) s->descending = std::atoi(v);
     if (const char* v = std::getenv("DSDX_GHOST_RUNS")) s->ghost_runs = std::atoi(v);
+    if (const char* v = std::getenv("DSDX_REVERSE_RUNS")) s->reverse_runs = std::atoi(v);
     return shape_supported(*s) ? DSDX_OK : fail(DSDX_EINVAL, "unsupported launch shape");
 }
 

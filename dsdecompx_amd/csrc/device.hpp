@@ -114,6 +114,9 @@ struct Shape {
     // Ghost writes staged through LDS (copy_runs_kernel): -1 when the owned
     // records exceed the memory-side cache, 0 never, 1 always.
     int ghost_runs = -1;
+    // reverse_halo with the ghost forces read as dense runs (1), per owner
+    // (0), or as the ghost writes go (-1: runs wherever the copy stages them).
+    int reverse_runs = -1;
 };
 
 // Owner tiles: the owner slots [0, bound_r) of replica r (relative to

@@ -178,6 +178,128 @@ __device__ __forceinline__ void ghost_runs_unit(RunStage& sh, const dsdx_particl
     __syncthreads();
 }
 
+// reverse_halo over a unit with the ghost forces read as dense runs: one
+// wave per owner tile holds its owners' forces; the unit's ghost forces come
+// into LDS in run order, then each owner adds its images in ascending send
+// slot (the per-owner kernel's order) from there.
+static_assert(kRunTiles == kRunWaves, "one owner tile per wave");
+
+struct ReverseStage {
+    double gx[kRunMap], gy[kRunMap], gz[kRunMap];
+    uint32_t tile_rank[kRunTiles][kSlots];
+    uint32_t first[kSlots];
+    uint32_t run_start[kSlots + 1];
+    uint32_t images;
+    SlotTable table;
+};
+
+__device__ __forceinline__ void reverse_runs_unit(ReverseStage& sh, double* __restrict__ fx,
+                                                  double* __restrict__ fy, double* __restrict__ fz,
+                                                  const ReplicaDesc* __restrict__ desc, const ReplicaDesc* d,
+                                                  uint32_t e0, uint32_t owners, const Layout& L,
+                                                  const PlanBuffers& B)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t t = uniform(threadIdx.x / kWave);
+    const uint32_t lut_l = d->bp.lut[lane];
+    const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * L.chunk_size;
+    const uint32_t* glist = B.ghost_list + d->ghost_list_offset;
+    if (t == 0) fill_slot_table(sh.table, desc, d);
+    for (uint32_t k = threadIdx.x; k < kRunTiles * kSlots; k += kRunWaves * kWave) (&sh.tile_rank[0][0])[k] = 0u;
+    if (threadIdx.x < kSlots) sh.first[threadIdx.x] = 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t e = e0 + t * kWave + lane;
+    const bool live = e < owners;
+    const uint64_t u = slot0 + e;
+    uint32_t oi = 0, code = 0, start = 0;
+    if (live) {
+        oi = B.owner_index[u];
+        code = B.owner_code[u];
+        start = B.image_start[u];
+    }
+    const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(code), kWave));
+    const uint64_t i = d->offset + oi;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (live) {
+        ax = fx[i];
+        ay = fy[i];
+        az = fz[i];
+    }
+    uint32_t present = uniform(wave_or(slots));
+    uint32_t leads = 0;
+    while (present) {
+        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(present));
+        present &= present - 1u;
+        const uint64_t b = __ballot((slots >> s) & 1u);
+        if (lane == static_cast<uint32_t>(__builtin_ctzll(b))) leads |= 1u << s;
+        if (lane == 0) sh.tile_rank[t][s] = static_cast<uint32_t>(__popcll(b));
+    }
+    for (; leads; leads &= leads - 1u) {
+        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(leads));
+        const uint32_t k = static_cast<uint32_t>(__popc(slots & ((1u << s) - 1u)));
+        atomicMin(&sh.first[s], glist[start + k]);
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t n = 0;
+        if (lane < kSlots)
+            for (uint32_t q = 0; q < kRunTiles; ++q) {
+                const uint32_t v = sh.tile_rank[q][lane];
+                sh.tile_rank[q][lane] = n;
+                n += v;
+            }
+        const uint32_t end = wave_inclusive_sum(n);
+        if (lane < kSlots) sh.run_start[lane] = end - n;
+        if (lane == kSlots - 1) {
+            sh.run_start[kSlots] = end;
+            sh.images = end;
+        }
+    }
+    __syncthreads();
+    const uint32_t images = sh.images;
+    if (images <= kRunMap) {
+        for (uint32_t r = threadIdx.x; r < images; r += kRunWaves * kWave) {
+            uint32_t lo = 0, hi = kSlots;   // the last slot whose run starts at or before r
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (sh.run_start[mid] <= r) lo = mid;
+                else hi = mid;
+            }
+            const uint64_t j = sh.table.ghost0[lo] + sh.first[lo] + (r - sh.run_start[lo]);
+            sh.gx[r] = fx[j];
+            sh.gy[r] = fy[j];
+            sh.gz[r] = fz[j];
+        }
+        __syncthreads();
+        uint32_t rest = uniform(wave_or(slots));
+        while (rest) {
+            const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
+            rest &= rest - 1u;
+            const uint64_t b = __ballot((slots >> s) & 1u);
+            if ((slots >> s) & 1u) {
+                const uint32_t r = sh.run_start[s] + sh.tile_rank[t][s] + lanes_below(b);
+                ax = ax + sh.gx[r];
+                ay = ay + sh.gy[r];
+                az = az + sh.gz[r];
+            }
+        }
+    } else if (live) {
+        uint32_t k = 0;
+        for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k) {
+            const uint64_t j = sh.table.ghost0[__builtin_ctz(rest)] + glist[start + k];
+            ax = ax + fx[j];
+            ay = ay + fy[j];
+            az = az + fz[j];
+        }
+    }
+    if (live) {
+        fx[i] = ax;
+        fy[i] = ay;
+        fz[i] = az;
+    }
+    __syncthreads();
+}
+
 // Owned records larger than the memory-side cache take the runs by default
 // (below that size the per-owner kernels measured faster: the arrays may be
 // cache hits); DSDX_GHOST_RUNS forces either kernel.

@@ -89,9 +89,10 @@ def compare_particles(dev, ora, count, what):
 
 @pytest.fixture(autouse=True, params=["per_owner", "runs"])
 def ghost_writes(request, monkeypatch):
-    """Runs a test module's tests twice: the ghost copy and refresh writing
-    one lane per owner (DSDX_GHOST_RUNS=0) and staging each unit's slot runs
-    through LDS (DSDX_GHOST_RUNS=1), whatever the sizes would pick.  The
-    variable is read when a plan, batch or grid is created."""
+    """Runs a test module's tests twice: the ghost copy, refresh and
+    reverse_halo working one lane per owner (DSDX_GHOST_RUNS=0) and staging
+    each unit's slot runs through LDS (DSDX_GHOST_RUNS=1; reverse_halo follows
+    it), whatever the sizes would pick.  The variable is read when a plan,
+    batch or grid is created."""
     monkeypatch.setenv("DSDX_GHOST_RUNS", "1" if request.param == "runs" else "0")
     return request.param
