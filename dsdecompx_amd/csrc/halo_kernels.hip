@@ -432,13 +432,10 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_runs_kernel(dsdx_parti
                                                                       Layout L, PlanBuffers B)
 {
     __shared__ RunStage sh;
-    const RunSource src{MODE == kRunsBuild, MODE != kRunsPositions};
     for_run_units(desc, unit_replica, L, B, [&](const ReplicaDesc* d, uint32_t e0, uint32_t owners) {
-        ghost_runs_unit<STREAM>(
-            sh, p, desc, d, e0, owners, L, B, src,
-            [&](uint64_t i, uint32_t s, uint32_t rel) {
-                if (fp.count) copy_field_words(fp, i, ghost_slot(desc, d, s, rel));
-            },
+        ghost_runs_unit<STREAM, MODE == kRunsBuild, MODE != kRunsPositions>(
+            sh, p, desc, d, e0, owners, L, B, fp.count != 0,
+            [&](uint64_t i, uint32_t s, uint32_t rel) { copy_field_words(fp, i, ghost_slot(desc, d, s, rel)); },
             [&](uint32_t ol, uint32_t s, uint32_t rel) {
                 write_run_ghost<MODE>(p, sh, ol, s, sh.table.ghost0[s] + rel);
             });

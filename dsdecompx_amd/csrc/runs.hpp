@@ -41,22 +41,19 @@ struct RunStage {
     SlotTable table;
 };
 
-// What a unit stages per owner: its position from the classification pass
-// (bpos) or from the particle arrays, and the rest of its record or not.
-struct RunSource {
-    bool positions_from_entries;
-    bool records;
-};
-
 // One unit (e0: its first owner, owners: the replica's owner count), all
-// waves of the block.  fields(i, s, rel) runs on the owner's lane for each
-// image (i: the owner's particle slot, rel: the image's ghost-list entry),
-// write(ol, s, rel) once per image from the LDS records.
-template <bool STREAM, class Fields, class Write>
+// waves of the block.  Each owner's position comes from the classification
+// pass (bpos, FROM_ENTRIES) or from the particle arrays, and the rest of its
+// record is staged when RECORDS.  Both are compile-time choices: a run-time
+// select between an ordinary load of bpos and a nontemporal load of x, y, z
+// compiles to one ordinary load.  fields(i, s, rel) runs on the owner's lane
+// for each image when RECORDS (i: the owner's particle slot, rel: the image's
+// ghost-list entry), write(ol, s, rel) once per image from the LDS records.
+template <bool STREAM, bool FROM_ENTRIES, bool RECORDS, class Fields, class Write>
 __device__ __forceinline__ void ghost_runs_unit(RunStage& sh, const dsdx_particles& p,
                                                 const ReplicaDesc* __restrict__ desc, const ReplicaDesc* d,
                                                 uint32_t e0, uint32_t owners, const Layout& L,
-                                                const PlanBuffers& B, RunSource src, Fields&& fields,
+                                                const PlanBuffers& B, bool with_fields, Fields&& fields,
                                                 Write&& write)
 {
     const uint32_t lane = lane_id();
@@ -84,7 +81,7 @@ __device__ __forceinline__ void ghost_runs_unit(RunStage& sh, const dsdx_particl
         const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(code), kWave));
         if (live) {
             const uint64_t i = d->offset + oi;
-            if (src.positions_from_entries) {
+            if constexpr (FROM_ENTRIES) {
                 const uint32_t E = B.owner_entry[u];
                 sh.px[ol] = B.bpos[E];
                 sh.py[ol] = B.bpos[B.bpos_stride + E];
@@ -94,7 +91,7 @@ __device__ __forceinline__ void ghost_runs_unit(RunStage& sh, const dsdx_particl
                 sh.py[ol] = read_once<STREAM>(p.y + i);
                 sh.pz[ol] = read_once<STREAM>(p.z + i);
             }
-            if (src.records) {
+            if constexpr (RECORDS) {
                 sh.vx[ol] = read_once<STREAM>(p.vx + i);
                 sh.vy[ol] = read_once<STREAM>(p.vy + i);
                 sh.vz[ol] = read_once<STREAM>(p.vz + i);
@@ -102,9 +99,11 @@ __device__ __forceinline__ void ghost_runs_unit(RunStage& sh, const dsdx_particl
                 sh.q[ol] = read_once<STREAM>(p.charge + i);
                 sh.ty[ol] = read_once<STREAM>(p.type + i);
                 sh.id[ol] = read_once<STREAM>(p.global_id + i);
-                uint32_t k = 0;
-                for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
-                    fields(i, static_cast<uint32_t>(__builtin_ctz(rest)), glist[start + k]);
+                if (with_fields) {
+                    uint32_t k = 0;
+                    for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k)
+                        fields(i, static_cast<uint32_t>(__builtin_ctz(rest)), glist[start + k]);
+                }
             }
         }
         sh.code[ol] = static_cast<uint8_t>(code);
