@@ -499,7 +499,13 @@ __global__ __launch_bounds__(kRunWaves * kWave) void reverse_runs_kernel(double*
 {
     __shared__ ReverseStage sh;
     for_run_units(desc, unit_replica, L, B, [&](const ReplicaDesc* d, uint32_t e0, uint32_t owners) {
-        reverse_runs_unit(sh, fx, fy, fz, desc, d, e0, owners, L, B);
+        reverse_runs_unit(sh, fx, fy, fz, desc, d, e0, owners, L, B,
+                          [&](uint32_t s, uint32_t rel, double& gx, double& gy, double& gz) {
+                              const uint64_t j = sh.table.ghost0[s] + rel;
+                              gx = fx[j];
+                              gy = fy[j];
+                              gz = fz[j];
+                          });
     });
 }
 

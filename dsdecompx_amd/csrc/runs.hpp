@@ -192,11 +192,15 @@ struct ReverseStage {
     SlotTable table;
 };
 
+// ghost(s, rel, gx, gy, gz) loads the returned force of the image with
+// ghost-list entry rel sent through slot s (from the ghost particle, or from
+// its record).
+template <class Ghost>
 __device__ __forceinline__ void reverse_runs_unit(ReverseStage& sh, double* __restrict__ fx,
                                                   double* __restrict__ fy, double* __restrict__ fz,
                                                   const ReplicaDesc* __restrict__ desc, const ReplicaDesc* d,
                                                   uint32_t e0, uint32_t owners, const Layout& L,
-                                                  const PlanBuffers& B)
+                                                  const PlanBuffers& B, Ghost&& ghost)
 {
     const uint32_t lane = lane_id();
     const uint32_t t = uniform(threadIdx.x / kWave);
@@ -264,10 +268,7 @@ __device__ __forceinline__ void reverse_runs_unit(ReverseStage& sh, double* __re
                 if (sh.run_start[mid] <= r) lo = mid;
                 else hi = mid;
             }
-            const uint64_t j = sh.table.ghost0[lo] + sh.first[lo] + (r - sh.run_start[lo]);
-            sh.gx[r] = fx[j];
-            sh.gy[r] = fy[j];
-            sh.gz[r] = fz[j];
+            ghost(lo, sh.first[lo] + (r - sh.run_start[lo]), sh.gx[r], sh.gy[r], sh.gz[r]);
         }
         __syncthreads();
         uint32_t rest = uniform(wave_or(slots));
@@ -285,10 +286,11 @@ __device__ __forceinline__ void reverse_runs_unit(ReverseStage& sh, double* __re
     } else if (live) {
         uint32_t k = 0;
         for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k) {
-            const uint64_t j = sh.table.ghost0[__builtin_ctz(rest)] + glist[start + k];
-            ax = ax + fx[j];
-            ay = ay + fy[j];
-            az = az + fz[j];
+            double gx, gy, gz;
+            ghost(static_cast<uint32_t>(__builtin_ctz(rest)), glist[start + k], gx, gy, gz);
+            ax = ax + gx;
+            ay = ay + gy;
+            az = az + gz;
         }
     }
     if (live) {
