@@ -209,6 +209,9 @@ int reserve_buffers(Plan* plan, uint64_t ghost_list_elems)
     DSDX_HIP(plan->owner_code.reserve(slots), "hipMalloc");
     DSDX_HIP(plan->image_start.reserve(slots * 4), "hipMalloc");
     DSDX_HIP(plan->owner_entry.reserve(slots * 4), "hipMalloc");
+    // a replica of owned_r particles has at most ceil(owned_r / kRunOwners)
+    // ghost-run units, plus its end row
+    DSDX_HIP(plan->unit_rows.reserve((slots / kRunOwners + 2 * replicas) * kSlots * 4), "hipMalloc");
     if (ghost_list_elems > plan->ghost_list_capacity) {
         DSDX_HIP(plan->ghost_list.reserve(ghost_list_elems * 4), "hipMalloc");
         plan->ghost_list_capacity = plan->ghost_list.bytes / 4;
@@ -551,7 +554,10 @@ int dsdx_plan_reserve(dsdx_plan* plan, uint64_t owned, uint64_t ghosts)
 int dsdx_plan_set_shape(dsdx_plan* plan, int32_t items, int32_t waves, int32_t scan_threads)
 {
     if (!plan) return fail(DSDX_EINVAL, "null plan");
-    const Shape s{items, waves, scan_threads, plan->shape.xcd_run, plan->shape.descending};
+    Shape s = plan->shape;   // the other choices (XCD run, order, staged kernels) stay
+    s.items = items;
+    s.waves = waves;
+    s.scan_threads = scan_threads;
     if (!shape_supported(s)) return fail(DSDX_EINVAL, "unsupported launch shape");
     if (s.items != plan->shape.items) {
         plan->classified = plan->built = plan->totals_cached = false;
@@ -846,7 +852,10 @@ int dsdx_batch_set_shape(dsdx_batch* batch, int32_t items, int32_t waves, int32_
 {
     if (!batch) return fail(DSDX_EINVAL, "null batch");
     if (!stream) return fail(DSDX_EINVAL, "batch setup requires an explicit stream");
-    const Shape s{items, waves, scan_threads, batch->shape.xcd_run, batch->shape.descending};
+    Shape s = batch->shape;
+    s.items = items;
+    s.waves = waves;
+    s.scan_threads = scan_threads;
     if (!shape_supported(s)) return fail(DSDX_EINVAL, "unsupported launch shape");
     const bool relayout = s.items != batch->shape.items;
     batch->shape = s;

@@ -78,6 +78,7 @@ struct Plan {
     DeviceBuffer entries, bpos, counts, prefix, totals, owner_index, owner_code, image_start,
         ghost_list, owner_entry;
     DeviceBuffer recv;   // process grids only: receive-block bases and the grid status
+    DeviceBuffer unit_rows;   // first ghost-list value of each slot per ghost-run unit
     bool grid = false;   // replicas send to each other (dsdx_grid), not to themselves
     uint64_t ghost_list_capacity = 0;   // elements
     uint32_t* host_totals = nullptr;    // pinned copy of totals (R x kTotals)
@@ -134,7 +135,7 @@ struct Plan {
                 totals.as<uint32_t>(), owner_index.as<uint32_t>(), owner_code.as<uint8_t>(),
                 image_start.as<uint32_t>(), ghost_list.as<uint32_t>(),
                 owner_entry.as<uint32_t>(), desc.size() == 1 && !grid ? host_totals_dev : nullptr,
-                grid ? recv.as<uint32_t>() : nullptr};
+                grid ? recv.as<uint32_t>() : nullptr, unit_rows.as<uint32_t>()};
     }
 };
 

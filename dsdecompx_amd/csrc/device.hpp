@@ -167,6 +167,12 @@ struct PlanBuffers {
     // recv[r * kTotals + m] = first ghost of receive block m of replica r,
     // recv[R * kTotals] = overflow bits of the whole grid.
     uint32_t* recv;
+    // Written by the route pass for the ghost-run kernels (null: not kept):
+    // unit_rows[(first_unit_r + r + k) * kSlots + s] = the ghost-list value of
+    // the first image through send slot s at or after owner k * kRunOwners of
+    // replica r, and row k = the replica's unit count holds one past each
+    // slot's last value, so row k + 1 minus row k counts unit k's images.
+    uint32_t* unit_rows;
 };
 
 // Migration between the subdomains of a process grid (grid_kernels.hip).

@@ -293,6 +293,13 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
     const uint32_t ebase = chunk * K;
     const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * K + owner_base;
     uint32_t* glist = B.ghost_list + d->ghost_list_offset;
+    uint32_t* rows = B.unit_rows ? B.unit_rows + static_cast<uint64_t>(d->first_unit + r) * kSlots : nullptr;
+    if (rows && chunk == d->first_chunk && lane < kSlots) {
+        // the end row: one past the replica's last ghost of each slot
+        const uint32_t owners = totals[kColOwners];
+        rows[static_cast<uint64_t>((owners + kRunOwners - 1) / kRunOwners) * kSlots + lane] =
+            slot_first + totals[lane];
+    }
 
     for (uint32_t e0 = 0; e0 < n_boundary; e0 += kWave) {
         const uint32_t e = e0 + lane;
@@ -311,6 +318,12 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
             B.image_start[u] = first_image;
             B.owner_entry[u] = ebase + e;
         }
+        // These 64 owners hold at most one first owner of a ghost-run unit:
+        // the one in lane `at`, if any.
+        const uint32_t at = (kRunOwners - (owner_base + e0) % kRunOwners) % kRunOwners;
+        const bool unit_start = rows && at < kWave && e0 + at < n_boundary;
+        const uint64_t before_at = at < kWave ? (1ull << at) - 1u : ~0ull;
+        uint32_t row_value = slot_base + slot_run;   // lane s: slot s's next ghost-list value
         uint32_t present = uniform(wave_or(slots));
         uint32_t written = 0;
         while (present) {
@@ -325,8 +338,13 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
                 glist[first_image + written] = rel;
                 ++written;
             }
-            if (lane == s) slot_run += static_cast<uint32_t>(__popcll(ballot));
+            if (lane == s) {
+                slot_run += static_cast<uint32_t>(__popcll(ballot));
+                row_value += static_cast<uint32_t>(__popcll(ballot & before_at));
+            }
         }
+        if (unit_start && lane < kSlots)
+            rows[static_cast<uint64_t>((owner_base + e0 + at) / kRunOwners) * kSlots + lane] = row_value;
     }
 }
 
@@ -501,7 +519,7 @@ __global__ __launch_bounds__(kRunWaves * kWave) void reverse_runs_kernel(double*
     for_run_units(desc, unit_replica, L, B, [&](const ReplicaDesc* d, uint32_t e0, uint32_t owners) {
         reverse_runs_unit(sh, fx, fy, fz, desc, d, e0, owners, L, B,
                           [&](uint32_t s, uint32_t rel, double& gx, double& gy, double& gz) {
-                              const uint64_t j = sh.table.ghost0[s] + rel;
+                              const uint64_t j = ghost_slot(desc, d, s, rel);
                               gx = fx[j];
                               gy = fy[j];
                               gz = fz[j];

@@ -177,23 +177,68 @@ __device__ __forceinline__ void ghost_runs_unit(RunStage& sh, const dsdx_particl
     __syncthreads();
 }
 
+// The runs of one unit from its rows of PlanBuffers::unit_rows: lane s <
+// kSlots holds slot s's run (the ghost-list value of its first image, its
+// first index in the unit's runs laid end to end, its length), every lane the
+// unit's image count.
+struct UnitRuns {
+    uint32_t first, start, count, images;
+};
+
+__device__ __forceinline__ const uint32_t* unit_row(const ReplicaDesc* __restrict__ desc, const ReplicaDesc* d,
+                                                    uint32_t e0, const PlanBuffers& B)
+{
+    return B.unit_rows +
+           static_cast<uint64_t>(d->first_unit + static_cast<uint32_t>(d - desc) + e0 / kRunOwners) * kSlots;
+}
+
+// first_raw, next_raw: lane s's words of the unit's row and of the next row
+// (lanes past the slots load the last slot's and ignore it).
+__device__ __forceinline__ UnitRuns unit_runs(uint32_t first_raw, uint32_t next_raw)
+{
+    const uint32_t lane = lane_id();
+    UnitRuns u;
+    u.first = lane < kSlots ? first_raw : 0u;
+    u.count = lane < kSlots ? next_raw - first_raw : 0u;
+    const uint32_t end = wave_inclusive_sum(u.count);
+    u.start = end - u.count;
+    u.images = uniform(static_cast<uint32_t>(__shfl(static_cast<int>(end), kWave - 1, kWave)));
+    return u;
+}
+
+// The slot whose run holds index r < images: the last slot whose run starts
+// at or before r (an empty run starts where the next one does).
+__device__ __forceinline__ uint32_t run_slot(uint32_t start, uint32_t r)
+{
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 1; k < kSlots; ++k) s += static_cast<uint32_t>(__builtin_amdgcn_readlane(start, k)) <= r ? 1u : 0u;
+    return s;
+}
+
+// Lane s's v; every lane of the wave must be active (a shuffle reads 0 from
+// an inactive lane).
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, uint32_t s)
+{
+    return static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>(s), kWave));
+}
+
 // reverse_halo over a unit with the ghost forces read as dense runs: one
 // wave per owner tile holds its owners' forces; the unit's ghost forces come
 // into LDS in run order, then each owner adds its images in ascending send
-// slot (the per-owner kernel's order) from there.
+// slot (the per-owner kernel's order) from there.  The unit's runs come from
+// its rows, loaded with the owner maps, so the ghost loads need no barrier;
+// they are issued before the owners' forces, whose loads then stay in flight
+// across the barrier (a wave waits for its loads in issue order).
 static_assert(kRunTiles == kRunWaves, "one owner tile per wave");
 
 struct ReverseStage {
     double gx[kRunMap], gy[kRunMap], gz[kRunMap];
-    uint32_t tile_rank[kRunTiles][kSlots];
-    uint32_t first[kSlots];
-    uint32_t run_start[kSlots + 1];
-    uint32_t images;
-    SlotTable table;
+    uint32_t tile_count[kRunTiles][kSlots];   // owners of each tile with each slot
 };
 
 // ghost(s, rel, gx, gy, gz) loads the returned force of the image with
-// ghost-list entry rel sent through slot s (from the ghost particle, or from
+// ghost-list value rel sent through slot s (from the ghost particle, or from
 // its record).
 template <class Ghost>
 __device__ __forceinline__ void reverse_runs_unit(ReverseStage& sh, double* __restrict__ fx,
@@ -204,90 +249,68 @@ __device__ __forceinline__ void reverse_runs_unit(ReverseStage& sh, double* __re
 {
     const uint32_t lane = lane_id();
     const uint32_t t = uniform(threadIdx.x / kWave);
-    const uint32_t lut_l = d->bp.lut[lane];
     const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * L.chunk_size;
-    const uint32_t* glist = B.ghost_list + d->ghost_list_offset;
-    if (t == 0) fill_slot_table(sh.table, desc, d);
-    for (uint32_t k = threadIdx.x; k < kRunTiles * kSlots; k += kRunWaves * kWave) (&sh.tile_rank[0][0])[k] = 0u;
-    if (threadIdx.x < kSlots) sh.first[threadIdx.x] = 0xFFFFFFFFu;
-    __syncthreads();
     const uint32_t e = e0 + t * kWave + lane;
     const bool live = e < owners;
-    const uint64_t u = slot0 + e;
-    uint32_t oi = 0, code = 0, start = 0;
-    if (live) {
-        oi = B.owner_index[u];
-        code = B.owner_code[u];
-        start = B.image_start[u];
+    const uint64_t u = slot0 + (live ? e : e0);
+    const uint32_t sl = lane < kSlots ? lane : kSlots - 1u;
+    const uint32_t* row = unit_row(desc, d, e0, B);
+    // every load the unit's layout needs, issued together
+    const uint32_t first_raw = row[sl];
+    const uint32_t next_raw = row[kSlots + sl];
+    const uint32_t oi_raw = B.owner_index[u];
+    const uint32_t code_raw = B.owner_code[u];
+    const uint32_t lut_l = d->bp.lut[lane];
+    const UnitRuns runs = unit_runs(first_raw, next_raw);
+    const uint32_t slots = lane_value(lut_l, live ? code_raw : 0u);
+    const bool staged = runs.images <= kRunMap;
+    if (staged) {
+        // every lane takes part in each round: a shuffle reads 0 from an inactive lane
+        for (uint32_t r0 = 0; r0 < runs.images; r0 += kRunWaves * kWave) {
+            const uint32_t r = r0 + threadIdx.x;
+            const uint32_t s = run_slot(runs.start, r);
+            const uint32_t rel = lane_value(runs.first, s) + (r - lane_value(runs.start, s));
+            if (r < runs.images) ghost(s, rel, sh.gx[r], sh.gy[r], sh.gz[r]);
+        }
     }
-    const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut_l), static_cast<int>(code), kWave));
-    const uint64_t i = d->offset + oi;
+    const uint64_t i = d->offset + (live ? oi_raw : 0u);
     double ax = 0.0, ay = 0.0, az = 0.0;
     if (live) {
         ax = fx[i];
         ay = fy[i];
         az = fz[i];
     }
-    uint32_t present = uniform(wave_or(slots));
-    uint32_t leads = 0;
-    while (present) {
-        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(present));
-        present &= present - 1u;
-        const uint64_t b = __ballot((slots >> s) & 1u);
-        if (lane == static_cast<uint32_t>(__builtin_ctzll(b))) leads |= 1u << s;
-        if (lane == 0) sh.tile_rank[t][s] = static_cast<uint32_t>(__popcll(b));
-    }
-    for (; leads; leads &= leads - 1u) {
-        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(leads));
-        const uint32_t k = static_cast<uint32_t>(__popc(slots & ((1u << s) - 1u)));
-        atomicMin(&sh.first[s], glist[start + k]);
-    }
-    __syncthreads();
-    if (t == 0) {
-        uint32_t n = 0;
-        if (lane < kSlots)
-            for (uint32_t q = 0; q < kRunTiles; ++q) {
-                const uint32_t v = sh.tile_rank[q][lane];
-                sh.tile_rank[q][lane] = n;
-                n += v;
-            }
-        const uint32_t end = wave_inclusive_sum(n);
-        if (lane < kSlots) sh.run_start[lane] = end - n;
-        if (lane == kSlots - 1) {
-            sh.run_start[kSlots] = end;
-            sh.images = end;
-        }
-    }
-    __syncthreads();
-    const uint32_t images = sh.images;
-    if (images <= kRunMap) {
-        for (uint32_t r = threadIdx.x; r < images; r += kRunWaves * kWave) {
-            uint32_t lo = 0, hi = kSlots;   // the last slot whose run starts at or before r
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) / 2;
-                if (sh.run_start[mid] <= r) lo = mid;
-                else hi = mid;
-            }
-            ghost(lo, sh.first[lo] + (r - sh.run_start[lo]), sh.gx[r], sh.gy[r], sh.gz[r]);
-        }
-        __syncthreads();
-        uint32_t rest = uniform(wave_or(slots));
-        while (rest) {
+    if (staged) {
+        const uint32_t present = uniform(wave_or(slots));
+        uint32_t mine = 0;   // lane s: this tile's owners with slot s
+        for (uint32_t rest = present; rest; rest &= rest - 1u) {
             const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
-            rest &= rest - 1u;
+            const uint64_t b = __ballot((slots >> s) & 1u);
+            if (lane == s) mine = static_cast<uint32_t>(__popcll(b));
+        }
+        if (lane < kSlots) sh.tile_count[t][lane] = mine;
+        __syncthreads();
+        uint32_t before = 0;   // lane s: owners with slot s in the unit's earlier tiles
+        if (lane < kSlots)
+            for (uint32_t q = 0; q < t; ++q) before += sh.tile_count[q][lane];
+        for (uint32_t rest = present; rest; rest &= rest - 1u) {
+            const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
             const uint64_t b = __ballot((slots >> s) & 1u);
             if ((slots >> s) & 1u) {
-                const uint32_t r = sh.run_start[s] + sh.tile_rank[t][s] + lanes_below(b);
+                const uint32_t r = static_cast<uint32_t>(__builtin_amdgcn_readlane(runs.start, s)) +
+                                   static_cast<uint32_t>(__builtin_amdgcn_readlane(before, s)) + lanes_below(b);
                 ax = ax + sh.gx[r];
                 ay = ay + sh.gy[r];
                 az = az + sh.gz[r];
             }
         }
     } else if (live) {
+        // more images than LDS holds: each owner reads its own images
+        const uint32_t* list = B.ghost_list + d->ghost_list_offset + B.image_start[u];
         uint32_t k = 0;
         for (uint32_t rest = slots; rest; rest &= rest - 1u, ++k) {
             double gx, gy, gz;
-            ghost(static_cast<uint32_t>(__builtin_ctz(rest)), glist[start + k], gx, gy, gz);
+            ghost(static_cast<uint32_t>(__builtin_ctz(rest)), list[k], gx, gy, gz);
             ax = ax + gx;
             ay = ay + gy;
             az = az + gz;
