@@ -869,7 +869,7 @@ hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Lay
     const Layout O = ordered(L, s, kOrderCopy);
     if (L.tiles == 0) return hipSuccess;
     // Each workgroup walks units g, g + grid, ... up to the device's owner count.
-    if (ghost_runs(L))
+    if (ghost_runs(L, B))
         return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr bool S = decltype(streaming)::value;
             return launch(copy_runs_kernel<kRunsBuild, S>, run_grid(L), dim3(kRunWaves * kWave), stream, p, fp,
@@ -890,7 +890,7 @@ hipError_t launch_reverse_forces(double* fx, double* fy, double* fz, const Layou
 {
     const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
-    if (s.reverse_runs < 0 ? ghost_runs(L) : s.reverse_runs > 0)
+    if (B.unit_rows && (s.reverse_runs < 0 ? ghost_runs(L, B) : s.reverse_runs > 0))
         return launch(reverse_runs_kernel, run_grid(L), dim3(kRunWaves * kWave), stream, fx, fy, fz, L.desc,
                       L.unit_replica, O, B);
     return dispatch_waves(s, [&](auto waves) {
@@ -929,7 +929,7 @@ hipError_t launch_refresh(const dsdx_particles& p, const FieldParams& fp, int32_
 {
     const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
-    if (ghost_runs(L))
+    if (ghost_runs(L, B))
         return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr bool S = decltype(streaming)::value;
             if (positions_only)
