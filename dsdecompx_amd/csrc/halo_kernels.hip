@@ -506,8 +506,9 @@ __global__ __launch_bounds__(WAVES * kWave) void reverse_forces_kernel(
 
 // reverse_halo through the staged runs (Shape::reverse_runs): a unit's ghost
 // forces read as dense runs into LDS, each owner's images added from there in
-// ascending send slot (runs.hpp).  512 replica boxes of 2,048-262,144
-// particles: 434 -> 418 us; one box of 16 M particles: 102.7 -> 100.6 us.
+// ascending send slot (runs.hpp).  Against the per-owner kernel: 512 replica
+// boxes of 2,048-262,144 particles 434 -> 398 us; one box of 16 M particles
+// 102.7 -> 94.0 us.
 __global__ __launch_bounds__(kRunWaves * kWave) void reverse_runs_kernel(double* __restrict__ fx,
                                                                          double* __restrict__ fy,
                                                                          double* __restrict__ fz,
