@@ -40,7 +40,7 @@ __device__ __forceinline__ uint32_t slab_code(double px, double py, double pz, c
 // With WRAP the pass first applies migrate's periodic wrap to the positions
 // it loaded (storing the changed ones) and classifies the wrapped values:
 // migrate followed by a rebuild with one read of x, y, z.
-template <int ITEMS, int WAVES, bool WRAP, bool STREAM>
+template <int ITEMS, int WAVES, bool WRAP, bool STREAM, int SUB>
 __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
     double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
     const ReplicaDesc* __restrict__ desc, const uint32_t* __restrict__ chunk_replica, Layout L,
@@ -48,6 +48,7 @@ __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
 {
     constexpr uint32_t K = ITEMS * kWave;
     constexpr uint32_t kRing = 2 * kWave;
+    static_assert(ITEMS % SUB == 0, "the chunk's items split into whole rounds");
     __shared__ uint32_t s_count[WAVES][kWave];
     __shared__ uint32_t s_ring_e[WAVES][kRing];
     __shared__ double s_ring_p[WAVES][3][kRing];
@@ -62,20 +63,6 @@ __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
     const uint64_t off = d->offset;
     const uint32_t first = (chunk - d->first_chunk) * K;
     const bool full = first + K <= owned;
-    double px[ITEMS], py[ITEMS], pz[ITEMS];
-    load_items<ITEMS, STREAM>(x + off, first, owned, full, px);
-    load_items<ITEMS, STREAM>(y + off, first, owned, full, py);
-    load_items<ITEMS, STREAM>(z + off, first, owned, full, pz);
-    if (WRAP) {
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const uint32_t i = first + k * kWave + lane;
-            const bool valid = i < owned;
-            wrap_store(px[k], x + off + i, valid, d->bp.periodic[0], d->bp.lower[0], d->bp.extent[0]);
-            wrap_store(py[k], y + off + i, valid, d->bp.periodic[1], d->bp.lower[1], d->bp.extent[1]);
-            wrap_store(pz[k], z + off + i, valid, d->bp.periodic[2], d->bp.lower[2], d->bp.extent[2]);
-        }
-    }
     // Boundary particles go through a per-wave LDS ring of 2 * 64 slots and
     // leave it 64 at a time, so the entry and position stores are full-wave
     // and contiguous.  The ring never holds more than 127 entries: it is
@@ -100,26 +87,47 @@ __global__ __launch_bounds__(WAVES * kWave) void classify_kernel(
     uint32_t running = 0;
     uint32_t drained = 0;
     uint32_t images = 0;
+    // The chunk's items in rounds of SUB: the loads of one round are in flight
+    // together; fewer items per round hold fewer registers, so more waves run.
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {
-        const uint32_t i = first + k * kWave + lane;
-        const uint32_t code = i < owned ? slab_code(px[k], py[k], pz[k], d->bp) : 0u;
-        const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut), static_cast<int>(code), kWave));
-        const bool boundary = slots != 0u;
-        const uint64_t ballot = __ballot(boundary);
-        if (boundary) {
-            const uint32_t q = (running + lanes_below(ballot)) & (kRing - 1);
-            s_ring_e[wave][q] = ((static_cast<uint32_t>(k) * kWave + lane) << 8) | code;
-            s_ring_p[wave][0][q] = px[k];
-            s_ring_p[wave][1][q] = py[k];
-            s_ring_p[wave][2][q] = pz[k];
-            images += static_cast<uint32_t>(__popc(slots));
-            for (uint32_t m = slots; m; m &= m - 1u) atomicAdd(&s_count[wave][__builtin_ctz(m)], 1u);
+    for (uint32_t k0 = 0; k0 < ITEMS; k0 += SUB) {
+        const uint32_t round_first = first + k0 * kWave;
+        double px[SUB], py[SUB], pz[SUB];
+        load_items<SUB, STREAM>(x + off, round_first, owned, full, px);
+        load_items<SUB, STREAM>(y + off, round_first, owned, full, py);
+        load_items<SUB, STREAM>(z + off, round_first, owned, full, pz);
+        if (WRAP) {
+#pragma unroll
+            for (int k = 0; k < SUB; ++k) {
+                const uint32_t i = round_first + k * kWave + lane;
+                const bool valid = i < owned;
+                wrap_store(px[k], x + off + i, valid, d->bp.periodic[0], d->bp.lower[0], d->bp.extent[0]);
+                wrap_store(py[k], y + off + i, valid, d->bp.periodic[1], d->bp.lower[1], d->bp.extent[1]);
+                wrap_store(pz[k], z + off + i, valid, d->bp.periodic[2], d->bp.lower[2], d->bp.extent[2]);
+            }
         }
-        running += static_cast<uint32_t>(__popcll(ballot));
-        if (running - drained >= kWave) {
-            drain(drained, true);
-            drained += kWave;
+#pragma unroll
+        for (int k = 0; k < SUB; ++k) {
+            const uint32_t item = k0 + static_cast<uint32_t>(k);
+            const uint32_t i = first + item * kWave + lane;
+            const uint32_t code = i < owned ? slab_code(px[k], py[k], pz[k], d->bp) : 0u;
+            const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut), static_cast<int>(code), kWave));
+            const bool boundary = slots != 0u;
+            const uint64_t ballot = __ballot(boundary);
+            if (boundary) {
+                const uint32_t q = (running + lanes_below(ballot)) & (kRing - 1);
+                s_ring_e[wave][q] = ((item * kWave + lane) << 8) | code;
+                s_ring_p[wave][0][q] = px[k];
+                s_ring_p[wave][1][q] = py[k];
+                s_ring_p[wave][2][q] = pz[k];
+                images += static_cast<uint32_t>(__popc(slots));
+                for (uint32_t m = slots; m; m &= m - 1u) atomicAdd(&s_count[wave][__builtin_ctz(m)], 1u);
+            }
+            running += static_cast<uint32_t>(__popcll(ballot));
+            if (running - drained >= kWave) {
+                drain(drained, true);
+                drained += kWave;
+            }
         }
     }
     if (running > drained) drain(drained, lane < running - drained);
@@ -169,6 +177,78 @@ __global__ __launch_bounds__(kMaxScanThreads) void scan_kernel(const ReplicaDesc
     const uint64_t base = static_cast<uint64_t>(col) * L.chunks + d->first_chunk;
     const uint32_t carry = scan_column(B.counts + base, B.prefix + base, d->chunks, s_tile, s_wave);
     if (threadIdx.x == 0) column_done(B, L, col, r, carry);
+}
+
+// Long columns cut into parts of one LDS tile (blockDim.x * kScanPer values):
+// grid (kCols * parts, replicas).  The block of part p loads its tile and,
+// in the same round of loads, every value before the tile, whose sum it
+// reduces itself; the parts of a column therefore need nothing from each
+// other, and every block waits for one round of loads instead of one per
+// tile.  The part that ends the column writes its total.  A part reduces up
+// to kMaxScanParts - 1 tiles before its own; longer columns take scan_kernel.
+constexpr uint32_t kMaxScanParts = 8;
+
+__global__ __launch_bounds__(kMaxScanThreads) void scan_parts_kernel(const ReplicaDesc* __restrict__ desc,
+                                                                     Layout L, PlanBuffers B)
+{
+    __shared__ uint32_t s_tile[kScanTile + kScanTile / 32];
+    __shared__ uint32_t s_wave[kMaxScanThreads / kWave + 1];
+    const uint32_t T = blockDim.x;
+    const uint32_t t = threadIdx.x;
+    const uint32_t col = blockIdx.x % kCols;
+    const uint32_t part = blockIdx.x / kCols;
+    const uint32_t r = blockIdx.y;
+    const ReplicaDesc* d = desc + r;
+    const uint32_t n = d->chunks;
+    const uint32_t span = T * kScanPer;
+    const uint32_t lo = part * span;
+    if (part > 0 && lo >= n) return;   // block-uniform: this replica has fewer parts
+    const uint64_t base = static_cast<uint64_t>(col) * L.chunks + d->first_chunk;
+    const uint32_t* src = B.counts + base;
+    uint32_t* dst = B.prefix + base;
+    const uint32_t hi = min(n, lo + span);
+    uint32_t v[kScanPer];
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const uint32_t i = lo + k * T + t;
+        v[k] = i < hi ? src[i] : 0u;
+    }
+    uint32_t before = 0;
+    for (uint32_t i0 = 0; i0 < lo; i0 += kScanPer * T) {
+        uint32_t w[kScanPer];
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) {
+            const uint32_t i = i0 + k * T + t;
+            w[k] = i < lo ? src[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < kScanPer; ++k) before += w[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) s_tile[scan_pad(k * T + t)] = v[k];
+    uint32_t carry = 0;
+    (void)block_exclusive(before, s_wave, T / kWave, &carry);   // its barriers also publish s_tile
+    uint32_t mine[kScanPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+        mine[j] = s_tile[scan_pad(t * kScanPer + j)];
+        sum += mine[j];
+    }
+    uint32_t total = 0;
+    uint32_t run = carry + block_exclusive(sum, s_wave, T / kWave, &total);
+#pragma unroll
+    for (int j = 0; j < kScanPer; ++j) {
+        s_tile[scan_pad(t * kScanPer + j)] = run;
+        run += mine[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const uint32_t i = lo + k * T + t;
+        if (i < hi) dst[i] = s_tile[scan_pad(k * T + t)];
+    }
+    if (hi == n && t == 0) column_done(B, L, col, r, carry + total);
 }
 
 // Short columns (every replica at most kShortScanChunks chunks, as in a batch
@@ -250,6 +330,12 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
     const uint32_t lane = lane_id();
     const uint32_t chunk = wave_chunk<WAVES>(L, L.chunks);
     if (chunk >= L.chunks) return;
+    const uint32_t ebase = chunk * K;
+    // The chunk's prefix of every column (lane c: column c) and its first 64
+    // entries (the entry region always holds K) are loaded with the totals,
+    // before anything waits (18.6 -> 17.1 us at 16 M particles).
+    const uint32_t col_prefix = lane < kCols ? B.prefix[static_cast<uint64_t>(lane) * L.chunks + chunk] : 0u;
+    const uint32_t entry_first = B.entries[ebase + lane];
     const uint32_t r = replica_of(chunk_replica, chunk);
     const ReplicaDesc* d = desc + r;
     uint32_t* totals = B.totals + static_cast<uint64_t>(r) * kTotals;
@@ -283,14 +369,13 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
         slot_first = __shfl(recv_base, lane < kSlots ? kSlots - 1 - lane : 0, kWave);
     }
     const uint32_t slot_base =
-        lane < kSlots ? slot_first + B.prefix[static_cast<uint64_t>(lane) * L.chunks + chunk] : 0u;
+        lane < kSlots ? slot_first + col_prefix : 0u;
     uint32_t slot_run = 0;   // lane s: ghosts of slot s already routed from this chunk
 
     const uint32_t n_boundary = uniform(B.counts[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
-    const uint32_t owner_base = uniform(B.prefix[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
-    uint32_t image_run = uniform(B.prefix[static_cast<uint64_t>(kColImages) * L.chunks + chunk]);
+    const uint32_t owner_base = static_cast<uint32_t>(__builtin_amdgcn_readlane(col_prefix, kColOwners));
+    uint32_t image_run = static_cast<uint32_t>(__builtin_amdgcn_readlane(col_prefix, kColImages));
     const uint32_t first = (chunk - d->first_chunk) * K;
-    const uint32_t ebase = chunk * K;
     const uint64_t slot0 = static_cast<uint64_t>(d->first_chunk) * K + owner_base;
     uint32_t* glist = B.ghost_list + d->ghost_list_offset;
     uint32_t* rows = B.unit_rows ? B.unit_rows + static_cast<uint64_t>(d->first_unit + r) * kSlots : nullptr;
@@ -304,7 +389,7 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
     for (uint32_t e0 = 0; e0 < n_boundary; e0 += kWave) {
         const uint32_t e = e0 + lane;
         const bool valid = e < n_boundary;
-        const uint32_t entry = valid ? B.entries[ebase + e] : 0u;
+        const uint32_t entry = valid ? (e0 == 0 ? entry_first : B.entries[ebase + e]) : 0u;
         const uint32_t code = entry & 63u;
         const uint32_t slots = static_cast<uint32_t>(__shfl(static_cast<int>(lut), static_cast<int>(code), kWave));
         const uint32_t images = static_cast<uint32_t>(__popc(slots));
@@ -833,10 +918,15 @@ hipError_t launch_classify(double* x, double* y, double* z, bool wrap_first, con
         return dispatch_stream(L.streaming, [&](auto streaming) {
             constexpr int I = decltype(items)::value, W = decltype(waves)::value;
             constexpr bool S = decltype(streaming)::value;
+            // The fused pass takes a chunk in rounds of at most 8 items: with
+            // the wrap's stores it holds fewer registers and runs more waves
+            // (16 M particles: 80 -> 74.7 us; rounds of 4 measured the same).
+            // The plain pass measured the same either way and keeps one round.
+            constexpr int kWrapRound = I < 8 ? I : 8;
             if (wrap_first)
-                return launch(classify_kernel<I, W, true, S>, chunk_grid(L, W), dim3(W * kWave),
+                return launch(classify_kernel<I, W, true, S, kWrapRound>, chunk_grid(L, W), dim3(W * kWave),
                               stream, x, y, z, L.desc, L.chunk_replica, O, B);
-            return launch(classify_kernel<I, W, false, S>, chunk_grid(L, W), dim3(W * kWave),
+            return launch(classify_kernel<I, W, false, S, I>, chunk_grid(L, W), dim3(W * kWave),
                           stream, x, y, z, L.desc, L.chunk_replica, O, B);
         });
     });
@@ -850,6 +940,14 @@ hipError_t launch_scan(const Layout& L, const PlanBuffers& B, const Shape& s, hi
         return launch(scan_short_kernel, dim3((pairs + kShortScanWaves - 1) / kShortScanWaves),
                       dim3(kShortScanWaves * kWave), stream, L.desc, L, B);
     }
+    // Up to kMaxScanParts tiles per column: one block per tile, each reducing
+    // the values before its tile itself (16 M particles: 9.5 -> 5.0 us).  Longer
+    // columns: one block per column walking its tiles.
+    const uint32_t span = static_cast<uint32_t>(s.scan_threads) * kScanPer;
+    const uint32_t parts = std::max<uint32_t>(1, (L.max_replica_chunks + span - 1) / span);
+    if (parts <= kMaxScanParts)
+        return launch(scan_parts_kernel, dim3(kCols * parts, L.replicas), dim3(s.scan_threads), stream, L.desc,
+                      L, B);
     return launch(scan_kernel, dim3(kCols, L.replicas), dim3(s.scan_threads), stream, L.desc, L, B);
 }
 

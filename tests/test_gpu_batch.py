@@ -55,14 +55,24 @@ def _snapshot(b):
 
 # 12 replicas: the owner-major passes walk the live tiles they count on the
 # device; 80 replicas (more than a wave has lanes): the tiles numbered on the
-# host, one per wave.
-@pytest.mark.parametrize("count,seed,nmax", [(12, 5, 30000), (80, 31, 3000)], ids=["12", "80"])
-def test_batch_matches_oracle_and_loop(gpu, count, seed, nmax):
+# host, one per wave.  "long": one replica of 600,000 particles among small
+# ones, in chunks of 256 with 256 scan threads, so that its count columns are
+# scanned in 2 parts while the small replicas' second parts have nothing to do.
+@pytest.mark.parametrize("count,seed,nmax,big,shape", [(12, 5, 30000, 0, None), (80, 31, 3000, 0, None),
+                                                       (6, 7, 30000, 600000, (4, 4, 256))],
+                         ids=["12", "80", "long"])
+def test_batch_matches_oracle_and_loop(gpu, count, seed, nmax, big, shape):
     torch = gpu
-    from dsdecompx_amd.batch import LoopedReplicas, ReplicaBatch
+    from dsdecompx_amd.batch import LoopedReplicas, ReplicaBatch, ReplicaSpec
+    from dsdecompx_amd.halo import BoxConfig
     specs = _specs(count, seed, nmax=nmax)
-    say("batch", [s.owned for s in specs])
+    if big:
+        L = float((big / 0.9) ** (1.0 / 3.0))
+        specs[2] = ReplicaSpec(big, BoxConfig(upper=(L, L, L), width=2.8))
+    say("batch", [s.owned for s in specs], shape)
     b = ReplicaBatch(specs)
+    if shape:
+        b.set_shape(*shape)
     _fill(torch, b, 6)
     start = _snapshot(b)
     rng = np.random.default_rng(9)
