@@ -14,6 +14,8 @@ oracle on identical inputs:
   with a halo so wide (w = 4 in L = 20) that most units hold more images than
   the staging map and are written one lane per owner; then the reused-plan
   refresh, positions only and the whole record with the fields.
+- 2,500,000 and 5,000,000 particles in chunks of 256 with 256 scan threads:
+  count columns scanned in 5 parts, and walked by one block per column.
 """
 import numpy as np
 import pytest
@@ -203,3 +205,35 @@ def test_large_box_ghost_runs(gpu, monkeypatch, L, w, descending):
         compare_particles(dev, ref, n + g, f"large box refresh full={full}")
         assert_bits_equal(host(td[:n + g]), dens_ref[:n + g], "refresh density")
         assert_bits_equal(host(tv[:n + g]), vec_ref[:n + g], "refresh vec f32")
+
+
+@pytest.mark.parametrize("n", [2_500_000, 5_000_000], ids=["five_parts", "column_walk"])
+def test_long_count_columns(gpu, monkeypatch, n):
+    """Chunks of 256 particles and 256 scan threads: a part covers 2,048
+    chunks, so 2.5 M particles scan each count column in 5 parts, and 5 M
+    particles (more than 8 parts) take the one-block-per-column scan.  The
+    build and reverse_halo are bit-exact with the oracle either way."""
+    torch = gpu
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    monkeypatch.setenv("DSDX_ITEMS", "4")
+    monkeypatch.setenv("DSDX_SCAN_THREADS", "256")
+    L, w = float(round(n ** (1.0 / 3.0), 1)), 2.8
+    say("long count columns", n, L)
+    d = synth(n, L, 6060 + n % 97)
+    rng = np.random.default_rng(6061)
+    cap = int(n * 1.25) + 4096
+    ora = po.Oracle(po.Box(upper=(L,) * 3, width=w))
+    op = to_oracle(d, cap)
+    dev = to_device(d, cap)
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
+    g = ora.exchange_halo(op, n)
+    assert halo.exchange_halo(dev, n) == g
+    compare_particles(dev, op, n + g, f"long columns n={n}")
+    f = rng.uniform(-1, 1, (3, n + g))
+    for a, k in enumerate(("fx", "fy", "fz")):
+        getattr(op, k)[:n + g] = f[a]
+        getattr(dev, k)[:n + g] = torch.from_numpy(f[a]).cuda()
+    ora.reverse_halo(op, n)
+    halo.reverse_halo(dev, n)
+    for k in ("fx", "fy", "fz"):
+        assert_bits_equal(host(getattr(dev, k)[:n]), getattr(op, k)[:n], f"long columns reverse {k}")
