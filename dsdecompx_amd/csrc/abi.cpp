@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -232,6 +233,17 @@ int reserve_buffers(Plan* plan, uint64_t ghost_list_elems)
         DSDX_HIP(hipHostGetDevicePointer(&dev, host, 0), "hipHostGetDevicePointer");
         plan->host_totals_dev = static_cast<uint32_t*>(dev);
     }
+    if (!plan->host_counts) {
+        void* host = nullptr;
+        // mapped and coherent: the scan stores the stamped column totals into it
+        DSDX_HIP(hipHostMalloc(&host, kCols * sizeof(uint64_t), hipHostMallocMapped | hipHostMallocCoherent),
+                 "hipHostMalloc");
+        plan->host_counts = static_cast<uint64_t*>(host);
+        std::memset(host, 0, kCols * sizeof(uint64_t));   // stamp 0 is never a build's
+        void* dev = nullptr;
+        DSDX_HIP(hipHostGetDevicePointer(&dev, host, 0), "hipHostGetDevicePointer");
+        plan->host_counts_dev = static_cast<uint64_t*>(dev);
+    }
     return DSDX_OK;
 }
 
@@ -376,15 +388,51 @@ int set_single(Plan* plan, const BoxParams& bp, uint64_t owned, uint64_t capacit
     return DSDX_OK;
 }
 
-// Single-box plans: the scan also writes the column totals into pinned host
-// memory, and the host waits for the scan only, never for the ghost records
-// (the reference's rebuild likewise waits for the classification counts
-// alone, src/halo.cpp:482-491).  The overflow status is derived here exactly
-// as the route pass derives it on the device.
+namespace {
+
+// Column c of the last single-box classify has arrived when its stamp does.
+uint32_t arrived_columns(const Plan* plan)
+{
+    uint32_t c = 0;
+    while (c < static_cast<uint32_t>(kCols) &&
+           static_cast<uint32_t>(__atomic_load_n(plan->host_counts + c, __ATOMIC_ACQUIRE) >> 32) == plan->stamp)
+        ++c;
+    return c;
+}
+
+}  // namespace
+
+// Single-box plans: the scan also stores the column totals, each tagged with
+// the build's stamp, into pinned host memory, and the host waits for the
+// scan only, never for the ghost records (the reference's rebuild likewise
+// waits for the classification counts alone, src/halo.cpp:482-491).  The
+// host spins on the stamps.  Only after 50 ms without them does it ask the
+// stream, every 10 ms, whether it failed or drained without them (then they
+// can no longer arrive): a stream query queues work of its own on the
+// device, which held the GPU idle for 5 us after the ghost copy.  The
+// overflow status is derived here exactly as the route pass derives it on
+// the device.
 int wait_counts(Plan* plan)
 {
-    DSDX_HIP(hipEventSynchronize(plan->counted), "wait for halo counts");
+    if (!plan->host_counts) return fail(DSDX_ESTATE, "no halo counts to wait for");
+    using clock = std::chrono::steady_clock;
+    const clock::time_point start = clock::now();
+    clock::time_point next_query = start + std::chrono::milliseconds(50);
+    for (uint32_t spin = 1; arrived_columns(plan) < static_cast<uint32_t>(kCols); ++spin) {
+#if defined(__x86_64__)
+        __builtin_ia32_pause();
+#endif
+        if (spin % 256 || clock::now() < next_query) continue;
+        next_query = clock::now() + std::chrono::milliseconds(10);
+        const hipError_t q = hipStreamQuery(plan->count_stream);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return hip_fail(q, "wait for halo counts");
+        if (arrived_columns(plan) < static_cast<uint32_t>(kCols))
+            return fail(DSDX_EDEVICE, "the halo counts of the last classify did not arrive");
+    }
     uint32_t* t = plan->host_totals;
+    for (int c = 0; c < kCols; ++c) t[c] = static_cast<uint32_t>(plan->host_counts[c]);
+    t[kColReceived] = t[kColImages];
     const ReplicaDesc& d = plan->desc[0];
     const uint64_t ghosts = t[kColImages];
     uint32_t status = 0;
@@ -409,6 +457,9 @@ int read_totals(Plan* plan, hipStream_t stream)
 
 int run_classify(Plan* plan, const dsdx_particles* p, bool wrap_first, hipStream_t s)
 {
+    // a fresh stamp for the scan's host copy of the counts (0 is never used)
+    plan->stamp = plan->stamp + 1 ? plan->stamp + 1 : 1;
+    plan->count_stream = s;
     const Layout L = plan->layout();
     const PlanBuffers B = plan->buffers();
     {
@@ -419,17 +470,6 @@ int run_classify(Plan* plan, const dsdx_particles* p, bool wrap_first, hipStream
     {
         timing::Scope timed(timing::kScan, s);
         DSDX_HIP(launch_scan(L, B, plan->shape, s), "launch halo scan");
-    }
-    if (plan->desc.size() == 1) {
-        // the host waits for the counts on this event; a captured build is
-        // never waited for, so it records none
-        hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
-        DSDX_HIP(hipStreamIsCapturing(s, &capture), "query stream capture");
-        if (capture == hipStreamCaptureStatusNone) {
-            if (!plan->counted)
-                DSDX_HIP(hipEventCreateWithFlags(&plan->counted, hipEventDisableTiming), "hipEventCreate");
-            DSDX_HIP(hipEventRecord(plan->counted, s), "record halo counts");
-        }
     }
     plan->classified = true;
     return DSDX_OK;

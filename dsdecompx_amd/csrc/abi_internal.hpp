@@ -84,7 +84,12 @@ struct Plan {
     uint32_t* host_totals = nullptr;    // pinned copy of totals (R x kTotals)
     uint32_t* host_totals_dev = nullptr;  // the same memory, as the device addresses it
     size_t host_totals_words = 0;
-    hipEvent_t counted = nullptr;       // recorded after the scan of a single-box classify
+    // Single box: the scan's stamped column totals (PlanBuffers::host_counts),
+    // the stamp of the last classify and the stream it ran on.
+    uint64_t* host_counts = nullptr;
+    uint64_t* host_counts_dev = nullptr;
+    uint32_t stamp = 0;
+    hipStream_t count_stream = nullptr;
 
     Plan() = default;
     Plan(const Plan&) = delete;
@@ -92,7 +97,7 @@ struct Plan {
     ~Plan()
     {
         if (host_totals) (void)hipHostFree(host_totals);
-        if (counted) (void)hipEventDestroy(counted);
+        if (host_counts) (void)hipHostFree(host_counts);
     }
 
     uint32_t chunk_size() const { return static_cast<uint32_t>(shape.items) * kWave; }
@@ -134,8 +139,8 @@ struct Plan {
                 counts.as<uint32_t>(), prefix.as<uint32_t>(),
                 totals.as<uint32_t>(), owner_index.as<uint32_t>(), owner_code.as<uint8_t>(),
                 image_start.as<uint32_t>(), ghost_list.as<uint32_t>(),
-                owner_entry.as<uint32_t>(), desc.size() == 1 && !grid ? host_totals_dev : nullptr,
-                grid ? recv.as<uint32_t>() : nullptr, unit_rows.as<uint32_t>()};
+                owner_entry.as<uint32_t>(), desc.size() == 1 && !grid ? host_counts_dev : nullptr,
+                grid ? recv.as<uint32_t>() : nullptr, unit_rows.as<uint32_t>(), stamp};
     }
 };
 

@@ -162,7 +162,13 @@ struct PlanBuffers {
     uint32_t* image_start;
     uint32_t* ghost_list;
     uint32_t* owner_entry;   // owner slot -> its entry index (chunk * K + rank)
-    uint32_t* host_totals;   // null, or pinned host memory the scan also writes (single box)
+    // Single box (null otherwise): pinned host memory into which the scan also
+    // stores each column total tagged with the build's stamp,
+    // host_counts[col] = stamp << 32 | total, one 8-byte store per column.
+    // The host takes a column once its stamp matches, so no event on the
+    // stream marks the counts (an event between the scan and the route held
+    // the GPU idle for 5.5 us).
+    uint64_t* host_counts;
     // Process grids only (null when every replica sends to itself):
     // recv[r * kTotals + m] = first ghost of receive block m of replica r,
     // recv[R * kTotals] = overflow bits of the whole grid.
@@ -173,6 +179,7 @@ struct PlanBuffers {
     // replica r, and row k = the replica's unit count holds one past each
     // slot's last value, so row k + 1 minus row k counts unit k's images.
     uint32_t* unit_rows;
+    uint32_t stamp;   // this build's tag in host_counts
 };
 
 // Migration between the subdomains of a process grid (grid_kernels.hip).

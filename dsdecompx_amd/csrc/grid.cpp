@@ -29,6 +29,7 @@ struct dsdx_grid : Plan {
     uint32_t* new_owned_host = nullptr;
     uint32_t* new_owned_host_dev = nullptr;
     hipEvent_t migrated = nullptr;   // recorded after the migration counts
+    hipEvent_t counted = nullptr;    // recorded after the ghost-count exchange
     // pinned staging of the replica, chunk and tile tables: the upload does
     // not wait; the next one waits for `staged` before it rewrites the stage
     unsigned char* stage = nullptr;
@@ -41,6 +42,7 @@ struct dsdx_grid : Plan {
         if (new_owned_host) (void)hipHostFree(new_owned_host);
         if (stage) (void)hipHostFree(stage);
         if (migrated) (void)hipEventDestroy(migrated);
+        if (counted) (void)hipEventDestroy(counted);
         if (staged) (void)hipEventDestroy(staged);
     }
 };

@@ -158,10 +158,11 @@ __device__ __forceinline__ void column_done(const PlanBuffers& B, const Layout& 
     // a replica that sends to itself receives what it sends; a process
     // grid's exchange pass overwrites this with the neighbours' sends
     if (col == kColImages) tot[kColReceived] = carry;
-    if (B.host_totals) {
-        B.host_totals[static_cast<uint64_t>(r) * kTotals + col] = carry;
-        if (col == kColImages) B.host_totals[static_cast<uint64_t>(r) * kTotals + kColReceived] = carry;
-    }
+    // one 8-byte store carries the total and the stamp, so the host sees
+    // both or neither
+    if (B.host_counts)
+        __hip_atomic_store(B.host_counts + col, (static_cast<uint64_t>(B.stamp) << 32) | carry, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
     if (B.recv && col == 0 && r == 0) B.recv[static_cast<uint64_t>(L.replicas) * kTotals] = 0u;
 }
 
