@@ -16,6 +16,15 @@
 //                     as the ghost copy's gathers
 //   dense_write13     write the 13 ghost arrays of G ghosts contiguously (the
 //                     ghost copy's writes, were they one run)
+//   gather_write_dense  owner_gather7, then the 13 arrays of one record per
+//                     owner written at consecutive slots after the owned
+//                     particles: the copy's gathers and writes in one pass,
+//                     with no staging and every write in one dense run
+//   owner_rmw_first   owner_rmw_f, adding the force of the owner's first image
+//                     read from where the copy put it (reverse_halo's reads
+//                     and writes for owners with one image)
+//   library_copy,     the library's own ghost copy and reverse_halo on the
+//   library_reverse   same plan, timed the same way
 //
 // Each probe writes a small dense output (one value per owner or per wave) so
 // that its loads are not removed.  The first three are timed twice: back to
@@ -35,6 +44,7 @@
 
 #include "abi_internal.hpp"
 #include "dsdx_hip.h"
+#include "kernels.hpp"
 #include "owners.hpp"
 #include "wave.hpp"
 
@@ -142,6 +152,49 @@ __global__ __launch_bounds__(WAVES * kWave) void owner_gather7(dsdx_particles p,
     });
 }
 
+// The copy's gathers, then one record per owner at slot first + u.
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void gather_write_dense(dsdx_particles p, uint64_t first,
+                                                                    const ReplicaDesc* __restrict__ desc,
+                                                                    const uint32_t* __restrict__ tile_replica,
+                                                                    Layout L, PlanBuffers B)
+{
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        if (!o.live) return;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const uint32_t E = B.owner_entry[o.u];
+        const uint64_t j = first + o.u;
+        const double px = B.bpos[E], py = B.bpos[B.bpos_stride + E], pz = B.bpos[2 * B.bpos_stride + E];
+        const double vx = read_once<true>(p.vx + i), vy = read_once<true>(p.vy + i), vz = read_once<true>(p.vz + i);
+        const double m = read_once<true>(p.mass + i), q = read_once<true>(p.charge + i);
+        const int32_t ty = read_once<true>(p.type + i), id = read_once<true>(p.global_id + i);
+        p.x[j] = px; p.y[j] = py; p.z[j] = pz;
+        p.vx[j] = vx; p.vy[j] = vy; p.vz[j] = vz;
+        p.fx[j] = 0.0; p.fy[j] = 0.0; p.fz[j] = 0.0;
+        p.mass[j] = m; p.charge[j] = q;
+        p.type[j] = ty; p.global_id[j] = id;
+    });
+}
+
+// owner_rmw_f plus the force of the owner's first image.
+template <int WAVES>
+__global__ __launch_bounds__(WAVES * kWave) void owner_rmw_first(double* __restrict__ fx, double* __restrict__ fy,
+                                                                 double* __restrict__ fz,
+                                                                 const ReplicaDesc* __restrict__ desc,
+                                                                 const uint32_t* __restrict__ tile_replica, Layout L,
+                                                                 PlanBuffers B)
+{
+    for_owner_tiles<WAVES>(desc, tile_replica, L, B, [&](const OwnerLane& o) {
+        if (!o.live) return;
+        const uint64_t i = o.d->offset + B.owner_index[o.u];
+        const uint64_t j = o.d->offset + o.d->owned + B.ghost_list[o.d->ghost_list_offset + B.image_start[o.u]];
+        const double gx = fx[j], gy = fy[j], gz = fz[j];
+        fx[i] = fx[i] + gx;
+        fy[i] = fy[i] + gy;
+        fz[i] = fz[i] + gz;
+    });
+}
+
 __global__ __launch_bounds__(256) void dense_write13(dsdx_particles p, uint64_t first, uint64_t count)
 {
     const uint64_t k = blockIdx.x * 256ull + threadIdx.x;
@@ -225,6 +278,10 @@ int main(int argc, char** argv)
         {"owner_rmw_f", 48.0 * owners, 6, 0.0, 0.0},
         {"owner_gather7", 72.0 * owners, -1, 0.0, 0.0},
         {"dense_write13", 96.0 * ghosts, -1, 0.0, 0.0},
+        {"gather_write_dense", 168.0 * owners, -1, 0.0, 0.0},
+        {"owner_rmw_first", 72.0 * owners, 6, 0.0, 0.0},
+        {"library_copy", 168.0 * ghosts, -1, 0.0, 0.0},
+        {"library_reverse", 24.0 * ghosts + 48.0 * owners, 6, 0.0, 0.0},
     };
     auto run = [&](int k) {
         switch (k) {
@@ -236,6 +293,12 @@ int main(int argc, char** argv)
         case 3: hipLaunchKernelGGL((owner_gather7<W>), tiles, dim3(W * kWave), 0, s, p, out, Lay.desc,
                                    Lay.tile_replica, Lay, B); break;
         case 4: hipLaunchKernelGGL(dense_write13, dim3(blocks(ghosts)), dim3(256), 0, s, p, n, ghosts); break;
+        case 5: hipLaunchKernelGGL((gather_write_dense<W>), tiles, dim3(W * kWave), 0, s, p, n, Lay.desc,
+                                   Lay.tile_replica, Lay, B); break;
+        case 6: hipLaunchKernelGGL((owner_rmw_first<W>), tiles, dim3(W * kWave), 0, s, a[6], a[7], a[8], Lay.desc,
+                                   Lay.tile_replica, LD, B); break;
+        case 7: ok(launch_copy(p, FieldParams{}, Lay, B, P->shape, s), "library copy"); break;
+        case 8: ok(launch_reverse_forces(a[6], a[7], a[8], Lay, B, P->shape, s), "library reverse"); break;
         }
         ok(hipGetLastError(), probes[k].name);
     };
