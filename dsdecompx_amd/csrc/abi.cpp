@@ -457,11 +457,20 @@ int read_totals(Plan* plan, hipStream_t stream)
 
 int run_classify(Plan* plan, const dsdx_particles* p, bool wrap_first, hipStream_t s)
 {
-    // a fresh stamp for the scan's host copy of the counts (0 is never used)
-    plan->stamp = plan->stamp + 1 ? plan->stamp + 1 : 1;
-    plan->count_stream = s;
+    // A build the host may wait for takes a fresh stamp for the scan's host
+    // copy of the counts (0 is never used).  A captured build is never waited
+    // for: its scan leaves the host copy alone, so a replay cannot overwrite
+    // the counts of the last build the host waited for.
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    DSDX_HIP(hipStreamIsCapturing(s, &capture), "query stream capture");
+    const bool waitable = capture == hipStreamCaptureStatusNone;
+    if (waitable) {
+        plan->stamp = plan->stamp + 1 ? plan->stamp + 1 : 1;
+        plan->count_stream = s;
+    }
     const Layout L = plan->layout();
-    const PlanBuffers B = plan->buffers();
+    PlanBuffers B = plan->buffers();
+    if (!waitable) B.host_counts = nullptr;
     {
         timing::Scope timed(wrap_first ? timing::kMigrateClassify : timing::kClassify, s);
         DSDX_HIP(launch_classify(p->x, p->y, p->z, wrap_first, L, B, plan->shape, s),
