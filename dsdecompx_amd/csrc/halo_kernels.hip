@@ -332,19 +332,23 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
     const uint32_t chunk = wave_chunk<WAVES>(L, L.chunks);
     if (chunk >= L.chunks) return;
     const uint32_t ebase = chunk * K;
-    // The chunk's prefix of every column (lane c: column c) and its first 64
-    // entries (the entry region always holds K) are loaded with the totals,
-    // before anything waits (18.6 -> 17.1 us at 16 M particles).
-    const uint32_t col_prefix = lane < kCols ? B.prefix[static_cast<uint64_t>(lane) * L.chunks + chunk] : 0u;
-    const uint32_t entry_first = B.entries[ebase + lane];
     const uint32_t r = replica_of(chunk_replica, chunk);
     const ReplicaDesc* d = desc + r;
     uint32_t* totals = B.totals + static_cast<uint64_t>(r) * kTotals;
+    // Every load the chunk needs before its walk goes out together, before
+    // anything waits: the chunk's prefix of every column (lane c: column c),
+    // the replica's totals row (lane w: word w), the chunk's boundary count
+    // and its first 64 entries (the entry region always holds K); 18.6 ->
+    // 17.1 us at 16 M particles with the prefix and entries.
+    const uint32_t col_prefix = lane < kCols ? B.prefix[static_cast<uint64_t>(lane) * L.chunks + chunk] : 0u;
+    const uint32_t total_row = totals[lane & (kTotals - 1)];
+    const uint32_t entry_first = B.entries[ebase + lane];
+    const uint32_t n_boundary = uniform(B.counts[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
     const uint32_t lut = d->bp.lut[lane];
 
     const uint32_t owned = d->owned;
-    const uint32_t received = uniform(totals[kColReceived]);
-    const uint32_t images = uniform(totals[kColImages]);
+    const uint32_t received = static_cast<uint32_t>(__builtin_amdgcn_readlane(total_row, kColReceived));
+    const uint32_t images = static_cast<uint32_t>(__builtin_amdgcn_readlane(total_row, kColImages));
     uint32_t overflow = 0;
     if (static_cast<uint64_t>(owned) + received > d->capacity) overflow |= kOverflowStorage;
     if (images > d->ghost_list_capacity) overflow |= kOverflowGhostList;
@@ -365,7 +369,8 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
         const uint32_t peer = lane < kSlots ? d->dest[lane] : kNoPeer;
         slot_first = peer != kNoPeer ? B.recv[static_cast<uint64_t>(peer) * kTotals + kSlots - 1 - lane] : 0u;
     } else {
-        const uint32_t block_size = lane < kSlots ? totals[kSlots - 1 - lane] : 0u;
+        const uint32_t block_size =
+            static_cast<uint32_t>(__shfl(static_cast<int>(total_row), lane < kSlots ? kSlots - 1 - lane : 0, kWave));
         const uint32_t recv_base = wave_inclusive_sum(block_size) - block_size;
         slot_first = __shfl(recv_base, lane < kSlots ? kSlots - 1 - lane : 0, kWave);
     }
@@ -373,7 +378,6 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
         lane < kSlots ? slot_first + col_prefix : 0u;
     uint32_t slot_run = 0;   // lane s: ghosts of slot s already routed from this chunk
 
-    const uint32_t n_boundary = uniform(B.counts[static_cast<uint64_t>(kColOwners) * L.chunks + chunk]);
     const uint32_t owner_base = static_cast<uint32_t>(__builtin_amdgcn_readlane(col_prefix, kColOwners));
     uint32_t image_run = static_cast<uint32_t>(__builtin_amdgcn_readlane(col_prefix, kColImages));
     const uint32_t first = (chunk - d->first_chunk) * K;
@@ -382,9 +386,9 @@ __global__ __launch_bounds__(WAVES * kWave) void route_kernel(
     uint32_t* rows = B.unit_rows ? B.unit_rows + static_cast<uint64_t>(d->first_unit + r) * kSlots : nullptr;
     if (rows && chunk == d->first_chunk && lane < kSlots) {
         // the end row: one past the replica's last ghost of each slot
-        const uint32_t owners = totals[kColOwners];
+        const uint32_t owners = static_cast<uint32_t>(__builtin_amdgcn_readlane(total_row, kColOwners));
         rows[static_cast<uint64_t>((owners + kRunOwners - 1) / kRunOwners) * kSlots + lane] =
-            slot_first + totals[lane];
+            slot_first + total_row;
     }
 
     for (uint32_t e0 = 0; e0 < n_boundary; e0 += kWave) {
