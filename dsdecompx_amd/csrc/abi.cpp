@@ -415,6 +415,8 @@ uint32_t arrived_columns(const Plan* plan)
 int wait_counts(Plan* plan)
 {
     if (!plan->host_counts) return fail(DSDX_ESTATE, "no halo counts to wait for");
+    if (plan->counts_captured)
+        return fail(DSDX_ESTATE, "the counts of a build captured into a graph cannot be waited for");
     using clock = std::chrono::steady_clock;
     const clock::time_point start = clock::now();
     clock::time_point next_query = start + std::chrono::milliseconds(50);
@@ -468,6 +470,7 @@ int run_classify(Plan* plan, const dsdx_particles* p, bool wrap_first, hipStream
         plan->stamp = plan->stamp + 1 ? plan->stamp + 1 : 1;
         plan->count_stream = s;
     }
+    plan->counts_captured = !waitable;
     const Layout L = plan->layout();
     PlanBuffers B = plan->buffers();
     if (!waitable) B.host_counts = nullptr;

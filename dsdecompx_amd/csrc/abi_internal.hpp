@@ -90,6 +90,7 @@ struct Plan {
     uint64_t* host_counts_dev = nullptr;
     uint32_t stamp = 0;
     hipStream_t count_stream = nullptr;
+    bool counts_captured = false;   // the last classify was captured into a graph
 
     Plan() = default;
     Plan(const Plan&) = delete;

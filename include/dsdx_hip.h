@@ -139,7 +139,9 @@ int dsdx_migrate_wrap(const dsdx_box* box, double* x, double* y, double* z,
  * owned + ghosts > capacity it returns DSDX_ECAPACITY without writing any
  * ghost, and dsdx_halo_scatter may be called with larger storage.  If
  * `ghosts` is null nothing waits; an overflow is recorded in the plan and
- * reported by dsdx_plan_query. */
+ * reported by dsdx_plan_query.  A build captured into a graph must pass a
+ * null `ghosts`: waiting for the counts of a captured build (here, or in
+ * dsdx_plan_query with a stream) returns DSDX_ESTATE. */
 int dsdx_halo_build(dsdx_plan* plan, const dsdx_box* box,
                     const dsdx_particles* particles, const dsdx_field* fields,
                     uint32_t field_count, uint64_t owned, uint64_t capacity,

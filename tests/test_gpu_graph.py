@@ -5,7 +5,8 @@ unfused and fused, and the eager batch and the fused per-box loop, all end
 bit-identical to the plain per-box loop after three steps; and one box
 captured right after dsdx_plan_reserve in global capture mode (any device
 allocation or host wait inside the captured build would fail the capture)
-replays bit-identical to the same steps run eagerly."""
+replays bit-identical to the same steps run eagerly.  A captured single-box
+build that asks for its ghost count fails at once instead of waiting."""
 import json
 import os
 import subprocess
@@ -31,3 +32,42 @@ def test_cpp_captured_graphs_bit_exact():
     assert set(res["differing_words"]) == {"loop_fused", "batch", "graph", "batch_fused", "graph_fused",
                                            "single_box_graph_after_reserve"}
     assert all(v == 0 for v in res["differing_words"].values())
+
+
+def test_captured_build_cannot_wait_for_its_counts():
+    """A single-box build captured into a graph cannot report its ghost
+    count: asked for it, the call returns DSDX_ESTATE at once (nothing waits
+    on counts that a capture never produces), the capture still ends cleanly,
+    and the next eager build of the same plan waits for its own counts and
+    matches the oracle bit for bit."""
+    import ctypes as C
+
+    import torch
+    from dsdecompx_amd import _abi
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    from oracle import pyoracle as po
+    from tests.gpu_util import compare_particles, say, synth, to_device, to_oracle
+
+    n, L, w = 20000, 27.0, 2.8
+    say("captured build with a count output", n)
+    d = synth(n, L, 77)
+    dev = to_device(d, 3 * n)
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
+    s = torch.cuda.Stream()
+    first = halo.exchange_halo(dev, n, stream=s)   # eager: sizes the plan
+    s.synchronize()
+    graph = C.c_void_p()
+    _abi.check(_abi.lib().dsdx_graph_begin(C.c_void_p(s.cuda_stream)), "graph begin")
+    try:
+        with pytest.raises(RuntimeError, match="captured"):
+            halo.exchange_halo(dev, n, stream=s)
+    finally:
+        rc = _abi.lib().dsdx_graph_end(C.c_void_p(s.cuda_stream), C.byref(graph))
+    _abi.check(rc, "graph end")
+    _abi.check(_abi.lib().dsdx_graph_destroy(graph), "graph destroy")
+    ora = po.Oracle(po.Box(upper=(L,) * 3, width=w))
+    op = to_oracle(d, 3 * n)
+    g = ora.exchange_halo(op, n)
+    assert halo.exchange_halo(dev, n, stream=s) == g == first
+    s.synchronize()
+    compare_particles(dev, op, n + g, "eager build after a captured one")
