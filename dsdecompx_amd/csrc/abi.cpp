@@ -859,6 +859,10 @@ int dsdx_batch_create(const dsdx_replica* replicas, uint32_t count, void* stream
     if (!stream) return fail(DSDX_EINVAL, "batch setup requires an explicit stream");
     dsdx_batch* b = new (std::nothrow) dsdx_batch();
     if (!b) return fail(DSDX_EDEVICE, "out of host memory");
+    // Over a batch of small boxes the fused migrate + classify pass walks
+    // last to first (512 boxes of 2,048-262,144 particles: graph step 2.300
+    // against 2.331 ms first to last); DSDX_DESCENDING overrides.
+    b->shape.descending |= kOrderFused;
     int rc = shape_from_env(&b->shape);
     b->desc.resize(count);
     uint64_t ghost_list = 0;

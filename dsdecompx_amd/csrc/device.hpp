@@ -93,6 +93,7 @@ constexpr int kOrderRoute = 4;
 constexpr int kOrderCopy = 8;
 constexpr int kOrderOwners = 16;
 constexpr int kOrderMigrateScatter = 32;   // process grids: the migration's record moves
+constexpr int kOrderFused = 64;            // the fused migrate + classify pass
 
 // Launch shape of the classify / scatter / owner-major kernels: items per
 // lane (the chunk is kWave * items particles) and waves per block.
@@ -101,15 +102,17 @@ struct Shape {
     int waves = 4;
     int scan_threads = 1024;
     int xcd_run = 16;   // consecutive blocks per XCD turn (0: hardware order)
-    // Bit mask of passes that walk their chunks / tiles last to first.  The
-    // first pass over x, y, z after the application wrote them (migrate, or
-    // the fused migrate + classify) walks backwards: the most recently
-    // written lines are the ones still held by the 256 MB memory-side cache.
-    // The classify pass that follows migrate then walks forwards, into the
-    // lines migrate read last.  The owner-major passes after the rebuild
-    // (reverse, refresh, field exchanges) walk backwards for the same reason:
-    // the application has just written the forces (or positions) in index
-    // order, and the highest owners' lines are the cached ones.
+    // Bit mask of passes that walk their chunks / tiles last to first.
+    // migrate, the first pass over x, y, z after the application wrote them,
+    // walks backwards: the most recently written lines are the ones still
+    // held by the 256 MB memory-side cache.  The classify pass that follows
+    // migrate then walks forwards, into the lines migrate read last.  The
+    // owner-major passes after the rebuild (reverse, refresh, field
+    // exchanges) walk backwards for the same reason: the application has just
+    // written the forces (or positions) in index order, and the highest
+    // owners' lines are the cached ones.  The fused migrate + classify pass,
+    // which takes its chunk in rounds of 8 items, measured faster walking
+    // forwards (16 M particles: 71.0 against 74.9 us).
     int descending = kOrderMigrate | kOrderOwners;
     // Ghost writes staged through LDS (copy_runs_kernel): -1 when the owned
     // records exceed the memory-side cache, 0 never, 1 always.

@@ -917,7 +917,7 @@ bool shape_supported(const Shape& s)
 hipError_t launch_classify(double* x, double* y, double* z, bool wrap_first, const Layout& L,
                            const PlanBuffers& B, const Shape& s, hipStream_t stream)
 {
-    const Layout O = ordered(L, s, wrap_first ? kOrderMigrate : kOrderClassify);
+    const Layout O = ordered(L, s, wrap_first ? kOrderFused : kOrderClassify);
     if (L.chunks == 0) return hipSuccess;
     return dispatch(s, [&](auto items, auto waves) {
         return dispatch_stream(L.streaming, [&](auto streaming) {

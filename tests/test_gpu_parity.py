@@ -323,7 +323,7 @@ def test_one_plan_across_sizes(gpu):
         compare_particles(dev, op, n + g, f"sizes n={n}")
 
 
-@pytest.mark.parametrize("items,waves,descending", [(16, 4, 63), (4, 2, 0), (8, 8, 24)])
+@pytest.mark.parametrize("items,waves,descending", [(16, 4, 127), (4, 2, 0), (8, 8, 24)])
 def test_launch_shapes_and_walk_orders(gpu, monkeypatch, items, waves, descending):
     """Launch shape and walk order (DSDX_ITEMS, DSDX_WAVES, DSDX_DESCENDING,
     read when a plan is made) change no result: migrate, build, reverse and
@@ -363,15 +363,18 @@ def test_launch_shapes_and_walk_orders(gpu, monkeypatch, items, waves, descendin
     compare_particles(dev, op, n + g, f"refresh {items}x{waves} order {descending}")
 
 
+@pytest.mark.parametrize("fused_order", [0, 64], ids=["first_to_last", "last_to_first"])
 @pytest.mark.parametrize("case", [c for c in CASES if c[0] in ("uniform_32k", "z_nonperiodic",
                                                                  "shifted_box", "empty")],
                          ids=lambda c: c[0])
-def test_fused_migrate_build_equals_separate(gpu, case):
-    """dsdx_migrate_halo_build == dsdx_migrate_wrap + dsdx_halo_build, bit for bit."""
+def test_fused_migrate_build_equals_separate(gpu, monkeypatch, case, fused_order):
+    """dsdx_migrate_halo_build == dsdx_migrate_wrap + dsdx_halo_build, bit for bit,
+    with the fused pass walking its chunks either way (DSDX_DESCENDING bit 64)."""
     import torch
     from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    monkeypatch.setenv("DSDX_DESCENDING", str(1 | 16 | fused_order))
     name, n, L, w, per, edges, lower = case
-    say("fused", name)
+    say("fused", name, fused_order)
     upper = tuple(lower + L for _ in range(3))
     box = BoxConfig(lower=(lower,) * 3, upper=upper, periodic=per, width=w, edges_and_corners=edges)
     d = synth(n, L, 77, lower=lower)

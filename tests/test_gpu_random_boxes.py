@@ -39,7 +39,7 @@ def _case(seed):
     edges = bool(rng.uniform() < 0.8)
     sigma = float(rng.uniform(0.0, 0.5))
     shape = (int(rng.choice([4, 8, 16])), int(rng.choice([2, 4, 8])), int(rng.choice([256, 1024])),
-             int(rng.integers(0, 64)))
+             int(rng.integers(0, 128)))
     return n, L, w, lower, periodic, edges, sigma, shape
 
 
