@@ -71,3 +71,41 @@ def test_captured_build_cannot_wait_for_its_counts():
     assert halo.exchange_halo(dev, n, stream=s) == g == first
     s.synchronize()
     compare_particles(dev, op, n + g, "eager build after a captured one")
+
+
+def test_count_wait_behind_a_long_kernel():
+    """The host waits for a build's counts by polling the stamped words the
+    scan stores; after 50 ms it also asks the stream, every 10 ms, whether it
+    failed or drained.  A build queued behind a few hundred ms of other work on
+    the same stream takes that path and still returns the oracle's ghost
+    count and ghosts."""
+    import time
+
+    import torch
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    from oracle import pyoracle as po
+    from tests.gpu_util import compare_particles, say, synth, to_device, to_oracle
+
+    n, L, w = 20000, 27.0, 2.8
+    say("count wait behind a long kernel", n)
+    d = synth(n, L, 78)
+    dev = to_device(d, 3 * n)
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
+    s = torch.cuda.Stream()
+    ora = po.Oracle(po.Box(upper=(L,) * 3, width=w))
+    op = to_oracle(d, 3 * n)
+    g = ora.exchange_halo(op, n)
+    # a spin kernel of a few hundred ms on this GPU (calibrated roughly)
+    with torch.cuda.stream(s):
+        t0 = time.perf_counter()
+        torch.cuda._sleep(10_000_000)
+        s.synchronize()
+        per_cycle = (time.perf_counter() - t0) / 10_000_000
+        torch.cuda._sleep(int(0.4 / per_cycle))
+    t0 = time.perf_counter()
+    assert halo.exchange_halo(dev, n, stream=s) == g
+    waited = time.perf_counter() - t0
+    s.synchronize()
+    say("waited", round(waited * 1e3, 1), "ms")
+    assert waited > 0.05, "the build did not wait behind the spin kernel"
+    compare_particles(dev, op, n + g, "build behind a long kernel")
