@@ -102,18 +102,19 @@ struct Shape {
     int waves = 4;
     int scan_threads = 1024;
     int xcd_run = 16;   // consecutive blocks per XCD turn (0: hardware order)
-    // Bit mask of passes that walk their chunks / tiles last to first.
-    // migrate, the first pass over x, y, z after the application wrote them,
-    // walks backwards: the most recently written lines are the ones still
-    // held by the 256 MB memory-side cache.  The classify pass that follows
-    // migrate then walks forwards, into the lines migrate read last.  The
-    // owner-major passes after the rebuild (reverse, refresh, field
-    // exchanges) walk backwards for the same reason: the application has just
-    // written the forces (or positions) in index order, and the highest
-    // owners' lines are the cached ones.  The fused migrate + classify pass,
-    // which takes its chunk in rounds of 8 items, measured faster walking
-    // forwards (16 M particles: 71.0 against 74.9 us).
-    int descending = kOrderMigrate | kOrderOwners;
+    // Bit mask of passes that walk their chunks / tiles last to first, each
+    // choice measured at 16 M particles.  migrate, the first pass over x, y,
+    // z after the application wrote them, walks backwards: the most recently
+    // written lines are the ones still held by the 256 MB memory-side cache
+    // (70.8 against 97 us forwards).  The classify pass that follows walks
+    // backwards too (64.4 against 65.9 us).  The owner-major passes after the
+    // rebuild (reverse, refresh, field exchanges) walk backwards for the
+    // reason migrate does: the application has just written the forces (or
+    // positions) in index order, and the highest owners' lines are the cached
+    // ones (reverse 95 against 111 us).  The fused migrate + classify pass,
+    // which takes its chunk in rounds of 8 items, walks forwards (71.0
+    // against 74.9 us).
+    int descending = kOrderMigrate | kOrderClassify | kOrderOwners;
     // Ghost writes staged through LDS (copy_runs_kernel): -1 when the owned
     // records exceed the memory-side cache, 0 never, 1 always.
     int ghost_runs = -1;
