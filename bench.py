@@ -489,8 +489,8 @@ def measure_batched(args):
 # is scanned by scan_parts_kernel, a longer one by scan_kernel).
 DEVICE_KERNELS = {"migrate_wrap": ("wrap_box_kernel", "migrate_wrap_kernel"),
                   "halo_classify": ("classify_kernel",), "halo_scan": ("scan_parts_kernel", "scan_kernel"),
-                  "halo_scatter": ("route_kernel",), "halo_copy": ("copy_runs_kernel", "copy_kernel"),
-                  "reverse_halo": ("reverse_runs_kernel", "reverse_forces_kernel")}
+                  "halo_scatter": ("route_kernel",), "halo_copy": ("copy_chunks_kernel", "copy_runs_kernel", "copy_kernel"),
+                  "reverse_halo": ("reverse_chunks_kernel", "reverse_runs_kernel", "reverse_forces_kernel")}
 CALIBRATION_BYTES = 1 << 30
 
 

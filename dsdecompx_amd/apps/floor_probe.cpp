@@ -257,6 +257,9 @@ int main(int argc, char** argv)
     dsdx_ok(dsdx_halo_build(plan, &box, &p, nullptr, 0, n, cap, s, &ghosts), "halo build");
     dsdx_ok(dsdx_plan_query(plan, nullptr, nullptr, nullptr, &owners, nullptr, nullptr), "query");
     const abi::Plan* P = plan;
+    // the probes walk the owner maps, which a build over chunk units leaves
+    // to the route pass
+    dsdx_ok(abi::ensure_routed(P, s), "route");
     const Layout Lay = P->layout();
     const PlanBuffers B = P->buffers();
     Layout LD = Lay;

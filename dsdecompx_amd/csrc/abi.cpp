@@ -367,6 +367,7 @@ int set_single(Plan* plan, const BoxParams& bp, uint64_t owned, uint64_t capacit
                uint64_t ghost_list_elems, hipStream_t stream)
 {
     apply_auto_items(plan, owned);
+    plan->single_box = true;
     ReplicaDesc d;
     std::memset(&d, 0, sizeof(d));
     d.bp = bp;
@@ -491,7 +492,10 @@ int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipS
 {
     const Layout L = plan->layout();
     const PlanBuffers B = plan->buffers();
-    {
+    // the copy over chunk units places every ghost from the entries and
+    // prefixes: the route waits until a call needs the owner maps
+    plan->routed = L.unit_chunks == 0;
+    if (plan->routed) {
         timing::Scope timed(timing::kScatter, s);
         DSDX_HIP(launch_route(L, B, plan->shape, s), "launch halo route");
     }
@@ -499,6 +503,23 @@ int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipS
         timing::Scope timed(timing::kCopy, s);
         DSDX_HIP(launch_copy(*p, fp, L, B, plan->shape, s), "launch halo copy");
     }
+    return DSDX_OK;
+}
+
+// The owner maps and ghost list of the last build, for the calls that walk
+// them; a no-op once written.  The route derives everything from the build's
+// entries, prefixes and totals, which stay on the device until the next
+// build, so it may run any time after the build on the same stream order.
+int ensure_routed(const Plan* plan, hipStream_t s)
+{
+    if (plan->routed || !plan->classified) return DSDX_OK;
+    Layout L = plan->layout();
+    L.unit_chunks = 0;
+    {
+        timing::Scope timed(timing::kScatter, s);
+        DSDX_HIP(launch_route(L, plan->buffers(), plan->shape, s), "launch halo route");
+    }
+    plan->routed = true;
     return DSDX_OK;
 }
 
@@ -511,6 +532,7 @@ int shape_from_env(Shape* s)
     if (const char* v = std::getenv("DSDX_DESCENDING")) s->descending = std::atoi(v);
     if (const char* v = std::getenv("DSDX_GHOST_RUNS")) s->ghost_runs = std::atoi(v);
     if (const char* v = std::getenv("DSDX_REVERSE_RUNS")) s->reverse_runs = std::atoi(v);
+    if (const char* v = std::getenv("DSDX_UNIT_OWNERS")) s->unit_owners = std::atoi(v);
     return shape_supported(*s) ? DSDX_OK : fail(DSDX_EINVAL, "unsupported launch shape");
 }
 
@@ -753,6 +775,7 @@ int dsdx_halo_refresh(dsdx_plan* plan, const dsdx_box* box, const dsdx_particles
             if (int rc = upload_table(plan, s)) return rc;
         }
     }
+    if (int rc = ensure_routed(plan, s)) return rc;
     timing::Scope timed(timing::kRefresh, s);
     DSDX_HIP(launch_refresh(*particles, fp, positions_only ? 1 : 0, plan->layout(), plan->buffers(),
                             plan->shape, s),
@@ -791,6 +814,7 @@ int dsdx_plan_ghost_owners(const dsdx_plan* plan, int32_t* owners, void* stream)
     if (!plan->built) return DSDX_OK;
     if (!stream || !owners) return fail(DSDX_EINVAL, "ghost owner query needs output and stream");
     const hipStream_t s = as_stream(stream);
+    if (int rc = ensure_routed(plan, s)) return rc;
     timing::Scope timed(timing::kGhostOwners, s);
     DSDX_HIP(launch_ghost_owners(owners, plan->layout(), plan->buffers(), plan->shape, s),
              "launch ghost owners");
@@ -807,9 +831,11 @@ int dsdx_reverse_halo(const dsdx_plan* plan, double* fx, double* fy, double* fz,
         return fail(DSDX_EINVAL, "reverse exchange owned count differs from the last device halo");
     if (!fx || !fy || !fz) return fail(DSDX_EINVAL, "invalid device reverse-halo view");
     const hipStream_t s = as_stream(stream);
+    const Layout L = plan->layout();
+    if (!chunk_reverse(L, plan->shape))
+        if (int rc = ensure_routed(plan, s)) return rc;
     timing::Scope timed(timing::kReverse, s);
-    DSDX_HIP(launch_reverse_forces(fx, fy, fz, plan->layout(), plan->buffers(), plan->shape, s),
-             "launch reverse halo");
+    DSDX_HIP(launch_reverse_forces(fx, fy, fz, L, plan->buffers(), plan->shape, s), "launch reverse halo");
     return DSDX_OK;
 }
 
@@ -824,6 +850,7 @@ int dsdx_reverse_exchange(const dsdx_plan* plan, const dsdx_field* fields, uint3
     if (!single_matches(plan, owned))
         return fail(DSDX_EINVAL, "reverse exchange owned count differs from the last device halo");
     const hipStream_t s = as_stream(stream);
+    if (int rc = ensure_routed(plan, s)) return rc;
     timing::Scope timed(timing::kReverseFields, s);
     DSDX_HIP(launch_reverse_fields(tf, plan->layout(), plan->buffers(), plan->shape, s),
              "launch reverse exchange");
@@ -841,6 +868,7 @@ int dsdx_forward_exchange(const dsdx_plan* plan, const dsdx_field* fields, uint3
     if (!single_matches(plan, owned))
         return fail(DSDX_EINVAL, "forward exchange owned count differs from the last device halo");
     const hipStream_t s = as_stream(stream);
+    if (int rc = ensure_routed(plan, s)) return rc;
     timing::Scope timed(timing::kForward, s);
     DSDX_HIP(launch_forward_fields(fp, plan->layout(), plan->buffers(), plan->shape, s),
              "launch forward exchange");

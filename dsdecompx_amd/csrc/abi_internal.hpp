@@ -63,6 +63,7 @@ struct Plan {
     void note_live_tiles()
     {
         if (desc.size() > static_cast<size_t>(kWave) || !host_totals) return;
+        counts_seen = true;
         uint32_t n = 0;
         for (size_t r = 0; r < desc.size(); ++r) {
             const uint32_t* t = host_totals + r * kTotals;
@@ -70,6 +71,17 @@ struct Plan {
         }
         live_tiles = n;
     }
+
+    // Set by set_single (the single-box entry points): such a plan's copy and
+    // reverse may take the chunk units (Layout::unit_chunks).
+    bool single_box = false;
+    // The owner maps and ghost list of the last build are written.  A build
+    // over chunk units leaves them out; ensure_routed writes them for the
+    // calls that walk them (refresh, field exchanges, ghost owners).
+    mutable bool routed = true;
+    // host_totals hold the counts of some earlier build (the chunk units are
+    // sized from them).
+    bool counts_seen = false;
 
     bool classified = false;   // classify + scan of the current table are on the device
     bool built = false;        // ghosts and owner map written: reverse/forward usable
@@ -130,7 +142,25 @@ struct Plan {
         L.ghost_runs = shape.ghost_runs < 0 ? L.streaming : static_cast<uint32_t>(shape.ghost_runs != 0);
         L.xcd_run = static_cast<uint32_t>(shape.xcd_run);
         L.descending = 0;
+        L.unit_chunks = 0;
+        if (single_box && !grid && desc.size() == 1 && shape.unit_owners > 0 && L.ghost_runs && unit_rows.ptr &&
+            chunks)
+            L.unit_chunks = unit_chunks();
         return L;
+    }
+
+    // Chunks per unit: about shape.unit_owners owners (and twice as many
+    // images) by the mean of the last counts the host read, 1 to 64 chunks;
+    // 8 before any count is known.  Any value gives the same results.
+    uint32_t unit_chunks() const
+    {
+        if (!counts_seen) return 8;
+        const uint64_t owners = host_totals[kColOwners], images = host_totals[kColImages];
+        const uint64_t target = static_cast<uint64_t>(shape.unit_owners);
+        uint64_t g = 64;
+        if (owners) g = std::min<uint64_t>(g, target * chunks / owners);
+        if (images) g = std::min<uint64_t>(g, 2 * target * chunks / images);
+        return static_cast<uint32_t>(std::max<uint64_t>(1, g));
     }
 
     PlanBuffers buffers() const
@@ -160,6 +190,7 @@ int read_totals(Plan* plan, hipStream_t stream);
 int wait_counts(Plan* plan);
 int run_classify(Plan* plan, const dsdx_particles* p, bool wrap_first, hipStream_t s);
 int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipStream_t s);
+int ensure_routed(const Plan* plan, hipStream_t s);
 int shape_from_env(Shape* s);
 
 }  // namespace dsdx::abi

@@ -122,6 +122,11 @@ struct Shape {
     // reverse_halo with the ghost forces read as dense runs (1), per owner
     // (0), or as the ghost writes go (-1: runs wherever the copy stages them).
     int reverse_runs = -1;
+    // One box whose ghost writes take the staged runs: the copy and
+    // reverse_halo walk units of whole chunks holding about this many
+    // boundary owners, and the build runs no route pass (0: units of owner
+    // tiles after a route pass, as batches and grids do).
+    int unit_owners = 544;
 };
 
 // Owner tiles: the owner slots [0, bound_r) of replica r (relative to
@@ -153,6 +158,8 @@ struct Layout {
     uint32_t ghost_runs;            // ghost writes through copy_runs_kernel (Shape::ghost_runs)
     uint32_t xcd_run;               // block renumbering (see xcd_block)
     uint32_t descending;            // this launch walks its chunks / tiles last to first
+    uint32_t unit_chunks;           // one box: chunks per unit of the route-free copy and reverse
+                                    // (copy_chunks_kernel); 0: owner-tile units after a route pass
 };
 
 struct PlanBuffers {

@@ -10,6 +10,7 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "chunk_units.hpp"
 #include "owners.hpp"
 #include "runs.hpp"
 #include "wave.hpp"
@@ -516,8 +517,8 @@ constexpr int kRunsBuild = 0;
 constexpr int kRunsRefresh = 1;
 constexpr int kRunsPositions = 2;
 
-template <int MODE>
-__device__ __forceinline__ void write_run_ghost(const dsdx_particles& p, const RunStage& sh, uint32_t ol,
+template <int MODE, class Stage>
+__device__ __forceinline__ void write_run_ghost(const dsdx_particles& p, const Stage& sh, uint32_t ol,
                                                 uint32_t s, uint64_t j)
 {
     p.x[j] = sh.px[ol] + sh.table.shift[s][0];
@@ -616,6 +617,249 @@ __global__ __launch_bounds__(kRunWaves * kWave) void reverse_runs_kernel(double*
                               gz = fz[j];
                           });
     });
+}
+
+// ---------------------------------------------------------------------------
+// The rebuild's copy and reverse_halo of one box over chunk units
+// (chunk_units.hpp): every ghost position comes from the entries and the
+// prefixes, so the build skips the route pass; the owner maps and the ghost
+// list are routed only when a later call needs them.
+
+struct ChunkStage {
+    double px[kChunkOwners], py[kChunkOwners], pz[kChunkOwners];
+    double vx[kChunkOwners], vy[kChunkOwners], vz[kChunkOwners], m[kChunkOwners], q[kChunkOwners];
+    int32_t ty[kChunkOwners], id[kChunkOwners];
+    uint16_t map[kChunkMap];   // run index -> owner (index in the unit)
+    SlotTable table;
+};
+
+// One workgroup per unit.  Wave w ranks the owners of the unit's chunks w, w
+// + kRunWaves, ... per slot, gathers their records (positions from the
+// classification pass) into LDS and maps each image's run index to its
+// owner; after one barrier lane k of a round writes run index k.  A unit
+// that does not fit the stage writes each owner's images from its lane.
+template <bool STREAM>
+__global__ __launch_bounds__(kRunWaves * kWave) void copy_chunks_kernel(dsdx_particles p, FieldParams fp,
+                                                                        const ReplicaDesc* __restrict__ desc,
+                                                                        Layout L, PlanBuffers B)
+{
+    __shared__ ChunkStage sh;
+    const uint32_t units = chunk_units_of(L);
+    const uint32_t b = xcd_block(L.xcd_run);
+    if (b >= units) return;
+    const uint32_t unit = L.descending ? units - 1u - b : b;
+    const ReplicaDesc* d = desc;
+    const uint32_t lane = lane_id();
+    const uint32_t w = uniform(threadIdx.x / kWave);
+    if (w == 0) fill_slot_table(sh.table, desc, d);
+    const ChunkUnit u = chunk_unit(d, unit, L, B);
+    if (u.overflow) {
+        // the status word the route pass sets; nothing is copied
+        if (unit == 0 && threadIdx.x == 0) B.totals[kStatusWord] = u.overflow;
+        return;
+    }
+    const bool staged = u.owners <= kChunkOwners && u.runs.images <= kChunkMap;
+    const uint32_t lut = d->bp.lut[lane];
+    const uint32_t K = L.chunk_size;
+    for (uint32_t c = u.c0 + w; c < u.c1; c += kRunWaves) {
+        const ChunkHead h = chunk_head(c, L, B);
+        const uint64_t ebase = static_cast<uint64_t>(c) * K;
+        const uint64_t first = d->offset + static_cast<uint64_t>(c - d->first_chunk) * K;
+        const uint32_t ol0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(h.pc, kColOwners)) - u.owner0;
+        const uint32_t rbase = u.runs.start + (h.pc - u.pre0);   // lane s: run index of the chunk's first image through s
+        const uint32_t gbase = u.slot_first + h.pc;              // lane s: its ghost-list value
+        uint32_t run = 0;
+        for (uint32_t e0 = 0; e0 < h.n; e0 += kChunkRounds * kWave) {
+            // every load of the rounds first, then each round's ranks and stores
+            uint32_t entry[kChunkRounds];
+            double x[kChunkRounds], y[kChunkRounds], z[kChunkRounds], vx[kChunkRounds], vy[kChunkRounds],
+                vz[kChunkRounds], m[kChunkRounds], q[kChunkRounds];
+            int32_t ty[kChunkRounds], id[kChunkRounds];
+#pragma unroll
+            for (uint32_t k = 0; k < kChunkRounds; ++k) {
+                entry[k] = chunk_entry(h, e0, k, ebase, B);
+                x[k] = y[k] = z[k] = vx[k] = vy[k] = vz[k] = m[k] = q[k] = 0.0;
+                ty[k] = id[k] = 0;
+                const uint32_t e = e0 + k * kWave + lane;
+                if (e < h.n) {
+                    const uint64_t i = first + (entry[k] >> 8);
+                    x[k] = B.bpos[ebase + e];
+                    y[k] = B.bpos[B.bpos_stride + ebase + e];
+                    z[k] = B.bpos[2 * B.bpos_stride + ebase + e];
+                    vx[k] = read_once<STREAM>(p.vx + i);
+                    vy[k] = read_once<STREAM>(p.vy + i);
+                    vz[k] = read_once<STREAM>(p.vz + i);
+                    m[k] = read_once<STREAM>(p.mass + i);
+                    q[k] = read_once<STREAM>(p.charge + i);
+                    ty[k] = read_once<STREAM>(p.type + i);
+                    id[k] = read_once<STREAM>(p.global_id + i);
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kChunkRounds; ++k) {
+                if (e0 + k * kWave >= h.n) break;   // wave-uniform
+                const uint32_t e = e0 + k * kWave + lane;
+                const bool valid = e < h.n;
+                const uint32_t code_slots = lane_value(lut, entry[k] & 63u);
+                const uint32_t slots = valid ? code_slots : 0u;
+                const uint64_t i = first + (entry[k] >> 8);
+                const uint32_t ol = ol0 + e;
+                if (staged && valid) {
+                    sh.px[ol] = x[k]; sh.py[ol] = y[k]; sh.pz[ol] = z[k];
+                    sh.vx[ol] = vx[k]; sh.vy[ol] = vy[k]; sh.vz[ol] = vz[k];
+                    sh.m[ol] = m[k]; sh.q[ol] = q[k]; sh.ty[ol] = ty[k]; sh.id[ol] = id[k];
+                }
+                rank_round(slots, run, [&](uint32_t s, uint32_t r, bool mine) {
+                    const uint32_t rb = static_cast<uint32_t>(__builtin_amdgcn_readlane(rbase, s));
+                    const uint32_t g = static_cast<uint32_t>(__builtin_amdgcn_readlane(gbase, s));
+                    if (!mine) return;
+                    const uint64_t j = ghost_slot(desc, d, s, g + r);
+                    if (staged) {
+                        sh.map[rb + r] = static_cast<uint16_t>(ol);
+                    } else {
+                        p.x[j] = x[k] + d->bp.shift[s][0];
+                        p.y[j] = y[k] + d->bp.shift[s][1];
+                        p.z[j] = z[k] + d->bp.shift[s][2];
+                        if (p.fx) p.fx[j] = 0.0;
+                        if (p.fy) p.fy[j] = 0.0;
+                        if (p.fz) p.fz[j] = 0.0;
+                        p.vx[j] = vx[k]; p.vy[j] = vy[k]; p.vz[j] = vz[k];
+                        p.mass[j] = m[k]; p.charge[j] = q[k];
+                        p.type[j] = ty[k]; p.global_id[j] = id[k];
+                    }
+                    if (fp.count) copy_field_words(fp, i, j);
+                });
+            }
+        }
+    }
+    if (!staged) return;
+    __syncthreads();
+    // lane k of a round writes run index k (every lane takes part in the
+    // shuffles of each round)
+    for (uint32_t r0 = 0; r0 < u.runs.images; r0 += kRunWaves * kWave) {
+        const uint32_t r = r0 + threadIdx.x;
+        const uint32_t s = run_slot(u.runs.start, r);
+        const uint32_t rel = lane_value(u.runs.first, s) + (r - lane_value(u.runs.start, s));
+        if (r < u.runs.images) write_run_ghost<kRunsBuild>(p, sh, sh.map[r], s, sh.table.ghost0[s] + rel);
+    }
+}
+
+struct ChunkReverseStage {
+    double gx[kChunkMap], gy[kChunkMap], gz[kChunkMap];
+};
+
+// reverse_halo over chunk units: the unit's ghost forces come into LDS as
+// dense runs; wave w then walks its chunks' owners, ranks them per slot with
+// ballots and adds each owner's images in ascending send slot, the order of
+// the per-owner kernel.  The wave's first chunk and the forces of its first
+// kChunkRounds * 64 owners are loaded before the ghost forces are staged, so
+// both sets of loads are in flight together.  A unit with more images than
+// LDS holds reads each image's force from its ghost.
+__global__ __launch_bounds__(kRunWaves * kWave) void reverse_chunks_kernel(double* __restrict__ fx,
+                                                                           double* __restrict__ fy,
+                                                                           double* __restrict__ fz,
+                                                                           const ReplicaDesc* __restrict__ desc,
+                                                                           Layout L, PlanBuffers B)
+{
+    __shared__ ChunkReverseStage sh;
+    const uint32_t units = chunk_units_of(L);
+    const uint32_t b = xcd_block(L.xcd_run);
+    if (b >= units) return;
+    const uint32_t unit = L.descending ? units - 1u - b : b;
+    const ReplicaDesc* d = desc;
+    const uint32_t lane = lane_id();
+    const uint32_t w = uniform(threadIdx.x / kWave);
+    const ChunkUnit u = chunk_unit(d, unit, L, B);
+    if (u.overflow) return;
+    const bool staged = u.runs.images <= kChunkMap;
+    const uint32_t lut = d->bp.lut[lane];
+    const uint32_t K = L.chunk_size;
+    uint32_t c = u.c0 + w;
+    ChunkHead h{};
+    double hx[kChunkRounds], hy[kChunkRounds], hz[kChunkRounds];
+#pragma unroll
+    for (uint32_t k = 0; k < kChunkRounds; ++k) hx[k] = hy[k] = hz[k] = 0.0;
+    if (c < u.c1) {
+        h = chunk_head(c, L, B);
+        const uint64_t first = d->offset + static_cast<uint64_t>(c - d->first_chunk) * K;
+#pragma unroll
+        for (uint32_t k = 0; k < kChunkRounds; ++k) {
+            if (k * kWave + lane < h.n) {
+                const uint64_t i = first + (h.entry[k] >> 8);
+                hx[k] = fx[i];
+                hy[k] = fy[i];
+                hz[k] = fz[i];
+            }
+        }
+    }
+    if (staged) {
+        for (uint32_t r0 = 0; r0 < u.runs.images; r0 += kRunWaves * kWave) {
+            const uint32_t r = r0 + threadIdx.x;
+            const uint32_t s = run_slot(u.runs.start, r);
+            const uint32_t rel = lane_value(u.runs.first, s) + (r - lane_value(u.runs.start, s));
+            if (r < u.runs.images) {
+                const uint64_t j = ghost_slot(desc, d, s, rel);
+                sh.gx[r] = fx[j];
+                sh.gy[r] = fy[j];
+                sh.gz[r] = fz[j];
+            }
+        }
+        __syncthreads();
+    }
+    for (bool head = true; c < u.c1; c += kRunWaves, head = false) {
+        if (!head) h = chunk_head(c, L, B);
+        const uint64_t ebase = static_cast<uint64_t>(c) * K;
+        const uint64_t first = d->offset + static_cast<uint64_t>(c - d->first_chunk) * K;
+        const uint32_t rbase = u.runs.start + (h.pc - u.pre0);
+        const uint32_t gbase = u.slot_first + h.pc;
+        uint32_t run = 0;
+        for (uint32_t e0 = 0; e0 < h.n; e0 += kChunkRounds * kWave) {
+            uint32_t entry[kChunkRounds];
+            double ax[kChunkRounds], ay[kChunkRounds], az[kChunkRounds];
+#pragma unroll
+            for (uint32_t k = 0; k < kChunkRounds; ++k) {
+                entry[k] = chunk_entry(h, e0, k, ebase, B);
+                ax[k] = hx[k];
+                ay[k] = hy[k];
+                az[k] = hz[k];
+                if (!(head && e0 == 0) && e0 + k * kWave + lane < h.n) {
+                    const uint64_t i = first + (entry[k] >> 8);
+                    ax[k] = fx[i];
+                    ay[k] = fy[i];
+                    az[k] = fz[i];
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kChunkRounds; ++k) {
+                if (e0 + k * kWave >= h.n) break;   // wave-uniform
+                const bool valid = e0 + k * kWave + lane < h.n;
+                const uint32_t code_slots = lane_value(lut, entry[k] & 63u);
+                const uint32_t slots = valid ? code_slots : 0u;
+                double sx = ax[k], sy = ay[k], sz = az[k];
+                rank_round(slots, run, [&](uint32_t s, uint32_t r, bool mine) {
+                    const uint32_t rb = static_cast<uint32_t>(__builtin_amdgcn_readlane(rbase, s));
+                    const uint32_t g = static_cast<uint32_t>(__builtin_amdgcn_readlane(gbase, s));
+                    if (!mine) return;
+                    if (staged) {
+                        sx = sx + sh.gx[rb + r];
+                        sy = sy + sh.gy[rb + r];
+                        sz = sz + sh.gz[rb + r];
+                    } else {
+                        const uint64_t j = ghost_slot(desc, d, s, g + r);
+                        sx = sx + fx[j];
+                        sy = sy + fy[j];
+                        sz = sz + fz[j];
+                    }
+                });
+                if (valid) {
+                    const uint64_t i = first + (entry[k] >> 8);
+                    fx[i] = sx;
+                    fy[i] = sy;
+                    fz[i] = sz;
+                }
+            }
+        }
+    }
 }
 
 // Up to C components at once: one pass over the images, each image's
@@ -908,7 +1152,8 @@ bool shape_supported(const Shape& s)
 {
     return (s.items == 4 || s.items == 8 || s.items == 16) &&
            (s.waves == 2 || s.waves == 4 || s.waves == 8) &&
-           (s.scan_threads == 256 || s.scan_threads == 512 || s.scan_threads == 1024);
+           (s.scan_threads == 256 || s.scan_threads == 512 || s.scan_threads == 1024) &&
+           s.unit_owners >= 0 && s.unit_owners <= 1024;
 }
 
 // ---------------------------------------------------------------------------
@@ -972,6 +1217,12 @@ hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Lay
 {
     const Layout O = ordered(L, s, kOrderCopy);
     if (L.tiles == 0) return hipSuccess;
+    if (L.unit_chunks)
+        return dispatch_stream(L.streaming, [&](auto streaming) {
+            constexpr bool S = decltype(streaming)::value;
+            return launch(copy_chunks_kernel<S>, dim3((L.chunks + L.unit_chunks - 1) / L.unit_chunks),
+                          dim3(kRunWaves * kWave), stream, p, fp, L.desc, O, B);
+        });
     // Each workgroup walks units g, g + grid, ... up to the device's owner count.
     if (ghost_runs(L, B))
         return dispatch_stream(L.streaming, [&](auto streaming) {
@@ -994,6 +1245,9 @@ hipError_t launch_reverse_forces(double* fx, double* fy, double* fz, const Layou
 {
     const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
+    if (chunk_reverse(L, s))
+        return launch(reverse_chunks_kernel, dim3((L.chunks + L.unit_chunks - 1) / L.unit_chunks),
+                      dim3(kRunWaves * kWave), stream, fx, fy, fz, L.desc, O, B);
     if (B.unit_rows && (s.reverse_runs < 0 ? ghost_runs(L, B) : s.reverse_runs > 0))
         return launch(reverse_runs_kernel, run_grid(L), dim3(kRunWaves * kWave), stream, fx, fy, fz, L.desc,
                       L.unit_replica, O, B);

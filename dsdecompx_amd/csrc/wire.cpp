@@ -116,6 +116,7 @@ int dsdx_halo_build_records(dsdx_plan* plan, const dsdx_box* box, const dsdx_par
         timing::Scope timed(timing::kScatter, s);
         DSDX_HIP(launch_route(plan->layout(), plan->buffers(), plan->shape, s), "launch halo route");
     }
+    plan->routed = true;
     plan->host_totals[kStatusWord] = 0;
     plan->built = true;
     *records = g;
@@ -150,6 +151,7 @@ int dsdx_halo_pack(const dsdx_plan* plan, const dsdx_particles* particles, const
                                     DSDX_WIRE_PARTICLE, false, &rf))
         return rc;
     const hipStream_t s = as_stream(stream);
+    if (int rc = ensure_routed(plan, s)) return rc;
     timing::Scope timed(timing::kPack, s);
     DSDX_HIP(launch_pack_halo(*particles, rf, positions_only ? 1 : 0, 0, static_cast<unsigned char*>(send),
                               plan->layout(), plan->buffers(), plan->shape, s),
@@ -195,6 +197,7 @@ int dsdx_accumulate_reverse_records(const dsdx_plan* plan, const void* records, 
 {
     if (int rc = check_plan(plan, owned)) return rc;
     if (!stream || !records || !fx || !fy || !fz) return fail(DSDX_EINVAL, "invalid reverse accumulate");
+    if (int rc = ensure_routed(plan, as_stream(stream))) return rc;
     timing::Scope timed(timing::kAccumulate, as_stream(stream));
     DSDX_HIP(launch_accumulate_reverse(static_cast<const unsigned char*>(records), fx, fy, fz, plan->layout(),
                                        plan->buffers(), plan->shape, as_stream(stream)),
@@ -233,6 +236,7 @@ int dsdx_gather_field_records(const dsdx_plan* plan, const dsdx_field* fields, u
     if (!stream || (field_count && !out)) return fail(DSDX_EINVAL, "invalid field record gather");
     RecordFields rf;
     if (int rc = make_record_fields(fields, field_count, DSDX_WIRE_EXCHANGE, false, &rf)) return rc;
+    if (int rc = ensure_routed(plan, as_stream(stream))) return rc;
     DSDX_HIP(launch_gather_field_records(rf, static_cast<unsigned char*>(out), plan->layout(), plan->buffers(),
                                          plan->shape, as_stream(stream)),
              "launch field record gather");
@@ -246,6 +250,7 @@ int dsdx_reduce_field_records(const dsdx_plan* plan, const dsdx_field* fields, u
     if (!stream || (field_count && !records)) return fail(DSDX_EINVAL, "invalid field record reduce");
     RecordFields rf;
     if (int rc = make_record_fields(fields, field_count, DSDX_WIRE_EXCHANGE, true, &rf)) return rc;
+    if (int rc = ensure_routed(plan, as_stream(stream))) return rc;
     DSDX_HIP(launch_reduce_field_records(rf, static_cast<const unsigned char*>(records), plan->layout(),
                                          plan->buffers(), plan->shape, as_stream(stream)),
              "launch field record reduce");

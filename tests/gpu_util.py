@@ -96,3 +96,16 @@ def ghost_writes(request, monkeypatch):
     batch or grid is created."""
     monkeypatch.setenv("DSDX_GHOST_RUNS", "1" if request.param == "runs" else "0")
     return request.param
+
+
+@pytest.fixture(autouse=True, params=["per_owner", "runs", "tile_runs"])
+def box_ghost_writes(request, monkeypatch):
+    """The single-box modules' form of ``ghost_writes``, with a third case.
+    With the staged runs a single box's build copies, and reverse_halo
+    reduces, over units of whole chunks with no route pass ("runs", the
+    default); "tile_runs" (DSDX_UNIT_OWNERS=0) takes the units of owner tiles
+    after a route pass, as batches and process grids do."""
+    monkeypatch.setenv("DSDX_GHOST_RUNS", "0" if request.param == "per_owner" else "1")
+    if request.param == "tile_runs":
+        monkeypatch.setenv("DSDX_UNIT_OWNERS", "0")
+    return request.param

@@ -24,7 +24,7 @@ from oracle import pyoracle as po
 from tests.gpu_util import F64, I32, assert_bits_equal, compare_particles, host, say, synth, \
     to_device, to_oracle
 
-from tests.gpu_util import ghost_writes  # noqa: F401  (autouse: both ghost-write kernels)
+from tests.gpu_util import box_ghost_writes  # noqa: F401  (autouse: every ghost-write kernel)
 pytestmark = pytest.mark.gpu
 
 
