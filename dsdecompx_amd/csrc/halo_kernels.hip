@@ -653,6 +653,9 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_chunks_kernel(dsdx_par
     const uint32_t w = uniform(threadIdx.x / kWave);
     if (w == 0) fill_slot_table(sh.table, desc, d);
     const ChunkUnit u = chunk_unit(d, unit, L, B);
+    // a single box sends every slot to itself: its ghosts follow its owned
+    // particles
+    const uint64_t ghost0 = d->offset + d->owned;
     if (u.overflow) {
         // the status word the route pass sets; nothing is copied
         if (unit == 0 && threadIdx.x == 0) B.totals[kStatusWord] = u.overflow;
@@ -713,7 +716,7 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_chunks_kernel(dsdx_par
                     const uint32_t rb = static_cast<uint32_t>(__builtin_amdgcn_readlane(rbase, s));
                     const uint32_t g = static_cast<uint32_t>(__builtin_amdgcn_readlane(gbase, s));
                     if (!mine) return;
-                    const uint64_t j = ghost_slot(desc, d, s, g + r);
+                    const uint64_t j = ghost0 + (g + r);
                     if (staged) {
                         sh.map[rb + r] = static_cast<uint16_t>(ol);
                     } else {
@@ -744,17 +747,22 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_chunks_kernel(dsdx_par
     }
 }
 
+// Ghost forces of a unit staged by reverse_chunks_kernel: kReverseRounds
+// rounds of one value per thread.
+constexpr uint32_t kReverseRounds = 3;
+constexpr uint32_t kReverseStage = kReverseRounds * kRunWaves * kWave;
+
 struct ChunkReverseStage {
-    double gx[kChunkMap], gy[kChunkMap], gz[kChunkMap];
+    double gx[kReverseStage], gy[kReverseStage], gz[kReverseStage];
 };
 
 // reverse_halo over chunk units: the unit's ghost forces come into LDS as
 // dense runs; wave w then walks its chunks' owners, ranks them per slot with
 // ballots and adds each owner's images in ascending send slot, the order of
-// the per-owner kernel.  The wave's first chunk and the forces of its first
-// kChunkRounds * 64 owners are loaded before the ghost forces are staged, so
-// both sets of loads are in flight together.  A unit with more images than
-// LDS holds reads each image's force from its ghost.
+// the per-owner kernel.  The forces of the wave's first kChunkRounds * 64
+// owners are loaded before the barrier that publishes the staged ghost
+// forces.  A unit with more images than LDS holds reads each image's force
+// from its ghost.
 __global__ __launch_bounds__(kRunWaves * kWave) void reverse_chunks_kernel(double* __restrict__ fx,
                                                                            double* __restrict__ fy,
                                                                            double* __restrict__ fz,
@@ -770,42 +778,59 @@ __global__ __launch_bounds__(kRunWaves * kWave) void reverse_chunks_kernel(doubl
     const uint32_t lane = lane_id();
     const uint32_t w = uniform(threadIdx.x / kWave);
     const ChunkUnit u = chunk_unit(d, unit, L, B);
+    const uint64_t ghost0 = d->offset + d->owned;   // every slot sends to the box itself
     if (u.overflow) return;
-    const bool staged = u.runs.images <= kChunkMap;
+    const bool staged = u.runs.images <= kReverseStage;
     const uint32_t lut = d->bp.lut[lane];
     const uint32_t K = L.chunk_size;
+    // Loads in this order: the wave's first chunk (its entries), the unit's
+    // ghost forces, the forces of the chunk's first owners; only then are the
+    // ghost forces written to LDS.  A wave waits for its loads in issue
+    // order, so the owner loads wait for the entries only and stay in flight
+    // across the barrier.
+    // Every load up to the barrier is issued on one straight path (a lane or
+    // wave without work reads the box's or the chunk's first particle), so
+    // the compiler counts them exactly and each wait covers only what it
+    // needs.
     uint32_t c = u.c0 + w;
-    ChunkHead h{};
-    double hx[kChunkRounds], hy[kChunkRounds], hz[kChunkRounds];
+    ChunkHead h = chunk_head(min(c, u.c1 - 1u), L, B);
+    if (c >= u.c1) h.n = 0;
+    double gx[kReverseRounds], gy[kReverseRounds], gz[kReverseRounds];
 #pragma unroll
-    for (uint32_t k = 0; k < kChunkRounds; ++k) hx[k] = hy[k] = hz[k] = 0.0;
-    if (c < u.c1) {
-        h = chunk_head(c, L, B);
-        const uint64_t first = d->offset + static_cast<uint64_t>(c - d->first_chunk) * K;
+    for (uint32_t k = 0; k < kReverseRounds; ++k) {
+        const uint32_t r = k * kRunWaves * kWave + threadIdx.x;
+        const uint32_t s = run_slot(u.runs.start, r);
+        const uint32_t rel = lane_value(u.runs.first, s) + (r - lane_value(u.runs.start, s));
+        const uint64_t j = staged && r < u.runs.images ? ghost0 + rel : d->offset;
+        gx[k] = fx[j];
+        gy[k] = fy[j];
+        gz[k] = fz[j];
+    }
+    // keeps the scheduler from moving the owner loads above the ghost loads
+    __builtin_amdgcn_sched_barrier(0);
+    double hx[kChunkRounds], hy[kChunkRounds], hz[kChunkRounds];
+    {
+        const uint64_t first = d->offset + static_cast<uint64_t>(min(c, u.c1 - 1u) - d->first_chunk) * K;
 #pragma unroll
         for (uint32_t k = 0; k < kChunkRounds; ++k) {
-            if (k * kWave + lane < h.n) {
-                const uint64_t i = first + (h.entry[k] >> 8);
-                hx[k] = fx[i];
-                hy[k] = fy[i];
-                hz[k] = fz[i];
-            }
+            const uint32_t entry = k * kWave + lane < h.n ? h.entry[k] : 0u;
+            const uint64_t i = first + (entry >> 8);
+            hx[k] = fx[i];
+            hy[k] = fy[i];
+            hz[k] = fz[i];
         }
     }
-    if (staged) {
-        for (uint32_t r0 = 0; r0 < u.runs.images; r0 += kRunWaves * kWave) {
-            const uint32_t r = r0 + threadIdx.x;
-            const uint32_t s = run_slot(u.runs.start, r);
-            const uint32_t rel = lane_value(u.runs.first, s) + (r - lane_value(u.runs.start, s));
-            if (r < u.runs.images) {
-                const uint64_t j = ghost_slot(desc, d, s, rel);
-                sh.gx[r] = fx[j];
-                sh.gy[r] = fy[j];
-                sh.gz[r] = fz[j];
-            }
-        }
-        __syncthreads();
+    __builtin_amdgcn_sched_barrier(0);
+    // every stage slot is written, used or not: a write only where used would
+    // let the compiler sink the ghost loads to it, behind the owner loads
+#pragma unroll
+    for (uint32_t k = 0; k < kReverseRounds; ++k) {
+        const uint32_t r = k * kRunWaves * kWave + threadIdx.x;
+        sh.gx[r] = gx[k];
+        sh.gy[r] = gy[k];
+        sh.gz[r] = gz[k];
     }
+    __syncthreads();
     for (bool head = true; c < u.c1; c += kRunWaves, head = false) {
         if (!head) h = chunk_head(c, L, B);
         const uint64_t ebase = static_cast<uint64_t>(c) * K;
@@ -845,7 +870,7 @@ __global__ __launch_bounds__(kRunWaves * kWave) void reverse_chunks_kernel(doubl
                         sy = sy + sh.gy[rb + r];
                         sz = sz + sh.gz[rb + r];
                     } else {
-                        const uint64_t j = ghost_slot(desc, d, s, g + r);
+                        const uint64_t j = ghost0 + (g + r);
                         sx = sx + fx[j];
                         sy = sy + fy[j];
                         sz = sz + fz[j];
