@@ -213,6 +213,7 @@ int reserve_buffers(Plan* plan, uint64_t ghost_list_elems)
     // a replica of owned_r particles has at most ceil(owned_r / kRunOwners)
     // ghost-run units, plus its end row
     DSDX_HIP(plan->unit_rows.reserve((slots / kRunOwners + 2 * replicas) * kSlots * 4), "hipMalloc");
+    DSDX_HIP(plan->chunk_rows.reserve(c * kTotals * 4), "hipMalloc");
     if (ghost_list_elems > plan->ghost_list_capacity) {
         DSDX_HIP(plan->ghost_list.reserve(ghost_list_elems * 4), "hipMalloc");
         plan->ghost_list_capacity = plan->ghost_list.bytes / 4;
@@ -495,6 +496,7 @@ int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipS
     // the copy over chunk units places every ghost from the entries and
     // prefixes: the route waits until a call needs the owner maps
     plan->routed = L.unit_chunks == 0;
+    plan->rows_built = L.unit_chunks != 0;
     if (plan->routed) {
         timing::Scope timed(timing::kScatter, s);
         DSDX_HIP(launch_route(L, B, plan->shape, s), "launch halo route");
@@ -831,7 +833,8 @@ int dsdx_reverse_halo(const dsdx_plan* plan, double* fx, double* fy, double* fz,
         return fail(DSDX_EINVAL, "reverse exchange owned count differs from the last device halo");
     if (!fx || !fy || !fz) return fail(DSDX_EINVAL, "invalid device reverse-halo view");
     const hipStream_t s = as_stream(stream);
-    const Layout L = plan->layout();
+    Layout L = plan->layout();
+    if (!plan->rows_built) L.reverse_unit_chunks = 0;
     if (!chunk_reverse(L, plan->shape))
         if (int rc = ensure_routed(plan, s)) return rc;
     timing::Scope timed(timing::kReverse, s);

@@ -79,6 +79,9 @@ struct Plan {
     // over chunk units leaves them out; ensure_routed writes them for the
     // calls that walk them (refresh, field exchanges, ghost owners).
     mutable bool routed = true;
+    // The last build's copy took the chunk units and stored the chunk rows
+    // (PlanBuffers::chunk_rows) that reverse_halo over chunk units reads.
+    bool rows_built = false;
     // host_totals hold the counts of some earlier build (the chunk units are
     // sized from them).
     bool counts_seen = false;
@@ -91,6 +94,7 @@ struct Plan {
         ghost_list, owner_entry;
     DeviceBuffer recv;   // process grids only: receive-block bases and the grid status
     DeviceBuffer unit_rows;   // first ghost-list value of each slot per ghost-run unit
+    DeviceBuffer chunk_rows;  // one box over chunk units: each chunk's prefix row
     bool grid = false;   // replicas send to each other (dsdx_grid), not to themselves
     uint64_t ghost_list_capacity = 0;   // elements
     uint32_t* host_totals = nullptr;    // pinned copy of totals (R x kTotals)
@@ -142,21 +146,24 @@ struct Plan {
         L.ghost_runs = shape.ghost_runs < 0 ? L.streaming : static_cast<uint32_t>(shape.ghost_runs != 0);
         L.xcd_run = static_cast<uint32_t>(shape.xcd_run);
         L.descending = 0;
-        L.unit_chunks = 0;
+        L.unit_chunks = L.reverse_unit_chunks = 0;
         if (single_box && !grid && desc.size() == 1 && shape.unit_owners > 0 && L.ghost_runs && unit_rows.ptr &&
-            chunks)
-            L.unit_chunks = unit_chunks();
+            chunk_rows.ptr && chunks) {
+            const uint32_t target = static_cast<uint32_t>(shape.unit_owners);
+            L.unit_chunks = unit_chunks(target);
+            L.reverse_unit_chunks = unit_chunks(std::min(target, kReverseUnitOwners));
+        }
         return L;
     }
 
-    // Chunks per unit: about shape.unit_owners owners (and twice as many
-    // images) by the mean of the last counts the host read, 1 to 64 chunks;
-    // 8 before any count is known.  Any value gives the same results.
-    uint32_t unit_chunks() const
+    // Chunks per unit: about `target` owners (and twice as many images) by
+    // the mean of the last counts the host read, 1 to 64 chunks; 4 before any
+    // count is known.  Any value gives the same results.
+    uint32_t unit_chunks(uint32_t target_owners) const
     {
-        if (!counts_seen) return 8;
+        if (!counts_seen) return 4;
         const uint64_t owners = host_totals[kColOwners], images = host_totals[kColImages];
-        const uint64_t target = static_cast<uint64_t>(shape.unit_owners);
+        const uint64_t target = target_owners;
         uint64_t g = 64;
         if (owners) g = std::min<uint64_t>(g, target * chunks / owners);
         if (images) g = std::min<uint64_t>(g, 2 * target * chunks / images);
@@ -171,7 +178,8 @@ struct Plan {
                 totals.as<uint32_t>(), owner_index.as<uint32_t>(), owner_code.as<uint8_t>(),
                 image_start.as<uint32_t>(), ghost_list.as<uint32_t>(),
                 owner_entry.as<uint32_t>(), desc.size() == 1 && !grid ? host_counts_dev : nullptr,
-                grid ? recv.as<uint32_t>() : nullptr, unit_rows.as<uint32_t>(), stamp};
+                grid ? recv.as<uint32_t>() : nullptr, unit_rows.as<uint32_t>(), stamp,
+                chunk_rows.as<uint32_t>()};
     }
 };
 

@@ -48,9 +48,11 @@ __device__ __forceinline__ uint32_t chunk_units_of(const Layout& L)
     return (L.chunks + L.unit_chunks - 1) / L.unit_chunks;
 }
 
-// Every lane of the wave must be active.
+// Every lane of the wave must be active.  rows: the chunks' prefix rows
+// (PlanBuffers::chunk_rows, once the copy has stored them), else the prefix
+// columns are read.
 __device__ __forceinline__ ChunkUnit chunk_unit(const ReplicaDesc* d, uint32_t unit, const Layout& L,
-                                                const PlanBuffers& B)
+                                                const PlanBuffers& B, const uint32_t* rows = nullptr)
 {
     const uint32_t lane = lane_id();
     ChunkUnit u;
@@ -58,9 +60,15 @@ __device__ __forceinline__ ChunkUnit chunk_unit(const ReplicaDesc* d, uint32_t u
     u.c1 = min(L.chunks, u.c0 + L.unit_chunks);
     const uint64_t col = lane < kCols ? lane : 0u;
     const uint32_t total_row = B.totals[lane & (kTotals - 1)];
-    u.pre0 = B.prefix[col * L.chunks + u.c0];
     // past the last chunk the prefix is the column total
-    const uint32_t pre1 = u.c1 < L.chunks ? B.prefix[col * L.chunks + u.c1] : total_row;
+    uint32_t pre1;
+    if (rows) {
+        u.pre0 = rows[static_cast<uint64_t>(u.c0) * kTotals + col];
+        pre1 = u.c1 < L.chunks ? rows[static_cast<uint64_t>(u.c1) * kTotals + col] : total_row;
+    } else {
+        u.pre0 = B.prefix[col * L.chunks + u.c0];
+        pre1 = u.c1 < L.chunks ? B.prefix[col * L.chunks + u.c1] : total_row;
+    }
     const uint32_t received = static_cast<uint32_t>(__builtin_amdgcn_readlane(total_row, kColReceived));
     const uint32_t images = static_cast<uint32_t>(__builtin_amdgcn_readlane(total_row, kColImages));
     u.overflow = 0;
@@ -100,6 +108,21 @@ __device__ __forceinline__ ChunkHead chunk_head(uint32_t c, const Layout& L, con
     const uint64_t col = lane < kCols ? lane : 0u;
     ChunkHead h;
     h.pc = B.prefix[col * L.chunks + c];
+#pragma unroll
+    for (uint32_t q = 0; q < kChunkRounds; ++q)
+        h.entry[q] = B.entries[static_cast<uint64_t>(c) * L.chunk_size + q * kWave + lane];
+    h.n = uniform(B.counts[static_cast<uint64_t>(kColOwners) * L.chunks + c]);
+    return h;
+}
+
+// The part of a chunk's head the tile-lane reverse needs: the owners before
+// the chunk (uniform, in pc, from its prefix row), its owner count and its
+// first entries.
+__device__ __forceinline__ ChunkHead chunk_owner_head(uint32_t c, const Layout& L, const PlanBuffers& B)
+{
+    const uint32_t lane = lane_id();
+    ChunkHead h;
+    h.pc = uniform(B.chunk_rows[static_cast<uint64_t>(c) * kTotals + kColOwners]);
 #pragma unroll
     for (uint32_t q = 0; q < kChunkRounds; ++q)
         h.entry[q] = B.entries[static_cast<uint64_t>(c) * L.chunk_size + q * kWave + lane];

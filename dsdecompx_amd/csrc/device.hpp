@@ -158,9 +158,15 @@ struct Layout {
     uint32_t ghost_runs;            // ghost writes through copy_runs_kernel (Shape::ghost_runs)
     uint32_t xcd_run;               // block renumbering (see xcd_block)
     uint32_t descending;            // this launch walks its chunks / tiles last to first
-    uint32_t unit_chunks;           // one box: chunks per unit of the route-free copy and reverse
+    uint32_t unit_chunks;           // one box: chunks per unit of the route-free copy
                                     // (copy_chunks_kernel); 0: owner-tile units after a route pass
+    uint32_t reverse_unit_chunks;   // the same for reverse_halo (reverse_chunks_kernel), whose
+                                    // units hold at most kRunOwners owners
 };
+
+// The owners a unit of reverse_chunks_kernel is sized for: its stage holds
+// kRunOwners, and the owners of a unit of whole chunks vary about the mean.
+constexpr uint32_t kReverseUnitOwners = 7 * kRunOwners / 8;
 
 struct PlanBuffers {
     uint32_t* entries;
@@ -192,6 +198,11 @@ struct PlanBuffers {
     // slot's last value, so row k + 1 minus row k counts unit k's images.
     uint32_t* unit_rows;
     uint32_t stamp;   // this build's tag in host_counts
+    // One box over chunk units (null: not kept): the copy stores each
+    // chunk's prefix of all columns as one row, chunk_rows[chunk * kTotals +
+    // col] = prefix[col][chunk], for reverse_halo, which then reads a unit's
+    // bounds from two rows instead of two columns of 28 lines each.
+    uint32_t* chunk_rows;
 };
 
 // Migration between the subdomains of a process grid (grid_kernels.hip).

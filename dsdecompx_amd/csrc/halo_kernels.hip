@@ -666,6 +666,7 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_chunks_kernel(dsdx_par
     const uint32_t K = L.chunk_size;
     for (uint32_t c = u.c0 + w; c < u.c1; c += kRunWaves) {
         const ChunkHead h = chunk_head(c, L, B);
+        if (lane < kCols) B.chunk_rows[static_cast<uint64_t>(c) * kTotals + lane] = h.pc;
         const uint64_t ebase = static_cast<uint64_t>(c) * K;
         const uint64_t first = d->offset + static_cast<uint64_t>(c - d->first_chunk) * K;
         const uint32_t ol0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(h.pc, kColOwners)) - u.owner0;
@@ -747,22 +748,76 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_chunks_kernel(dsdx_par
     }
 }
 
-// Ghost forces of a unit staged by reverse_chunks_kernel: kReverseRounds
-// rounds of one value per thread.
-constexpr uint32_t kReverseRounds = 3;
+// reverse_halo over chunk units, one lane per owner reading each image's
+// force from its ghost: the units whose owners or images exceed the LDS
+// stage of reverse_chunks_kernel.  Wave w walks the unit's chunks w, w +
+// kRunWaves, ... and ranks each round of 64 owners per slot with ballots.
+__device__ __forceinline__ void reverse_chunk_walk(double* __restrict__ fx, double* __restrict__ fy,
+                                                   double* __restrict__ fz, const ReplicaDesc* d,
+                                                   const ChunkUnit& u, uint64_t ghost0, uint32_t lut,
+                                                   const Layout& L, const PlanBuffers& B)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t w = uniform(threadIdx.x / kWave);
+    const uint32_t K = L.chunk_size;
+    for (uint32_t c = u.c0 + w; c < u.c1; c += kRunWaves) {
+        const ChunkHead h = chunk_head(c, L, B);
+        const uint64_t ebase = static_cast<uint64_t>(c) * K;
+        const uint64_t first = d->offset + static_cast<uint64_t>(c - d->first_chunk) * K;
+        const uint32_t gbase = u.slot_first + h.pc;   // lane s: ghost-list value of the chunk's first image through s
+        uint32_t run = 0;
+        for (uint32_t e0 = 0; e0 < h.n; e0 += kWave) {
+            const uint32_t e = e0 + lane;
+            const bool valid = e < h.n;
+            const uint32_t entry = valid ? (e0 == 0 ? h.entry[0] : B.entries[ebase + e]) : 0u;
+            const uint32_t code_slots = lane_value(lut, entry & 63u);
+            const uint32_t slots = valid ? code_slots : 0u;
+            const uint64_t i = first + (entry >> 8);
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            if (valid) {
+                ax = fx[i];
+                ay = fy[i];
+                az = fz[i];
+            }
+            rank_round(slots, run, [&](uint32_t s, uint32_t r, bool mine) {
+                const uint32_t g = static_cast<uint32_t>(__builtin_amdgcn_readlane(gbase, s));
+                if (!mine) return;
+                const uint64_t j = ghost0 + g + r;
+                ax = ax + fx[j];
+                ay = ay + fy[j];
+                az = az + fz[j];
+            });
+            if (valid) {
+                fx[i] = ax;
+                fy[i] = ay;
+                fz[i] = az;
+            }
+        }
+    }
+}
+
+// The stage of reverse_chunks_kernel: a unit of at most kRunOwners owners
+// (kRunTiles tiles of 64) and kReverseStage images.
+constexpr uint32_t kReverseRounds = 2;
 constexpr uint32_t kReverseStage = kReverseRounds * kRunWaves * kWave;
 
 struct ChunkReverseStage {
     double gx[kReverseStage], gy[kReverseStage], gz[kReverseStage];
+    uint32_t entry[kRunOwners];   // the unit's entries in owner order
+    uint32_t at[kRunOwners];      // their entry indices (chunk * chunk_size + rank)
+    uint32_t tile_count[kRunTiles][kSlots];   // owners of each tile with each slot
 };
 
 // reverse_halo over chunk units: the unit's ghost forces come into LDS as
-// dense runs; wave w then walks its chunks' owners, ranks them per slot with
-// ballots and adds each owner's images in ascending send slot, the order of
-// the per-owner kernel.  The forces of the wave's first kChunkRounds * 64
-// owners are loaded before the barrier that publishes the staged ghost
-// forces.  A unit with more images than LDS holds reads each image's force
-// from its ghost.
+// dense runs; each wave copies its chunk's entries into LDS in owner order,
+// and after a barrier wave t takes owners 64 t .. 64 t + 63 of the unit (one
+// full tile instead of a chunk and the few owners past its first 64), counts
+// them per slot with ballots, and after a second barrier adds each owner's
+// images in ascending send slot, the order of the per-owner kernel.  The
+// loads go out in this order: the chunk's entries, the ghost forces, then,
+// after the first barrier, the owner forces, which stay in flight while the
+// ghost forces are written to LDS (a wave waits for its loads in issue
+// order).  Units that exceed the stage: reverse_chunk_walk.
 __global__ __launch_bounds__(kRunWaves * kWave) void reverse_chunks_kernel(double* __restrict__ fx,
                                                                            double* __restrict__ fy,
                                                                            double* __restrict__ fz,
@@ -777,52 +832,63 @@ __global__ __launch_bounds__(kRunWaves * kWave) void reverse_chunks_kernel(doubl
     const ReplicaDesc* d = desc;
     const uint32_t lane = lane_id();
     const uint32_t w = uniform(threadIdx.x / kWave);
-    const ChunkUnit u = chunk_unit(d, unit, L, B);
+    const ChunkUnit u = chunk_unit(d, unit, L, B, B.chunk_rows);
     const uint64_t ghost0 = d->offset + d->owned;   // every slot sends to the box itself
     if (u.overflow) return;
-    const bool staged = u.runs.images <= kReverseStage;
     const uint32_t lut = d->bp.lut[lane];
+    if (u.owners > kRunOwners || u.runs.images > kReverseStage) {
+        reverse_chunk_walk(fx, fy, fz, d, u, ghost0, lut, L, B);
+        return;
+    }
     const uint32_t K = L.chunk_size;
-    // Loads in this order: the wave's first chunk (its entries), the unit's
-    // ghost forces, the forces of the chunk's first owners; only then are the
-    // ghost forces written to LDS.  A wave waits for its loads in issue
-    // order, so the owner loads wait for the entries only and stay in flight
-    // across the barrier.
-    // Every load up to the barrier is issued on one straight path (a lane or
-    // wave without work reads the box's or the chunk's first particle), so
-    // the compiler counts them exactly and each wait covers only what it
-    // needs.
-    uint32_t c = u.c0 + w;
-    ChunkHead h = chunk_head(min(c, u.c1 - 1u), L, B);
+    // the wave's chunk (a wave past the unit's chunks reads the last one)
+    const uint32_t c = u.c0 + w;
+    ChunkHead h = chunk_owner_head(min(c, u.c1 - 1u), L, B);
     if (c >= u.c1) h.n = 0;
+    // the ghost forces, unconditionally (an idle lane reads the box's first
+    // particle), so that the compiler counts the loads exactly
     double gx[kReverseRounds], gy[kReverseRounds], gz[kReverseRounds];
 #pragma unroll
     for (uint32_t k = 0; k < kReverseRounds; ++k) {
         const uint32_t r = k * kRunWaves * kWave + threadIdx.x;
         const uint32_t s = run_slot(u.runs.start, r);
         const uint32_t rel = lane_value(u.runs.first, s) + (r - lane_value(u.runs.start, s));
-        const uint64_t j = staged && r < u.runs.images ? ghost0 + rel : d->offset;
+        const uint64_t j = r < u.runs.images ? ghost0 + rel : d->offset;
         gx[k] = fx[j];
         gy[k] = fy[j];
         gz[k] = fz[j];
     }
-    // keeps the scheduler from moving the owner loads above the ghost loads
     __builtin_amdgcn_sched_barrier(0);
-    double hx[kChunkRounds], hy[kChunkRounds], hz[kChunkRounds];
-    {
-        const uint64_t first = d->offset + static_cast<uint64_t>(min(c, u.c1 - 1u) - d->first_chunk) * K;
-#pragma unroll
-        for (uint32_t k = 0; k < kChunkRounds; ++k) {
-            const uint32_t entry = k * kWave + lane < h.n ? h.entry[k] : 0u;
-            const uint64_t i = first + (entry >> 8);
-            hx[k] = fx[i];
-            hy[k] = fy[i];
-            hz[k] = fz[i];
+    // the unit's entries into LDS in owner order (a unit of more chunks than
+    // waves, or a chunk of more than kChunkRounds * 64 owners, loads the rest)
+    for (uint32_t cc = c; cc < u.c1; cc += kRunWaves) {
+        if (cc != c) h = chunk_owner_head(cc, L, B);
+        const uint32_t o0 = h.pc - u.owner0;
+        const uint64_t ebase = static_cast<uint64_t>(cc) * K;
+        for (uint32_t e0 = 0; e0 < h.n; e0 += kWave) {
+            const uint32_t e = e0 + lane;
+            if (e < h.n) {
+                sh.entry[o0 + e] = e0 == 0 ? h.entry[0] : e0 == kWave ? h.entry[1] : B.entries[ebase + e];
+                sh.at[o0 + e] = static_cast<uint32_t>(ebase + e);
+            }
         }
     }
+    __syncthreads();
+    // tile w of the unit
+    const uint32_t o = w * kWave + lane;
+    const bool live = o < u.owners;
+    const uint32_t entry = live ? sh.entry[o] : 0u;
+    const uint32_t at = live ? sh.at[o] : u.c0 * K;
+    const uint64_t i = d->offset + (at & ~(K - 1u)) - static_cast<uint64_t>(d->first_chunk) * K + (entry >> 8);
+    double ax = fx[i], ay = fy[i], az = fz[i];
+    const uint32_t code_slots = lane_value(lut, entry & 63u);
+    const uint32_t slots = live ? code_slots : 0u;
+    const uint32_t present = uniform(wave_or(slots));
+    const uint32_t mine = tile_slot_count(slots, present);
+    if (lane < kSlots) sh.tile_count[w][lane] = mine;
     __builtin_amdgcn_sched_barrier(0);
     // every stage slot is written, used or not: a write only where used would
-    // let the compiler sink the ghost loads to it, behind the owner loads
+    // let the compiler sink the ghost loads to it
 #pragma unroll
     for (uint32_t k = 0; k < kReverseRounds; ++k) {
         const uint32_t r = k * kRunWaves * kWave + threadIdx.x;
@@ -831,59 +897,22 @@ __global__ __launch_bounds__(kRunWaves * kWave) void reverse_chunks_kernel(doubl
         sh.gz[r] = gz[k];
     }
     __syncthreads();
-    for (bool head = true; c < u.c1; c += kRunWaves, head = false) {
-        if (!head) h = chunk_head(c, L, B);
-        const uint64_t ebase = static_cast<uint64_t>(c) * K;
-        const uint64_t first = d->offset + static_cast<uint64_t>(c - d->first_chunk) * K;
-        const uint32_t rbase = u.runs.start + (h.pc - u.pre0);
-        const uint32_t gbase = u.slot_first + h.pc;
-        uint32_t run = 0;
-        for (uint32_t e0 = 0; e0 < h.n; e0 += kChunkRounds * kWave) {
-            uint32_t entry[kChunkRounds];
-            double ax[kChunkRounds], ay[kChunkRounds], az[kChunkRounds];
-#pragma unroll
-            for (uint32_t k = 0; k < kChunkRounds; ++k) {
-                entry[k] = chunk_entry(h, e0, k, ebase, B);
-                ax[k] = hx[k];
-                ay[k] = hy[k];
-                az[k] = hz[k];
-                if (!(head && e0 == 0) && e0 + k * kWave + lane < h.n) {
-                    const uint64_t i = first + (entry[k] >> 8);
-                    ax[k] = fx[i];
-                    ay[k] = fy[i];
-                    az[k] = fz[i];
-                }
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < kChunkRounds; ++k) {
-                if (e0 + k * kWave >= h.n) break;   // wave-uniform
-                const bool valid = e0 + k * kWave + lane < h.n;
-                const uint32_t code_slots = lane_value(lut, entry[k] & 63u);
-                const uint32_t slots = valid ? code_slots : 0u;
-                double sx = ax[k], sy = ay[k], sz = az[k];
-                rank_round(slots, run, [&](uint32_t s, uint32_t r, bool mine) {
-                    const uint32_t rb = static_cast<uint32_t>(__builtin_amdgcn_readlane(rbase, s));
-                    const uint32_t g = static_cast<uint32_t>(__builtin_amdgcn_readlane(gbase, s));
-                    if (!mine) return;
-                    if (staged) {
-                        sx = sx + sh.gx[rb + r];
-                        sy = sy + sh.gy[rb + r];
-                        sz = sz + sh.gz[rb + r];
-                    } else {
-                        const uint64_t j = ghost0 + (g + r);
-                        sx = sx + fx[j];
-                        sy = sy + fy[j];
-                        sz = sz + fz[j];
-                    }
-                });
-                if (valid) {
-                    const uint64_t i = first + (entry[k] >> 8);
-                    fx[i] = sx;
-                    fy[i] = sy;
-                    fz[i] = sz;
-                }
-            }
+    const uint32_t before = earlier_tiles(sh.tile_count, w);
+    for (uint32_t rest = present; rest; rest &= rest - 1u) {
+        const uint32_t s = static_cast<uint32_t>(__builtin_ctz(rest));
+        const uint64_t bl = __ballot((slots >> s) & 1u);
+        if ((slots >> s) & 1u) {
+            const uint32_t r = static_cast<uint32_t>(__builtin_amdgcn_readlane(u.runs.start, s)) +
+                               static_cast<uint32_t>(__builtin_amdgcn_readlane(before, s)) + lanes_below(bl);
+            ax = ax + sh.gx[r];
+            ay = ay + sh.gy[r];
+            az = az + sh.gz[r];
         }
+    }
+    if (live) {
+        fx[i] = ax;
+        fy[i] = ay;
+        fz[i] = az;
     }
 }
 
@@ -1270,9 +1299,12 @@ hipError_t launch_reverse_forces(double* fx, double* fy, double* fz, const Layou
 {
     const Layout O = ordered(L, s, kOrderOwners);
     if (L.tiles == 0) return hipSuccess;
-    if (chunk_reverse(L, s))
-        return launch(reverse_chunks_kernel, dim3((L.chunks + L.unit_chunks - 1) / L.unit_chunks),
-                      dim3(kRunWaves * kWave), stream, fx, fy, fz, L.desc, O, B);
+    if (chunk_reverse(L, s)) {
+        Layout R = O;
+        R.unit_chunks = L.reverse_unit_chunks;
+        return launch(reverse_chunks_kernel, dim3((R.chunks + R.unit_chunks - 1) / R.unit_chunks),
+                      dim3(kRunWaves * kWave), stream, fx, fy, fz, L.desc, R, B);
+    }
     if (B.unit_rows && (s.reverse_runs < 0 ? ghost_runs(L, B) : s.reverse_runs > 0))
         return launch(reverse_runs_kernel, run_grid(L), dim3(kRunWaves * kWave), stream, fx, fy, fz, L.desc,
                       L.unit_replica, O, B);

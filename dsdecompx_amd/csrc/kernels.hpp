@@ -17,7 +17,7 @@ hipError_t launch_route(const Layout& L, const PlanBuffers& B, const Shape& s, h
 // The build's copy over chunk units leaves the owner maps and the ghost list
 // unwritten; reverse_halo then needs no route pass when it takes the same
 // units.
-inline bool chunk_reverse(const Layout& L, const Shape& s) { return L.unit_chunks != 0 && s.reverse_runs != 0; }
+inline bool chunk_reverse(const Layout& L, const Shape& s) { return L.reverse_unit_chunks != 0 && s.reverse_runs != 0; }
 hipError_t launch_copy(const dsdx_particles& p, const FieldParams& fp, const Layout& L,
                        const PlanBuffers& B, const Shape& s, hipStream_t stream);
 hipError_t launch_reverse_forces(double* fx, double* fy, double* fz, const Layout& L,

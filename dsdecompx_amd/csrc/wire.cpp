@@ -117,6 +117,7 @@ int dsdx_halo_build_records(dsdx_plan* plan, const dsdx_box* box, const dsdx_par
         DSDX_HIP(launch_route(plan->layout(), plan->buffers(), plan->shape, s), "launch halo route");
     }
     plan->routed = true;
+    plan->rows_built = false;
     plan->host_totals[kStatusWord] = 0;
     plan->built = true;
     *records = g;
