@@ -484,9 +484,11 @@ def measure_batched(args):
 
 # Library kernel name -> device kernels (as rocprofv3 names them; Decomp::migrate
 # launches wrap_box_kernel, the C ABI's per-array form migrate_wrap_kernel; the
-# ghost copy and reverse of a box larger than the memory-side cache are
-# copy_runs_kernel and reverse_runs_kernel; a count column of up to 8 tiles
-# is scanned by scan_parts_kernel, a longer one by scan_kernel).
+# ghost copy and reverse of one box larger than the memory-side cache are
+# copy_chunks_kernel and reverse_chunks_kernel, and its build runs no route
+# pass (route_kernel runs in batches, grids and smaller boxes); a count column
+# of up to 8 tiles is scanned by scan_parts_kernel, a longer one by
+# scan_kernel).
 DEVICE_KERNELS = {"migrate_wrap": ("wrap_box_kernel", "migrate_wrap_kernel"),
                   "halo_classify": ("classify_kernel",), "halo_scan": ("scan_parts_kernel", "scan_kernel"),
                   "halo_scatter": ("route_kernel",), "halo_copy": ("copy_chunks_kernel", "copy_runs_kernel", "copy_kernel"),
