@@ -141,7 +141,13 @@ int dsdx_migrate_wrap(const dsdx_box* box, double* x, double* y, double* z,
  * `ghosts` is null nothing waits; an overflow is recorded in the plan and
  * reported by dsdx_plan_query.  A build captured into a graph must pass a
  * null `ghosts`: waiting for the counts of a captured build (here, or in
- * dsdx_plan_query with a stream) returns DSDX_ESTATE. */
+ * dsdx_plan_query with a stream) returns DSDX_ESTATE.
+ *
+ * A box whose particle records exceed the device's 256 MB memory-side cache
+ * is built without writing the owner map and ghost list: the ghosts and
+ * dsdx_reverse_halo need neither.  The first later call that walks them
+ * (dsdx_halo_refresh, the field exchanges, dsdx_plan_ghost_owners, the record
+ * calls) writes them first, with one more kernel on its stream. */
 int dsdx_halo_build(dsdx_plan* plan, const dsdx_box* box,
                     const dsdx_particles* particles, const dsdx_field* fields,
                     uint32_t field_count, uint64_t owned, uint64_t capacity,
