@@ -48,27 +48,17 @@ __device__ __forceinline__ uint32_t chunk_units_of(const Layout& L)
     return (L.chunks + L.unit_chunks - 1) / L.unit_chunks;
 }
 
-// Every lane of the wave must be active.  rows: the chunks' prefix rows
-// (PlanBuffers::chunk_rows, once the copy has stored them), else the prefix
-// columns are read.
-__device__ __forceinline__ ChunkUnit chunk_unit(const ReplicaDesc* d, uint32_t unit, const Layout& L,
-                                                const PlanBuffers& B, const uint32_t* rows = nullptr)
+// The unit of chunks [c0, c1) from lane l's prefix of column l at c0 and at
+// c1 (the column total when c1 is the last chunk) and the replica's totals
+// row.  Every lane of the wave must be active.
+__device__ __forceinline__ ChunkUnit chunk_unit_of(const ReplicaDesc* d, uint32_t c0, uint32_t c1, uint32_t pre0,
+                                                   uint32_t pre1, uint32_t total_row)
 {
     const uint32_t lane = lane_id();
     ChunkUnit u;
-    u.c0 = unit * L.unit_chunks;
-    u.c1 = min(L.chunks, u.c0 + L.unit_chunks);
-    const uint64_t col = lane < kCols ? lane : 0u;
-    const uint32_t total_row = B.totals[lane & (kTotals - 1)];
-    // past the last chunk the prefix is the column total
-    uint32_t pre1;
-    if (rows) {
-        u.pre0 = rows[static_cast<uint64_t>(u.c0) * kTotals + col];
-        pre1 = u.c1 < L.chunks ? rows[static_cast<uint64_t>(u.c1) * kTotals + col] : total_row;
-    } else {
-        u.pre0 = B.prefix[col * L.chunks + u.c0];
-        pre1 = u.c1 < L.chunks ? B.prefix[col * L.chunks + u.c1] : total_row;
-    }
+    u.c0 = c0;
+    u.c1 = c1;
+    u.pre0 = pre0;
     const uint32_t received = static_cast<uint32_t>(__builtin_amdgcn_readlane(total_row, kColReceived));
     const uint32_t images = static_cast<uint32_t>(__builtin_amdgcn_readlane(total_row, kColImages));
     u.overflow = 0;
@@ -86,6 +76,30 @@ __device__ __forceinline__ ChunkUnit chunk_unit(const ReplicaDesc* d, uint32_t u
     u.runs = unit_runs(u.slot_first + u.pre0, u.slot_first + pre1);
     return u;
 }
+
+// Unit `unit` of the layout.  rows: the chunks' prefix rows
+// (PlanBuffers::chunk_rows, once the copy has stored them), else the prefix
+// columns are read.  Every lane of the wave must be active.
+__device__ __forceinline__ ChunkUnit chunk_unit(const ReplicaDesc* d, uint32_t unit, const Layout& L,
+                                                const PlanBuffers& B, const uint32_t* rows = nullptr)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t c0 = unit * L.unit_chunks;
+    const uint32_t c1 = min(L.chunks, c0 + L.unit_chunks);
+    const uint64_t col = lane < kCols ? lane : 0u;
+    const uint32_t total_row = B.totals[lane & (kTotals - 1)];
+    // past the last chunk the prefix is the column total
+    uint32_t pre0, pre1;
+    if (rows) {
+        pre0 = rows[static_cast<uint64_t>(c0) * kTotals + col];
+        pre1 = c1 < L.chunks ? rows[static_cast<uint64_t>(c1) * kTotals + col] : total_row;
+    } else {
+        pre0 = B.prefix[col * L.chunks + c0];
+        pre1 = c1 < L.chunks ? B.prefix[col * L.chunks + c1] : total_row;
+    }
+    return chunk_unit_of(d, c0, c1, pre0, pre1, total_row);
+}
+
 
 // A wave takes its chunk's owners two rounds of 64 at a time, every load of
 // both rounds issued before the first is used: at 16 M particles a chunk

@@ -674,65 +674,87 @@ __global__ __launch_bounds__(kRunWaves * kWave) void copy_chunks_kernel(dsdx_par
         const uint32_t gbase = u.slot_first + h.pc;              // lane s: its ghost-list value
         uint32_t run = 0;
         for (uint32_t e0 = 0; e0 < h.n; e0 += kChunkRounds * kWave) {
-            // every load of the rounds first, then each round's ranks and stores
             uint32_t entry[kChunkRounds];
-            double x[kChunkRounds], y[kChunkRounds], z[kChunkRounds], vx[kChunkRounds], vy[kChunkRounds],
-                vz[kChunkRounds], m[kChunkRounds], q[kChunkRounds];
-            int32_t ty[kChunkRounds], id[kChunkRounds];
 #pragma unroll
-            for (uint32_t k = 0; k < kChunkRounds; ++k) {
-                entry[k] = chunk_entry(h, e0, k, ebase, B);
-                x[k] = y[k] = z[k] = vx[k] = vy[k] = vz[k] = m[k] = q[k] = 0.0;
-                ty[k] = id[k] = 0;
-                const uint32_t e = e0 + k * kWave + lane;
-                if (e < h.n) {
+            for (uint32_t k = 0; k < kChunkRounds; ++k) entry[k] = chunk_entry(h, e0, k, ebase, B);
+            if (staged) {
+                // every record of the rounds loaded first, straight into LDS,
+                // then each round's ranks
+#pragma unroll
+                for (uint32_t k = 0; k < kChunkRounds; ++k) {
+                    const uint32_t e = e0 + k * kWave + lane;
+                    if (e < h.n) {
+                        const uint64_t i = first + (entry[k] >> 8);
+                        const uint32_t ol = ol0 + e;
+                        sh.px[ol] = B.bpos[ebase + e];
+                        sh.py[ol] = B.bpos[B.bpos_stride + ebase + e];
+                        sh.pz[ol] = B.bpos[2 * B.bpos_stride + ebase + e];
+                        sh.vx[ol] = read_once<STREAM>(p.vx + i);
+                        sh.vy[ol] = read_once<STREAM>(p.vy + i);
+                        sh.vz[ol] = read_once<STREAM>(p.vz + i);
+                        sh.m[ol] = read_once<STREAM>(p.mass + i);
+                        sh.q[ol] = read_once<STREAM>(p.charge + i);
+                        sh.ty[ol] = read_once<STREAM>(p.type + i);
+                        sh.id[ol] = read_once<STREAM>(p.global_id + i);
+                    }
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kChunkRounds; ++k) {
+                    if (e0 + k * kWave >= h.n) break;   // wave-uniform
+                    const uint32_t e = e0 + k * kWave + lane;
+                    const uint32_t code_slots = lane_value(lut, entry[k] & 63u);
+                    const uint32_t slots = e < h.n ? code_slots : 0u;
                     const uint64_t i = first + (entry[k] >> 8);
-                    x[k] = B.bpos[ebase + e];
-                    y[k] = B.bpos[B.bpos_stride + ebase + e];
-                    z[k] = B.bpos[2 * B.bpos_stride + ebase + e];
-                    vx[k] = read_once<STREAM>(p.vx + i);
-                    vy[k] = read_once<STREAM>(p.vy + i);
-                    vz[k] = read_once<STREAM>(p.vz + i);
-                    m[k] = read_once<STREAM>(p.mass + i);
-                    q[k] = read_once<STREAM>(p.charge + i);
-                    ty[k] = read_once<STREAM>(p.type + i);
-                    id[k] = read_once<STREAM>(p.global_id + i);
-                }
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < kChunkRounds; ++k) {
-                if (e0 + k * kWave >= h.n) break;   // wave-uniform
-                const uint32_t e = e0 + k * kWave + lane;
-                const bool valid = e < h.n;
-                const uint32_t code_slots = lane_value(lut, entry[k] & 63u);
-                const uint32_t slots = valid ? code_slots : 0u;
-                const uint64_t i = first + (entry[k] >> 8);
-                const uint32_t ol = ol0 + e;
-                if (staged && valid) {
-                    sh.px[ol] = x[k]; sh.py[ol] = y[k]; sh.pz[ol] = z[k];
-                    sh.vx[ol] = vx[k]; sh.vy[ol] = vy[k]; sh.vz[ol] = vz[k];
-                    sh.m[ol] = m[k]; sh.q[ol] = q[k]; sh.ty[ol] = ty[k]; sh.id[ol] = id[k];
-                }
-                rank_round(slots, run, [&](uint32_t s, uint32_t r, bool mine) {
-                    const uint32_t rb = static_cast<uint32_t>(__builtin_amdgcn_readlane(rbase, s));
-                    const uint32_t g = static_cast<uint32_t>(__builtin_amdgcn_readlane(gbase, s));
-                    if (!mine) return;
-                    const uint64_t j = ghost0 + (g + r);
-                    if (staged) {
+                    const uint32_t ol = ol0 + e;
+                    rank_round(slots, run, [&](uint32_t s, uint32_t r, bool mine) {
+                        const uint32_t rb = static_cast<uint32_t>(__builtin_amdgcn_readlane(rbase, s));
+                        const uint32_t g = static_cast<uint32_t>(__builtin_amdgcn_readlane(gbase, s));
+                        if (!mine) return;
                         sh.map[rb + r] = static_cast<uint16_t>(ol);
-                    } else {
-                        p.x[j] = x[k] + d->bp.shift[s][0];
-                        p.y[j] = y[k] + d->bp.shift[s][1];
-                        p.z[j] = z[k] + d->bp.shift[s][2];
+                        if (fp.count) copy_field_words(fp, i, ghost0 + (g + r));
+                    });
+                }
+            } else {
+                // a unit larger than the stage: one round at a time, each
+                // owner writing its images from registers
+#pragma unroll
+                for (uint32_t k = 0; k < kChunkRounds; ++k) {
+                    if (e0 + k * kWave >= h.n) break;   // wave-uniform
+                    const uint32_t e = e0 + k * kWave + lane;
+                    const bool valid = e < h.n;
+                    const uint32_t code_slots = lane_value(lut, entry[k] & 63u);
+                    const uint32_t slots = valid ? code_slots : 0u;
+                    const uint64_t i = first + (entry[k] >> 8);
+                    double x = 0.0, y = 0.0, z = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, m = 0.0, q = 0.0;
+                    int32_t ty = 0, id = 0;
+                    if (valid) {
+                        x = B.bpos[ebase + e];
+                        y = B.bpos[B.bpos_stride + ebase + e];
+                        z = B.bpos[2 * B.bpos_stride + ebase + e];
+                        vx = read_once<STREAM>(p.vx + i);
+                        vy = read_once<STREAM>(p.vy + i);
+                        vz = read_once<STREAM>(p.vz + i);
+                        m = read_once<STREAM>(p.mass + i);
+                        q = read_once<STREAM>(p.charge + i);
+                        ty = read_once<STREAM>(p.type + i);
+                        id = read_once<STREAM>(p.global_id + i);
+                    }
+                    rank_round(slots, run, [&](uint32_t s, uint32_t r, bool mine) {
+                        const uint32_t g = static_cast<uint32_t>(__builtin_amdgcn_readlane(gbase, s));
+                        if (!mine) return;
+                        const uint64_t j = ghost0 + (g + r);
+                        p.x[j] = x + d->bp.shift[s][0];
+                        p.y[j] = y + d->bp.shift[s][1];
+                        p.z[j] = z + d->bp.shift[s][2];
                         if (p.fx) p.fx[j] = 0.0;
                         if (p.fy) p.fy[j] = 0.0;
                         if (p.fz) p.fz[j] = 0.0;
-                        p.vx[j] = vx[k]; p.vy[j] = vy[k]; p.vz[j] = vz[k];
-                        p.mass[j] = m[k]; p.charge[j] = q[k];
-                        p.type[j] = ty[k]; p.global_id[j] = id[k];
-                    }
-                    if (fp.count) copy_field_words(fp, i, j);
-                });
+                        p.vx[j] = vx; p.vy[j] = vy; p.vz[j] = vz;
+                        p.mass[j] = m; p.charge[j] = q;
+                        p.type[j] = ty; p.global_id[j] = id;
+                        if (fp.count) copy_field_words(fp, i, j);
+                    });
+                }
             }
         }
     }
