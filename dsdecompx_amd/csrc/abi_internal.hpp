@@ -150,7 +150,12 @@ struct Plan {
         if (single_box && !grid && desc.size() == 1 && shape.unit_owners > 0 && L.ghost_runs && unit_rows.ptr &&
             chunk_rows.ptr && chunks) {
             const uint32_t target = static_cast<uint32_t>(shape.unit_owners);
-            L.unit_chunks = unit_chunks(target);
+            // The copy's wave w walks the unit's chunks w, w + kRunWaves, ...
+            // and the block waits for its slowest wave, so from kRunWaves
+            // chunks on a unit holds a multiple of kRunWaves (16 M particles:
+            // units of 9 chunks 131.6 us, of 8 chunks 128.7 us).
+            const uint32_t g = unit_chunks(target);
+            L.unit_chunks = g >= static_cast<uint32_t>(kRunWaves) ? g - g % kRunWaves : g;
             L.reverse_unit_chunks = unit_chunks(std::min(target, kReverseUnitOwners));
         }
         return L;
