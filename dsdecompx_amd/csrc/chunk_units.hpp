@@ -3,19 +3,21 @@
 // scan's prefixes, so that a rebuild needs no route pass
 // (halo_kernels.hip: copy_chunks_kernel, reverse_chunks_kernel).
 //
-// A unit is Layout::unit_chunks consecutive chunks, one workgroup of
-// kRunWaves waves; wave w walks the unit's chunks w, w + kRunWaves, ...  A
-// chunk's boundary owners are its compacted entries, in owned order, and the
+// A unit is Layout::unit_chunks consecutive chunks (reverse_unit_chunks for
+// reverse_halo), one workgroup of kRunWaves waves.  A chunk's boundary
+// owners are its compacted entries, in owned order, and the
 // ghost of an owner through send slot s has ghost-list value
 //   slot_first[s] + prefix[s][chunk] + (its rank among the chunk's owners with s),
 // slot_first[s] being the first ghost of the receive block slot s feeds, the
 // numbering the route pass writes into the ghost list.  So the unit's images
 // through slot s are one run of consecutive ghosts, from slot_first[s] +
-// prefix[s][c0] to slot_first[s] + prefix[s][c1], and a wave ranks its
-// chunk's owners per slot with ballots.  The copy stages the unit's owner
-// records in LDS and writes each run with consecutive lanes (as the staged
-// runs of runs.hpp do); reverse_halo reads the unit's ghost forces as dense
-// runs into LDS and adds each owner's images from there in ascending send
+// prefix[s][c0] to slot_first[s] + prefix[s][c1].  In the copy wave w walks
+// the unit's chunks w, w + kRunWaves, ..., ranks each chunk's owners per slot
+// with ballots, stages their records in LDS and maps each image to its run
+// index; the unit's runs are then written with consecutive lanes (as the
+// staged runs of runs.hpp are).  reverse_halo reads the unit's ghost forces
+// as dense runs into LDS, copies the unit's entries into LDS in owner order,
+// and wave t adds the images of owners 64 t .. 64 t + 63 in ascending send
 // slot.  A unit with more owners or images than LDS holds is copied, or
 // reduced, one lane per owner.
 #pragma once
@@ -101,10 +103,10 @@ __device__ __forceinline__ ChunkUnit chunk_unit(const ReplicaDesc* d, uint32_t u
 }
 
 
-// A wave takes its chunk's owners two rounds of 64 at a time, every load of
-// both rounds issued before the first is used: at 16 M particles a chunk
-// holds about 65 boundary owners, and a second round whose loads waited for
-// the first's made reverse_halo 8 us slower than the tile units.
+// The copy's wave takes its chunk's owners two rounds of 64 at a time, every
+// load of both rounds issued before the first is used: at 16 M particles a
+// chunk holds about 65 boundary owners, and a second round whose loads
+// waited for the first's cost 8 us in a wave-per-chunk reverse_halo.
 constexpr uint32_t kChunkRounds = 2;
 
 // One chunk of the unit as its wave sees it: lane l < kCols holds

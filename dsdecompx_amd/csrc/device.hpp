@@ -122,10 +122,10 @@ struct Shape {
     // reverse_halo with the ghost forces read as dense runs (1), per owner
     // (0), or as the ghost writes go (-1: runs wherever the copy stages them).
     int reverse_runs = -1;
-    // One box whose ghost writes take the staged runs: the copy and
-    // reverse_halo walk units of whole chunks holding about this many
-    // boundary owners, and the build runs no route pass (0: units of owner
-    // tiles after a route pass, as batches and grids do).
+    // One box whose ghost writes take the staged runs: the copy walks units
+    // of whole chunks holding about this many boundary owners (reverse_halo
+    // at most kReverseUnitOwners), and the build runs no route pass (0:
+    // units of owner tiles after a route pass, as batches and grids do).
     int unit_owners = 544;
 };
 
