@@ -217,3 +217,29 @@ def test_build_records_equals_oracle_pack(gpu, case):
     halo.exchange_halo(ref, n, ref_f)
     for k in F64 + I32:
         assert_bits_equal(host(getattr(dev, k))[:n + g], host(getattr(ref, k))[:n + g], f"ghosts {k}")
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
+def test_reverse_halo_after_record_build(gpu, case):
+    """dsdx_reverse_halo on a plan built through the record path, whose ghosts
+    were unpacked from its records: that build writes the owner maps (its
+    route pass) and no chunk rows, so reverse_halo takes the owner-map
+    kernels whatever the ghost-write mode, and the sums match the oracle."""
+    torch = gpu
+    from dsdecompx_amd.wire import PARTICLE, Records, layout
+    say("reverse after record build", case[0])
+    ora, op, of, halo, dev, df, n = _setup(torch, case, seed=21)
+    g = ora.build(op, n)
+    rec = Records(halo)
+    send, got = rec.build(dev, n, df, capacity=g + 5)
+    assert got == g
+    rec.unpack(rec.loopback(send, layout(df, PARTICLE)[1]), g, dev, n, df)
+    rng = np.random.default_rng(23)
+    for k in ("fx", "fy", "fz"):
+        v = rng.uniform(-1.0, 1.0, n + g)
+        getattr(dev, k)[:n + g] = torch.from_numpy(v).cuda()
+        getattr(op, k)[:n + g] = v
+    halo.reverse_halo(dev, n)
+    ora.reverse_halo(op, n)
+    for k in ("fx", "fy", "fz"):
+        assert_bits_equal(host(getattr(dev, k))[:n], getattr(op, k)[:n], f"reverse {k}")
