@@ -18,13 +18,19 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_cpp_captured_graphs_bit_exact():
+@pytest.mark.parametrize("runs", ["default", "staged_runs"])
+def test_cpp_captured_graphs_bit_exact(runs):
+    """With staged_runs (DSDX_GHOST_RUNS=1) the single box's captured build
+    takes the chunk units (no route pass) and the batch the tile units."""
     exe = os.path.join(ROOT, "dsdecompx_amd", "bin", "batch_bench")
     if not os.path.exists(exe):
         pytest.fail(f"{exe} is missing: run __graft_entry__.build() first")
-    print("[case] batch_bench 512 replicas check", flush=True)
+    env = dict(os.environ)
+    if runs == "staged_runs":
+        env["DSDX_GHOST_RUNS"] = "1"
+    print(f"[case] batch_bench 512 replicas check, {runs}", flush=True)
     r = subprocess.run([exe, "512", "3", "0", "2029", "check"], stdout=subprocess.PIPE,
-                       stderr=subprocess.PIPE, text=True, timeout=300)
+                       stderr=subprocess.PIPE, text=True, timeout=300, env=env)
     print(r.stdout, r.stderr[-2000:])
     assert r.returncode == 0, r.stderr[-2000:]
     res = json.loads(r.stdout.strip().splitlines()[-1])
