@@ -493,9 +493,12 @@ int run_scatter(Plan* plan, const dsdx_particles* p, const FieldParams& fp, hipS
 {
     const Layout L = plan->layout();
     const PlanBuffers B = plan->buffers();
-    // the copy over chunk units places every ghost from the entries and
-    // prefixes: the route waits until a call needs the owner maps
-    plan->routed = L.unit_chunks == 0;
+    // The copy over chunk units places every ghost from the entries and
+    // prefixes: the route waits until a call needs the owner maps.  A build
+    // captured into a graph routes at once: each replay rebuilds without the
+    // host seeing it, so only a route in the same graph keeps the owner maps
+    // those of the last replayed build.
+    plan->routed = L.unit_chunks == 0 || plan->counts_captured;
     plan->rows_built = L.unit_chunks != 0;
     if (plan->routed) {
         timing::Scope timed(timing::kScatter, s);

@@ -147,7 +147,9 @@ int dsdx_migrate_wrap(const dsdx_box* box, double* x, double* y, double* z,
  * is built without writing the owner map and ghost list: the ghosts and
  * dsdx_reverse_halo need neither.  The first later call that walks them
  * (dsdx_halo_refresh, the field exchanges, dsdx_plan_ghost_owners, the record
- * calls) writes them first, with one more kernel on its stream. */
+ * calls) writes them first, with one more kernel on its stream.  A build
+ * captured into a graph writes them in the graph, so that every replay
+ * leaves the maps of the build it replayed. */
 int dsdx_halo_build(dsdx_plan* plan, const dsdx_box* box,
                     const dsdx_particles* particles, const dsdx_field* fields,
                     uint32_t field_count, uint64_t owned, uint64_t capacity,

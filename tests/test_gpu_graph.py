@@ -115,3 +115,68 @@ def test_count_wait_behind_a_long_kernel():
     say("waited", round(waited * 1e3, 1), "ms")
     assert waited > 0.05, "the build did not wait behind the spin kernel"
     compare_particles(dev, op, n + g, "build behind a long kernel")
+
+
+def test_replayed_route_free_build_then_owner_maps(monkeypatch):
+    """A single-box build over chunk units captured into a graph (staged
+    runs forced), replayed after the positions moved, then eager calls that
+    walk the owner maps (ghost owners, a full refresh): the graph routes with
+    the build, so the maps are those of the replayed build and every result
+    matches the oracle on the moved positions."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+    from dsdecompx_amd import _abi
+    from dsdecompx_amd.halo import BoxConfig, LoopbackHalo
+    from oracle import pyoracle as po
+    from tests.gpu_util import F64, I32, assert_bits_equal, compare_particles, host, say, synth, to_device, \
+        to_oracle
+
+    monkeypatch.setenv("DSDX_GHOST_RUNS", "1")
+    n, L, w = 30000, 30.0, 2.8
+    say("replayed route-free build", n)
+    d = synth(n, L, 91)
+    dev = to_device(d, 3 * n)
+    halo = LoopbackHalo(BoxConfig(upper=(L,) * 3, width=w))
+    s = torch.cuda.Stream()
+    halo.exchange_halo(dev, n, stream=s)   # eager: sizes the plan
+    s.synchronize()
+    graph = C.c_void_p()
+    _abi.check(_abi.lib().dsdx_graph_begin(C.c_void_p(s.cuda_stream)), "graph begin")
+    try:
+        halo.exchange_halo(dev, n, stream=s, wait=False)
+    finally:
+        rc = _abi.lib().dsdx_graph_end(C.c_void_p(s.cuda_stream), C.byref(graph))
+    _abi.check(rc, "graph end")
+    try:
+        rng = np.random.default_rng(5)
+        for step in range(3):
+            # move the owned particles inside the box, then replay the build
+            for k in ("x", "y", "z"):
+                v = np.clip(d[k] + rng.normal(0.0, 0.3, n), 0.0, np.nextafter(L, 0.0))
+                d[k] = v
+                getattr(dev, k)[:n] = torch.from_numpy(v).cuda()
+            torch.cuda.synchronize()
+            _abi.check(_abi.lib().dsdx_graph_launch(graph, C.c_void_p(s.cuda_stream)), "graph launch")
+            s.synchronize()
+            ora = po.Oracle(po.Box(upper=(L,) * 3, width=w))
+            op = to_oracle(d, 3 * n)
+            g = ora.exchange_halo(op, n)
+            compare_particles(dev, op, n + g, f"replay {step}")
+            with torch.cuda.stream(s):
+                own = host(halo.ghost_owners(g, stream=s))
+            assert_bits_equal(own, ora.ghost_owners(), f"ghost owners after replay {step}")
+            for k in ("x", "vx"):
+                getattr(dev, k)[:n] += 0.001
+                getattr(op, k)[:n] += 0.001
+            torch.cuda.synchronize()
+            halo.refresh(dev, n, stream=s)
+            s.synchronize()
+            ref = to_oracle({k: getattr(op, k)[:n].copy() for k in F64 + I32}, n + g)
+            ora.write_ghosts(ref, n)
+            compare_particles(dev, ref, n + g, f"refresh after replay {step}")
+            for k in ("x", "vx"):
+                d[k] = getattr(op, k)[:n].copy()
+    finally:
+        _abi.check(_abi.lib().dsdx_graph_destroy(graph), "graph destroy")
